@@ -1,0 +1,33 @@
+// bvh_build.hpp — host construction of the 64-byte wide BVH (pt_layout.hpp).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pt_layout.hpp"
+
+namespace fnd {
+
+struct Box {
+    float lo[3];
+    float hi[3];
+};
+
+struct WideTree {
+    std::vector<WideNode> nodes;   // root at 0; indices local to this tree
+    std::vector<uint32_t> order;   // leaf payload order: original primitive indices
+    uint32_t max_depth = 0;        // wide levels on the deepest root-to-leaf chain
+    Box bounds{};
+};
+
+// Binned SAH over primitive boxes (32 bins, leaves of at most `leaf_max`
+// primitives), collapsed to 4-wide nodes and quantised.  `threads` workers
+// build independent subtrees; the result does not depend on `threads`.
+bool build_wide_tree(const Box* boxes, uint32_t n, uint32_t leaf_max, unsigned threads, WideTree& out,
+                     std::string& err);
+
+// Widens a box to contain the 8 mapped corners of `b` under a row-major 3x4
+// map, rounded outward and padded by 2^-10 of the extent.
+Box transform_box_conservative(const Box& b, const float* m34);
+
+}  // namespace fnd

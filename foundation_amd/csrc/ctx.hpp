@@ -1,0 +1,92 @@
+// ctx.hpp — internal state behind the fnd_pt_* C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fnd_pt.h"
+#include "bvh_build.hpp"
+#include "scene_encode.hpp"
+#include "trace_kernel.hpp"
+
+namespace fnd {
+
+enum HandleKind : uint64_t { kHandleScene = 1, kHandleBuffer = 2, kHandleFramebuffer = 3 };
+
+inline fnd_handle make_handle(HandleKind k, uint64_t id) { return (uint64_t(k) << 56) | id; }
+inline HandleKind handle_kind(fnd_handle h) { return HandleKind(h >> 56); }
+
+struct Scene {
+    WideNode* nodes = nullptr;
+    float4* tris = nullptr;
+    InstanceRec* insts = nullptr;
+    DeviceMaterial* materials = nullptr;
+    DeviceLight* lights = nullptr;
+    uint32_t num_lights = 0;
+    uint32_t num_materials = 0;
+    float background[3] = {0, 0, 0};
+    uint32_t root = 0;
+    bool two_level = false;
+    uint64_t sz_nodes = 0, sz_tris = 0, sz_insts = 0, sz_materials = 0, sz_lights = 0;
+    fnd_scene_info info{};
+};
+
+struct Buffer {
+    void* ptr = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Framebuffer {
+    uint32_t width = 0, height = 0, flags = 0;
+    float* accum = nullptr;       // W*H*3 running sums (FP32)
+    double* accum64 = nullptr;    // W*H*3 running sums (FP64, oracle images only)
+    uint32_t samples = 0;
+};
+
+}  // namespace fnd
+
+struct fnd_pt_ctx {
+    fnd_host_allocator alloc{};
+    bool has_alloc = false;
+    fnd_log_fn log = nullptr;
+    void* log_user = nullptr;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::string last_error;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    uint32_t num_cus = 0;
+    std::string arch;
+    fnd_tuning tuning{};
+    fnd_pt_stats stats{};
+    uint64_t next_id = 1;
+    std::map<uint64_t, fnd::Scene*> scenes;
+    std::map<uint64_t, fnd::Buffer*> buffers;
+    std::map<uint64_t, fnd::Framebuffer*> framebuffers;
+    // scratch owned by the context
+    uint32_t* d_counters = nullptr;            // 64 work counters
+    unsigned long long* d_stats = nullptr;     // 8 counters
+    uint2* d_spill = nullptr;
+    uint64_t spill_bytes = 0;
+    void* d_tmp = nullptr;                     // host-array trace staging
+    uint64_t tmp_bytes = 0;
+    void* render_state = nullptr;              // wavefront buffers (render.cpp)
+};
+
+namespace fnd {
+
+fnd_status set_error(fnd_pt_ctx* ctx, fnd_status st, const char* fmt, ...);
+fnd_status check_hip(fnd_pt_ctx* ctx, hipError_t e, const char* what);
+fnd_status dev_alloc(fnd_pt_ctx* ctx, void** ptr, uint64_t bytes);
+void dev_free(fnd_pt_ctx* ctx, void* ptr, uint64_t bytes);
+fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_lds, uint32_t threads);
+Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
+Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
+uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block);
+void render_state_free(fnd_pt_ctx* ctx);
+
+}  // namespace fnd

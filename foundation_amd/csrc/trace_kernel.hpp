@@ -1,0 +1,291 @@
+// trace_kernel.hpp — the persistent traversal kernel (closest-hit / any-hit).
+//
+// Design for CDNA4 (wave64, 160 KB LDS per CU, no ray-tracing units):
+//  * One ray per lane.  A wave keeps its 64 lanes busy by refetching rays:
+//    once `refill` lanes are idle, one lane takes a block of ray indices with a
+//    single atomicAdd and the idle lanes take theirs by popcount of the ballot
+//    below them (Aila & Laine, "Understanding the Efficiency of Ray Traversal on
+//    GPUs", HPG 2009, dynamic fetch; here done with wave ballots).
+//  * The traversal stack is per lane in LDS, laid out [entry][thread] so any
+//    mix of stack depths across a wave hits distinct banks; entries past the
+//    LDS part go to a per-thread spill area in HBM sized from the tree depth.
+//    An entry is (word, entry distance); on pop, entries farther than the best
+//    hit so far are dropped.
+//  * Each loop iteration does one unit of work per lane: one 64-byte node
+//    (4 quantised slab tests, sort, push), one triangle, or an instance
+//    transition.  Leaf children are pushed as (first, count) triangle ranges.
+//  * Two-level scenes: an instance entry moves the ray into object space and
+//    pushes a return marker; popping the marker restores the world ray.
+#pragma once
+#include "pt_device.hpp"
+
+namespace fnd {
+
+struct SceneDev {
+    const WideNode* nodes;
+    const float4* tris;
+    const InstanceRec* insts;
+    uint32_t root;
+    uint32_t two_level;
+};
+
+enum TraceMode : int {
+    kModeClosest = 0,          // fnd_hit records (global primitive ids)
+    kModeAny = 1,              // fnd_hit records, first accepted hit
+    kModeClosestInternal = 2,  // (t, leaf triangle index, instance slot) for shading
+    kModeAnyInternal = 3       // visibility for shadow rays
+};
+
+struct TraceArgs {
+    SceneDev scene;            // used when the kernel is not per-ray scene
+    const SceneDev* scenes;    // per-ray scene table
+    const uint32_t* ray_scene; // per-ray scene index
+    const Ray* rays;
+    float4* hits;
+    uint32_t nrays;
+    const uint32_t* nrays_dev; // if set, the ray count is read here (wavefront queues)
+    uint32_t* counter;         // work counter, zero at launch
+    uint2* spill;
+    uint32_t spill_stride;     // threads in the grid
+    uint32_t refill;
+    unsigned long long* stats; // node visits, triangle tests, instance entries (COUNT)
+};
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+template <int MODE, bool TWO, bool MULTI, bool COUNT, int SLDS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
+{
+    constexpr bool ANY = (MODE & 1) != 0;
+    constexpr bool INTERNAL = MODE >= kModeClosestInternal;
+    __shared__ uint2 stk[SLDS * BLOCK];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gtid = blockIdx.x * BLOCK + tid;
+    const uint32_t nrays = args.nrays_dev ? *args.nrays_dev : args.nrays;
+
+    SceneDev sc = args.scene;
+    bool active = false, retired = false;
+    uint32_t ray_idx = 0;
+    RayK r;
+    float wox = 0.f, woy = 0.f, woz = 0.f, wdx = 0.f, wdy = 0.f, wdz = 0.f;
+    float tmin = 0.f, tmax = 0.f, tbest = 0.f, bu = 0.f, bv = 0.f;
+    uint32_t best = 0xFFFFFFFFu, best_aux = 0u;
+    uint32_t cur = 0, sp = 0, id_base = 0, inst_slot = 0xFFFFFFFFu;
+    uint32_t c_nodes = 0, c_tris = 0, c_inst = 0;
+
+    auto push = [&](uint32_t e, float t) {
+        const uint2 v = make_uint2(e, __float_as_uint(t));
+        if (sp < SLDS)
+            stk[sp * BLOCK + tid] = v;
+        else
+            args.spill[size_t(sp - SLDS) * args.spill_stride + gtid] = v;
+        ++sp;
+    };
+
+    for (;;) {
+        const uint64_t idle = __ballot(!active && !retired);
+        const uint64_t act = __ballot(active);
+        if (idle != 0 && (act == 0 || uint32_t(__popcll(idle)) >= args.refill)) {
+            const uint32_t nidle = uint32_t(__popcll(idle));
+            const int leader = __builtin_ctzll(idle);
+            uint32_t base = 0;
+            if (int(__lane_id()) == leader) base = atomicAdd(args.counter, nidle);
+            base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+            if (!active && !retired) {
+                const uint32_t idx = base + lanes_below(idle);
+                if (idx < nrays) {
+                    ray_idx = idx;
+                    const float4* rp = reinterpret_cast<const float4*>(args.rays + idx);
+                    const float4 ra = rp[0];
+                    const float4 rb = rp[1];
+                    if (MULTI) sc = args.scenes[args.ray_scene[idx]];
+                    wox = ra.x; woy = ra.y; woz = ra.z; tmin = ra.w;
+                    wdx = rb.x; wdy = rb.y; wdz = rb.z; tmax = rb.w;
+                    ray_setup(r, wox, woy, woz, wdx, wdy, wdz);
+                    tbest = tmax;
+                    best = 0xFFFFFFFFu;
+                    best_aux = 0xFFFFFFFFu;
+                    bu = bv = 0.f;
+                    sp = 0;
+                    cur = sc.root;
+                    id_base = 0;
+                    inst_slot = 0xFFFFFFFFu;
+                    active = true;
+                } else {
+                    retired = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+
+        bool pop = false, finish = false;
+        if (cur < kEntryInstance) {
+            // ---- one 64-byte wide node: four quantised slab tests
+            if (COUNT) ++c_nodes;
+            const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + cur);
+            const uint4 n0 = np[0];
+            const uint4 n1 = np[1];
+            const uint4 n2 = np[2];
+            const uint4 n3 = np[3];
+            const int ex = int(int8_t(n0.w & 0xffu));
+            const int ey = int(int8_t((n0.w >> 8) & 0xffu));
+            const int ez = int(int8_t((n0.w >> 16) & 0xffu));
+            const uint32_t imask = n0.w >> 24;
+            const uint32_t child_base = n1.x, prim_base = n1.y, meta = n1.z, flags = n1.w;
+            const float sx = exp2_int(ex) * r.ix;
+            const float sy = exp2_int(ey) * r.iy;
+            const float sz = exp2_int(ez) * r.iz;
+            const float cx = (__uint_as_float(n0.x) - r.ox) * r.ix;
+            const float cy = (__uint_as_float(n0.y) - r.oy) * r.iy;
+            const float cz = (__uint_as_float(n0.z) - r.oz) * r.iz;
+            // qlo = (n2.x, n2.y, n2.z), qhi = (n2.w, n3.x, n3.y)
+            const bool px_ = r.ix >= 0.f, py_ = r.iy >= 0.f, pz_ = r.iz >= 0.f;
+            const uint32_t nx = px_ ? n2.x : n2.w, fx = px_ ? n2.w : n2.x;
+            const uint32_t ny = py_ ? n2.y : n3.x, fy = py_ ? n3.x : n2.y;
+            const uint32_t nz = pz_ ? n2.z : n3.y, fz = pz_ ? n3.y : n2.z;
+            const bool top = TWO && (flags & kNodeFlagTop) != 0u;
+            float tk[4];
+            uint32_t ek[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float tn = fmaxf(fmaxf(fmaf(ubyte_f(nx, k), sx, cx), fmaf(ubyte_f(ny, k), sy, cy)),
+                                       fmaxf(fmaf(ubyte_f(nz, k), sz, cz), tmin));
+                const float tf = fminf(fminf(fmaf(ubyte_f(fx, k), sx, cx), fmaf(ubyte_f(fy, k), sy, cy)),
+                                       fminf(fmaf(ubyte_f(fz, k), sz, cz), tbest));
+                const uint32_t m = (meta >> (8 * k)) & 0xffu;
+                const bool internal = ((imask >> k) & 1u) != 0u;
+                const bool hit = (internal || m != 0u) && tn <= tf * kSlack;
+                uint32_t e;
+                if (internal)
+                    e = child_base + uint32_t(__popc(imask & ((1u << k) - 1u)));
+                else if (top)
+                    e = kEntryInstance | (prim_base + (m >> 3));
+                else
+                    e = kEntryTri | ((prim_base + (m >> 3)) << 3) | (m & 7u);
+                tk[k] = hit ? tn : __builtin_inff();
+                ek[k] = e;
+            }
+            if (!ANY) {
+                // sorting network (0,1)(2,3)(0,2)(1,3)(1,2): nearest child first
+                auto cs = [&](int a, int b) {
+                    const bool s = tk[b] < tk[a];
+                    const float ta = s ? tk[b] : tk[a], tb = s ? tk[a] : tk[b];
+                    const uint32_t ea = s ? ek[b] : ek[a], eb = s ? ek[a] : ek[b];
+                    tk[a] = ta; tk[b] = tb; ek[a] = ea; ek[b] = eb;
+                };
+                cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+                if (tk[3] != __builtin_inff()) push(ek[3], tk[3]);
+                if (tk[2] != __builtin_inff()) push(ek[2], tk[2]);
+                if (tk[1] != __builtin_inff()) push(ek[1], tk[1]);
+                if (tk[0] != __builtin_inff()) cur = ek[0]; else pop = true;
+            } else {
+                bool have = false;
+#pragma unroll
+                for (int k = 3; k >= 0; --k) {
+                    if (tk[k] != __builtin_inff()) {
+                        if (have) push(cur, 0.f);
+                        cur = ek[k];
+                        have = true;
+                    }
+                }
+                if (!have) pop = true;
+            }
+        } else if (cur & kEntryTri) {
+            // ---- one triangle of a leaf range
+            if (COUNT) ++c_tris;
+            const uint32_t ti = (cur >> 3) & 0x0FFFFFFFu;
+            const float4* tp = sc.tris + 3ull * ti;
+            const float4 a = tp[0];
+            const float4 b = tp[1];
+            const float4 c = tp[2];
+            float t, u, v;
+            if (isect_tri(r, a, b, c, t, u, v) && t > tmin && t < tmax) {
+                const uint32_t id = INTERNAL ? ti : id_base + __float_as_uint(a.w);
+                if (ANY) {
+                    tbest = t; best = id; bu = u; bv = v; best_aux = inst_slot;
+                    finish = true;
+                } else if (t < tbest || (t == tbest && id < best)) {
+                    tbest = t; best = id; bu = u; bv = v; best_aux = inst_slot;
+                }
+            }
+            if (!finish) {
+                if ((cur & 7u) > 1u)
+                    cur += 7u;  // first + 1, count - 1
+                else
+                    pop = true;
+            }
+        } else if (TWO && cur == kEntryReturn) {
+            ray_setup(r, wox, woy, woz, wdx, wdy, wdz);
+            id_base = 0;
+            inst_slot = 0xFFFFFFFFu;
+            pop = true;
+        } else if (TWO) {
+            // ---- enter an instance: object-space ray, return marker
+            if (COUNT) ++c_inst;
+            const uint32_t ii = cur & 0x3FFFFFFFu;
+            const float4* ip = reinterpret_cast<const float4*>(sc.insts + ii);
+            const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
+            const uint4 m3 = reinterpret_cast<const uint4*>(ip)[3];
+            const float mm[12] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
+            float rox, roy, roz, rdx, rdy, rdz;
+            xform_ray(mm, wox, woy, woz, wdx, wdy, wdz, rox, roy, roz, rdx, rdy, rdz);
+            ray_setup(r, rox, roy, roz, rdx, rdy, rdz);
+            push(kEntryReturn, -__builtin_inff());
+            cur = m3.x;
+            id_base = m3.y;
+            inst_slot = ii;
+        }
+        if (pop) {
+            for (;;) {
+                if (sp == 0) {
+                    finish = true;
+                    break;
+                }
+                --sp;
+                const uint2 e = sp < SLDS ? stk[sp * BLOCK + tid] : args.spill[size_t(sp - SLDS) * args.spill_stride + gtid];
+                if (ANY || __uint_as_float(e.y) <= tbest * kSlack) {
+                    cur = e.x;
+                    break;
+                }
+            }
+        }
+        if (finish) {
+            const bool miss = best == 0xFFFFFFFFu;
+            float4 out;
+            if (INTERNAL) {
+                out = make_float4(miss ? tmax : tbest, __uint_as_float(best), __uint_as_float(miss ? 0xFFFFFFFFu : best_aux),
+                                  0.f);
+            } else {
+                out = make_float4(miss ? tmax : tbest, __uint_as_float(best), miss ? 0.f : bu, miss ? 0.f : bv);
+            }
+            args.hits[ray_idx] = out;
+            active = false;
+        }
+    }
+    if (COUNT) {
+        atomicAdd(args.stats + 0, (unsigned long long)c_nodes);
+        atomicAdd(args.stats + 1, (unsigned long long)c_tris);
+        atomicAdd(args.stats + 2, (unsigned long long)c_inst);
+    }
+}
+
+// Host-side launcher: picks the instantiation matching (block, stack) and mode.
+struct TraceLaunch {
+    int mode;
+    bool two_level;
+    bool multi;
+    bool count;
+    uint32_t block;
+    uint32_t stack_lds;
+    uint32_t grid;
+};
+
+hipError_t launch_trace(const TraceLaunch& l, const TraceArgs& args, hipStream_t stream);
+bool trace_config_supported(uint32_t block, uint32_t stack_lds);
+
+}  // namespace fnd

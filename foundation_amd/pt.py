@@ -1,0 +1,297 @@
+"""Python face of the path-tracing pass: thin wrappers over the fnd_pt_* C ABI.
+
+Mirrors the C++ PathTracerPass wrapper (csrc/PathTracerPass.hpp): a
+``Context`` owns scenes, device buffers and framebuffers like the reference's
+Renderer owns its device objects (src/Renderer/Renderer.hpp:10-44), and every
+non-OK status raises ``FndError`` where the reference would CHECK/BugCheck
+(src/Core/Core.hpp:17).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+
+class FndError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{A.load().fnd_pt_status_string(status).decode()} ({status}): {message}")
+        self.status = status
+        self.message = message
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+# ---------------------------------------------------------------- generated scenes
+
+
+class GeneratedScene:
+    """A procedural scene made by the library's generators (fnd_gen_*)."""
+
+    def __init__(self, kind, **overrides):
+        lib = A.load()
+        p = A.GenParams()
+        st = lib.fnd_gen_default_params(kind, C.byref(p))
+        if st != A.OK:
+            raise FndError(st, f"unknown generator kind {kind}")
+        for k, v in overrides.items():
+            if not hasattr(p, k):
+                raise KeyError(k)
+            setattr(p, k, v)
+        self.params = p
+        h = C.c_void_p()
+        st = lib.fnd_gen_scene(C.byref(p), C.byref(h))
+        if st != A.OK:
+            raise FndError(st, "scene generation failed")
+        self._h = h
+        self.desc = A.SceneDesc()
+        self.camera = A.Camera()
+        lib.fnd_gen_scene_get(h, C.byref(self.desc), C.byref(self.camera))
+
+    def close(self):
+        if self._h:
+            A.load().fnd_gen_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def meshes(self):
+        """Copies of each mesh: positions (V,3) f32, indices (T,3) u32,
+        material (T,) u32, light (T,) i32."""
+        out = []
+        for i in range(self.desc.num_meshes):
+            m = self.desc.meshes[i]
+            pos = np.ctypeslib.as_array(m.positions, shape=(m.num_vertices * 3,)).reshape(-1, 3).copy()
+            idx = np.ctypeslib.as_array(m.indices, shape=(m.num_triangles * 3,)).reshape(-1, 3).copy()
+            mat = (np.ctypeslib.as_array(m.material_ids, shape=(m.num_triangles,)).copy()
+                   if m.material_ids else np.zeros(m.num_triangles, np.uint32))
+            lig = (np.ctypeslib.as_array(m.light_ids, shape=(m.num_triangles,)).copy()
+                   if m.light_ids else np.full(m.num_triangles, -1, np.int32))
+            out.append(dict(positions=pos, indices=idx, material=mat, light=lig))
+        return out
+
+    def instances(self):
+        out = []
+        for i in range(self.desc.num_instances):
+            d = self.desc.instances[i]
+            o2w = np.array(d.object_to_world[:], np.float32).reshape(3, 4)
+            w2o = np.array(d.world_to_object[:], np.float32).reshape(3, 4)
+            out.append(dict(mesh=d.mesh, o2w=o2w, w2o=w2o))
+        return out
+
+    def num_prims(self):
+        if self.desc.num_instances == 0:
+            return self.desc.meshes[0].num_triangles
+        return sum(self.desc.meshes[self.desc.instances[i].mesh].num_triangles
+                   for i in range(self.desc.num_instances))
+
+    def world_triangles(self):
+        """(P, 3, 3) float64 world-space vertices indexed by primitive id."""
+        ms = self.meshes()
+        if self.desc.num_instances == 0:
+            m = ms[0]
+            return m["positions"][m["indices"]].astype(np.float64)
+        parts = []
+        for inst in self.instances():
+            m = ms[inst["mesh"]]
+            v = m["positions"][m["indices"]].astype(np.float64)
+            o2w = inst["o2w"].astype(np.float64)
+            parts.append(v @ o2w[:, :3].T + o2w[:, 3])
+        return np.concatenate(parts, axis=0)
+
+
+# ---------------------------------------------------------------- host encoding
+
+
+WIDE_NODE_DTYPE = np.dtype([
+    ("p", "<f4", 3), ("e", "i1", 3), ("imask", "u1"), ("child_base", "<u4"), ("prim_base", "<u4"),
+    ("meta", "<u4"), ("flags", "<u4"), ("qlo", "<u4", 3), ("qhi", "<u4", 3), ("pad", "<u4", 2)])
+assert WIDE_NODE_DTYPE.itemsize == 64
+LEAF_TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("local_id", "<u4"), ("v1", "<f4", 3), ("material", "<u4"),
+                           ("v2", "<f4", 3), ("light", "<i4")])
+assert LEAF_TRI_DTYPE.itemsize == 48
+INSTANCE_DTYPE = np.dtype([("w2o", "<f4", 12), ("blas_root", "<u4"), ("id_base", "<u4"), ("mesh", "<u4"),
+                           ("orig_index", "<u4")])
+assert INSTANCE_DTYPE.itemsize == 64
+
+
+def encode_scene(desc):
+    """Host encoding of a scene (fnd_scene_encode): dict of numpy arrays with
+    the exact bytes fnd_pt_scene_create uploads."""
+    lib = A.load()
+    h = C.c_void_p()
+    err = C.create_string_buffer(512)
+    st = lib.fnd_scene_encode(C.byref(desc), C.byref(h), err, 512)
+    if st != A.OK:
+        raise FndError(st, err.value.decode())
+    try:
+        v = A.HostSceneView()
+        lib.fnd_host_scene_get(h, C.byref(v))
+
+        def grab(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            buf = (C.c_char * (n * dt.itemsize)).from_address(ptr)
+            return np.frombuffer(bytes(buf), dtype=dt).copy()
+
+        out = dict(nodes=grab(v.nodes, v.num_nodes, WIDE_NODE_DTYPE),
+                   tris=grab(v.tris, v.num_tris, LEAF_TRI_DTYPE),
+                   instances=grab(v.instances, v.num_instances, INSTANCE_DTYPE),
+                   root=v.root, two_level=bool(v.two_level),
+                   max_depth=v.info.max_depth, stack_entries=v.info.stack_entries,
+                   num_prims=v.info.num_prims)
+        return out
+    finally:
+        lib.fnd_host_scene_free(h)
+
+
+# ---------------------------------------------------------------- context
+
+
+class Context:
+    """One fnd_pt_ctx: one device, one stream, the factory of its objects."""
+
+    def __init__(self, device=0, device_memory_budget=0, allocator=None):
+        self.lib = A.load()
+        d = A.CreateDesc()
+        d.device = device
+        d.device_memory_budget = device_memory_budget
+        self._alloc = allocator
+        h = C.c_void_p()
+        st = self.lib.fnd_pt_create(C.byref(d), C.byref(allocator) if allocator else None, C.byref(h))
+        if st != A.OK:
+            raise FndError(st, "fnd_pt_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.fnd_pt_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, st):
+        if st != A.OK:
+            raise FndError(st, self.lib.fnd_pt_last_error(self.h).decode())
+
+    # scenes
+    def scene_create(self, desc):
+        h = A.fnd_handle()
+        self._ck(self.lib.fnd_pt_scene_create(self.h, C.byref(desc), C.byref(h)))
+        return h.value
+
+    def scene_destroy(self, scene):
+        self._ck(self.lib.fnd_pt_scene_destroy(self.h, scene))
+
+    def scene_info(self, scene):
+        info = A.SceneInfo()
+        self._ck(self.lib.fnd_pt_scene_info(self.h, scene, C.byref(info)))
+        return info
+
+    # buffers
+    def buffer_create(self, nbytes):
+        h = A.fnd_handle()
+        self._ck(self.lib.fnd_pt_buffer_create(self.h, nbytes, C.byref(h)))
+        return h.value
+
+    def buffer_destroy(self, buf):
+        self._ck(self.lib.fnd_pt_buffer_destroy(self.h, buf))
+
+    def buffer_write(self, buf, array, offset=0):
+        a = np.ascontiguousarray(array)
+        self._ck(self.lib.fnd_pt_buffer_write(self.h, buf, offset, a.ctypes.data, a.nbytes))
+
+    def buffer_read(self, buf, out, offset=0):
+        assert out.flags["C_CONTIGUOUS"]
+        self._ck(self.lib.fnd_pt_buffer_read(self.h, buf, offset, out.ctypes.data, out.nbytes))
+        return out
+
+    # ray queries
+    def trace(self, scene, rays, anyhit=False, count=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        flags = (A.TRACE_ANYHIT if anyhit else 0) | (A.TRACE_COUNT if count else 0)
+        self._ck(self.lib.fnd_pt_trace(self.h, scene, rays.ctypes.data, rays.shape[0], hits.ctypes.data, flags))
+        return hits
+
+    def trace_buffers(self, scene, ray_buf, hit_buf, n, flags=0):
+        self._ck(self.lib.fnd_pt_trace_buffers(self.h, scene, ray_buf, hit_buf, n, flags))
+
+    def wait(self):
+        self._ck(self.lib.fnd_pt_wait(self.h))
+
+    def stats(self):
+        s = A.Stats()
+        self._ck(self.lib.fnd_pt_get_stats(self.h, C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        self._ck(self.lib.fnd_pt_reset_stats(self.h))
+
+    def tuning(self):
+        t = A.Tuning()
+        self._ck(self.lib.fnd_pt_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, v)
+        self._ck(self.lib.fnd_pt_set_tuning(self.h, C.byref(t)))
+
+    # rendering
+    def framebuffer_create(self, width, height, fp64=False):
+        h = A.fnd_handle()
+        self._ck(self.lib.fnd_pt_framebuffer_create(self.h, width, height, A.FB_FP64 if fp64 else 0, C.byref(h)))
+        return h.value
+
+    def framebuffer_destroy(self, fb):
+        self._ck(self.lib.fnd_pt_framebuffer_destroy(self.h, fb))
+
+    def framebuffer_clear(self, fb):
+        self._ck(self.lib.fnd_pt_framebuffer_clear(self.h, fb))
+
+    def framebuffer_read(self, fb, width, height, fp64=False):
+        out = np.zeros((height, width, 3), np.float64 if fp64 else np.float32)
+        n = C.c_uint32()
+        fn = self.lib.fnd_pt_framebuffer_read_f64 if fp64 else self.lib.fnd_pt_framebuffer_read
+        self._ck(fn(self.h, fb, out.ctypes.data, C.byref(n)))
+        return out, n.value
+
+    def render(self, jobs):
+        arr = (A.RenderJob * len(jobs))(*jobs)
+        self._ck(self.lib.fnd_pt_render(self.h, arr, len(jobs)))
+
+
+def make_job(scene, framebuffer, camera, spp, max_depth, rr_depth=None, seed=1, sample_offset=0, flags=0):
+    j = A.RenderJob()
+    j.scene = scene
+    j.framebuffer = framebuffer
+    j.camera = camera
+    j.spp = spp
+    j.max_depth = max_depth
+    j.rr_depth = max_depth if rr_depth is None else rr_depth
+    j.seed = seed
+    j.sample_offset = sample_offset
+    j.flags = flags
+    return j

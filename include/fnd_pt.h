@@ -1,0 +1,350 @@
+/*
+ * fnd_pt.h — C ABI of the MI355X path-tracing pass.
+ *
+ * This is the drop-in boundary between a Foundation-style renderer and the
+ * HIP kernels that trace rays on a gfx950 GPU.  Plain C: no C++ types, no
+ * exceptions, plain pointers and sizes.  Every call returns an fnd_status and
+ * never aborts; the text of the last failure is kept per context
+ * (fnd_pt_last_error).
+ *
+ * Where each entry point sits against the reference (mos9527/Foundation):
+ *   - fnd_pt_create / fnd_pt_destroy replace the device-side half of
+ *     Renderer::Renderer(RHIApplicationObjectHandle<RHIDevice>, Core::Allocator*)
+ *     and Renderer::~Renderer() (src/Renderer/Renderer.hpp:41-42,
+ *     src/Renderer/Renderer.cpp:34-37,402-406).  The borrowed allocator is the
+ *     C form of the Vulkan allocation callbacks that forward to Core::Allocator
+ *     (src/Platform/RHI/Vulkan/Application.hpp:13-25).
+ *   - fnd_pt_scene_create replaces the vertex/index upload of the constructor
+ *     (src/Renderer/Renderer.cpp:133-198): it copies the caller's arrays and
+ *     builds the acceleration structure on the device.
+ *   - fnd_pt_render replaces the raster pass recorded by Renderer::Record and
+ *     submitted by Renderer::Draw (src/Renderer/Renderer.cpp:312-366,386-393).
+ *   - fnd_pt_wait is the fence wait after submit (src/Renderer/Renderer.cpp:394).
+ *   - Handles are uint64_t with ~0 as the invalid value, the convention of
+ *     src/Platform/RHI/Details/Details.hpp:11-12; the context is the factory that
+ *     owns every object it hands out and destroying it destroys them
+ *     (src/Platform/RHI/Details/Details.hpp:39-46,125-141).
+ *   - fnd_pt_trace / fnd_pt_trace_buffers have no counterpart in the reference
+ *     (it has no compute dispatch, src/Platform/RHI/Command.hpp:32-116); they are
+ *     the ray-query entry used by the traversal benchmark and the parity tests.
+ *
+ * Threading: a context is single-threaded, like the reference's one render
+ * thread (src/Editor/Editor.cpp:20-23).  One context owns one HIP stream.
+ * Calls are synchronous unless FND_ASYNC is set, then fnd_pt_wait is required.
+ */
+#ifndef FND_PT_H
+#define FND_PT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FND_PT_ABI_VERSION 1u
+
+typedef uint64_t fnd_handle;
+#define FND_INVALID_HANDLE (~(uint64_t)0)
+
+typedef enum fnd_status {
+    FND_OK = 0,
+    FND_E_INVALID_ARG = 1,
+    FND_E_OOM_HOST = 2,
+    FND_E_OOM_DEVICE = 3,
+    FND_E_DEVICE = 4,
+    FND_E_QUEUE_OVERFLOW = 5,
+    FND_E_UNSUPPORTED = 6,
+    FND_E_NOT_FOUND = 7
+} fnd_status;
+
+/* Host allocator: the C form of Core::Allocator injection
+ * (src/Core/Allocator/Allocator.hpp:17-41).  NULL means malloc/free. */
+typedef struct fnd_host_allocator {
+    void* user;
+    void* (*alloc)(void* user, size_t size, size_t align);
+    void* (*realloc)(void* user, void* ptr, size_t size, size_t align);
+    void (*free)(void* user, void* ptr);
+} fnd_host_allocator;
+
+typedef void (*fnd_log_fn)(void* user, int level, const char* message);
+
+typedef struct fnd_pt_create_desc {
+    int device;                     /* HIP device ordinal */
+    uint32_t flags;                 /* reserved, 0 */
+    uint64_t device_memory_budget;  /* bytes; 0 = unlimited.  Allocations past it fail with FND_E_OOM_DEVICE */
+    fnd_log_fn log;                 /* optional */
+    void* log_user;
+} fnd_pt_create_desc;
+
+typedef struct fnd_pt_ctx fnd_pt_ctx;
+
+/* ---------------------------------------------------------------- rays */
+
+/* 32 bytes.  A hit is accepted iff tmin < t < tmax.  dir need not be unit. */
+typedef struct fnd_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} fnd_ray;
+
+/* 16 bytes.  prim = FND_NO_HIT on a miss.  For a closest-hit query the
+ * record is the lexicographic minimum of (t, prim) over all accepted hits,
+ * (u, v) the barycentric weights of vertices 1 and 2.  For an any-hit query
+ * prim is some accepted primitive (which one is unspecified). */
+typedef struct fnd_hit {
+    float t;
+    uint32_t prim;
+    float u;
+    float v;
+} fnd_hit;
+#define FND_NO_HIT 0xFFFFFFFFu
+
+/* ---------------------------------------------------------------- scenes */
+
+typedef enum fnd_material_type {
+    FND_MAT_LAMBERT = 0,  /* albedo / pi */
+    FND_MAT_GGX = 1,      /* conductor: GGX D, Smith height-correlated G2, Schlick F with F0 = albedo, alpha = roughness^2 */
+    FND_MAT_EMISSIVE = 2  /* emits `emission`, reflects nothing */
+} fnd_material_type;
+
+typedef struct fnd_material {
+    uint32_t type;        /* fnd_material_type */
+    float albedo[3];
+    float roughness;
+    float emission[3];    /* a Lambert surface may also emit (furnace scenes) */
+} fnd_material;
+
+/* Parallelogram light: points corner + a*edge0 + b*edge1, a,b in [0,1],
+ * one-sided, emitting toward cross(edge0, edge1).  Sampled by next-event
+ * estimation; the caller must also put its two triangles in a mesh with an
+ * FND_MAT_EMISSIVE material and set their fnd_mesh_desc.light_ids to it. */
+typedef struct fnd_quad_light {
+    float corner[3];
+    float edge0[3];
+    float edge1[3];
+    float radiance[3];
+} fnd_quad_light;
+
+/* Indexed triangle mesh, the layout of src/Renderer/Renderer.cpp:23-27 with
+ * 32-bit indices (src/Platform/RHI/Vulkan/Command.cpp:289-305). */
+typedef struct fnd_mesh_desc {
+    const float* positions;        /* 3 floats per vertex */
+    uint32_t num_vertices;
+    const uint32_t* indices;       /* 3 per triangle */
+    uint32_t num_triangles;
+    const uint32_t* material_ids;  /* per triangle, or NULL for material 0 */
+    const int32_t* light_ids;      /* per triangle: index into lights or -1; NULL = all -1 */
+} fnd_mesh_desc;
+
+/* Row-major 3x4 affine maps.  The primitive id of a hit in instance i is
+ * id_base(i) + triangle index in its mesh, id_base = running sum over the
+ * instances before i of their mesh's triangle count. */
+typedef struct fnd_instance_desc {
+    uint32_t mesh;
+    uint32_t flags;                /* reserved, 0 */
+    float object_to_world[12];
+    float world_to_object[12];     /* must be given; used bit-for-bit to move rays into object space */
+} fnd_instance_desc;
+
+typedef struct fnd_scene_desc {
+    const fnd_mesh_desc* meshes;
+    uint32_t num_meshes;
+    const fnd_instance_desc* instances;  /* NULL/0: single level, mesh 0 in world space */
+    uint32_t num_instances;
+    const fnd_material* materials;
+    uint32_t num_materials;
+    const fnd_quad_light* lights;
+    uint32_t num_lights;
+    float background[3];                 /* radiance of rays that leave the scene */
+    uint32_t build_flags;                /* reserved, 0 */
+} fnd_scene_desc;
+
+typedef struct fnd_scene_info {
+    uint64_t num_triangles;        /* triangles stored (sum over meshes) */
+    uint64_t num_prims;            /* primitive ids addressable (sum over instances) */
+    uint64_t num_nodes;            /* 64-byte wide nodes */
+    uint32_t max_depth;            /* deepest root-to-leaf chain of wide nodes (TLAS + BLAS) */
+    uint32_t stack_entries;        /* worst-case traversal stack depth */
+    uint64_t device_bytes;         /* device memory held by the scene */
+    double build_ms;               /* host time to build (BVH + encode + upload) */
+    float bounds_lo[3];
+    float bounds_hi[3];
+} fnd_scene_info;
+
+/* ---------------------------------------------------------------- cameras and jobs */
+
+/* Pinhole camera, glm::lookAt conventions with a Z-up world as in
+ * src/Renderer/Renderer.cpp:373-380: forward = normalize(at - eye),
+ * right = normalize(cross(forward, up)), up' = cross(right, forward). */
+typedef struct fnd_camera {
+    float eye[3];
+    float at[3];
+    float up[3];
+    float fov_y_deg;
+} fnd_camera;
+
+#define FND_JOB_USE_CACHE 1u        /* terminate at the first diffuse vertex into the radiance cache */
+#define FND_JOB_TRAIN_CACHE 2u      /* also trace training paths and take one optimiser step */
+
+typedef struct fnd_render_job {
+    fnd_handle scene;
+    fnd_handle framebuffer;
+    fnd_camera camera;
+    uint32_t spp;                   /* samples per pixel added by this call */
+    uint32_t max_depth;             /* bounces; pbrt convention: 1 = direct light only */
+    uint32_t rr_depth;              /* Russian roulette from this bounce on; >= max_depth disables it */
+    uint32_t flags;                 /* FND_JOB_* */
+    uint64_t seed;
+    uint32_t sample_offset;         /* index of the first sample (progressive rendering) */
+    uint32_t reserved;
+} fnd_render_job;
+
+#define FND_FB_FP64 1u  /* also accumulate in double (oracle images only) */
+
+/* ---------------------------------------------------------------- stats and tuning */
+
+typedef struct fnd_pt_stats {
+    double last_kernel_ms;          /* device time of the last trace launch (HIP events on the context stream) */
+    double last_call_ms;            /* device time of the whole last trace/render call */
+    uint64_t rays_traced;           /* cumulative */
+    uint64_t node_visits;           /* cumulative, only with FND_TRACE_COUNT */
+    uint64_t tri_tests;             /* cumulative, only with FND_TRACE_COUNT */
+    uint64_t instance_visits;       /* cumulative, only with FND_TRACE_COUNT */
+    uint64_t paths;                 /* cumulative paths rendered */
+    uint64_t render_rays;           /* cumulative rays of fnd_pt_render (extension + shadow) */
+    uint64_t device_bytes;          /* current device allocation */
+    uint32_t num_cus;
+    uint32_t last_grid;             /* blocks of the last trace launch */
+} fnd_pt_stats;
+
+/* Knobs the autotuner searches.  0 means "library default". */
+typedef struct fnd_tuning {
+    uint32_t block_size;            /* 64, 128 or 256 threads */
+    uint32_t stack_lds;             /* per-lane LDS stack entries: 8, 12, 16, 24 or 32 */
+    uint32_t waves_per_cu;          /* persistent grid = CUs * waves_per_cu waves */
+    uint32_t refill;                /* refetch rays once this many lanes of a wave are idle (1..64) */
+    uint32_t queue_capacity;        /* paths per wavefront batch (render) */
+    uint32_t tile;                  /* pixel tile edge for the batch pixel order (render) */
+} fnd_tuning;
+
+#define FND_TRACE_CLOSEST 0u
+#define FND_TRACE_ANYHIT 1u
+#define FND_TRACE_COUNT 2u          /* count node visits / triangle tests into fnd_pt_stats */
+#define FND_ASYNC 0x100u            /* do not wait for the device before returning */
+
+/* ---------------------------------------------------------------- context */
+
+fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocator* allocator, fnd_pt_ctx** out_ctx);
+void fnd_pt_destroy(fnd_pt_ctx* ctx);
+const char* fnd_pt_last_error(const fnd_pt_ctx* ctx);
+const char* fnd_pt_status_string(fnd_status status);
+uint32_t fnd_pt_abi_version(void);
+fnd_status fnd_pt_wait(fnd_pt_ctx* ctx);
+fnd_status fnd_pt_get_stats(fnd_pt_ctx* ctx, fnd_pt_stats* out);
+fnd_status fnd_pt_reset_stats(fnd_pt_ctx* ctx);
+fnd_status fnd_pt_set_tuning(fnd_pt_ctx* ctx, const fnd_tuning* tuning);
+fnd_status fnd_pt_get_tuning(fnd_pt_ctx* ctx, fnd_tuning* out);
+
+/* ---------------------------------------------------------------- scenes */
+
+fnd_status fnd_pt_scene_create(fnd_pt_ctx* ctx, const fnd_scene_desc* desc, fnd_handle* out_scene);
+fnd_status fnd_pt_scene_destroy(fnd_pt_ctx* ctx, fnd_handle scene);
+fnd_status fnd_pt_scene_info(fnd_pt_ctx* ctx, fnd_handle scene, fnd_scene_info* out);
+/* Copies the encoded 64-byte wide nodes and the 48-byte leaf triangles
+ * (host copies of what the kernels read), for host-side checks.  Pass NULL
+ * pointers to query the byte sizes. */
+fnd_status fnd_pt_scene_export(fnd_pt_ctx* ctx, fnd_handle scene, void* nodes, uint64_t* nodes_bytes,
+                               void* tris, uint64_t* tris_bytes);
+
+/* ---------------------------------------------------------------- device buffers */
+
+fnd_status fnd_pt_buffer_create(fnd_pt_ctx* ctx, uint64_t bytes, fnd_handle* out_buffer);
+fnd_status fnd_pt_buffer_destroy(fnd_pt_ctx* ctx, fnd_handle buffer);
+fnd_status fnd_pt_buffer_write(fnd_pt_ctx* ctx, fnd_handle buffer, uint64_t offset, const void* src, uint64_t bytes);
+fnd_status fnd_pt_buffer_read(fnd_pt_ctx* ctx, fnd_handle buffer, uint64_t offset, void* dst, uint64_t bytes);
+
+/* ---------------------------------------------------------------- ray queries */
+
+/* Host arrays in and out (uploads, traces, downloads). */
+fnd_status fnd_pt_trace(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, uint32_t num_rays,
+                        fnd_hit* hits, uint32_t flags);
+/* Device-resident rays and hits (fnd_ray / fnd_hit arrays in buffers). */
+fnd_status fnd_pt_trace_buffers(fnd_pt_ctx* ctx, fnd_handle scene, fnd_handle rays, fnd_handle hits,
+                                uint32_t num_rays, uint32_t flags);
+
+/* ---------------------------------------------------------------- rendering */
+
+fnd_status fnd_pt_framebuffer_create(fnd_pt_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags,
+                                     fnd_handle* out_fb);
+fnd_status fnd_pt_framebuffer_destroy(fnd_pt_ctx* ctx, fnd_handle fb);
+fnd_status fnd_pt_framebuffer_clear(fnd_pt_ctx* ctx, fnd_handle fb);
+/* Mean radiance per pixel, RGB, row 0 at the top: width*height*3 values. */
+fnd_status fnd_pt_framebuffer_read(fnd_pt_ctx* ctx, fnd_handle fb, float* rgb, uint32_t* out_samples);
+fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle fb, double* rgb, uint32_t* out_samples);
+/* Renders all jobs in one batched wavefront launch sequence. */
+fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
+
+/* ---------------------------------------------------------------- host encoding */
+
+/* The host half of fnd_pt_scene_create without a device: validates the
+ * description and encodes the acceleration structure exactly as
+ * fnd_pt_scene_create uploads it (64-byte wide nodes, 48-byte leaf
+ * triangles, 64-byte instance records).  For host-side checks and tools. */
+typedef struct fnd_host_scene fnd_host_scene;
+typedef struct fnd_host_scene_view {
+    const void* nodes;
+    uint64_t num_nodes;
+    const void* tris;
+    uint64_t num_tris;
+    const void* instances;
+    uint32_t num_instances;
+    uint32_t root;
+    uint32_t two_level;
+    uint32_t reserved;
+    fnd_scene_info info;
+} fnd_host_scene_view;
+
+fnd_status fnd_scene_encode(const fnd_scene_desc* desc, fnd_host_scene** out, char* err, size_t err_len);
+fnd_status fnd_host_scene_get(const fnd_host_scene* scene, fnd_host_scene_view* out);
+void fnd_host_scene_free(fnd_host_scene* scene);
+
+/* ---------------------------------------------------------------- scene generators */
+
+/* Procedural scenes of the benchmark configurations.  Owned host objects;
+ * the pointers in the filled fnd_scene_desc stay valid until
+ * fnd_gen_scene_free. */
+typedef enum fnd_gen_kind {
+    FND_GEN_CORNELL_SOUP = 1,        /* Cornell box (5 quads + 2 boxes) + random soup, one quad light */
+    FND_GEN_DISPLACED_SPHERE = 2,    /* one displaced icosphere + random soup, single level */
+    FND_GEN_INSTANCED_SPHERES = 3,   /* instances of distinct displaced spheres + 4 quad lights, two level */
+    FND_GEN_FURNACE = 4              /* closed sphere seen from inside: emitting + reflecting Lambert */
+} fnd_gen_kind;
+
+typedef struct fnd_gen_params {
+    uint32_t kind;                   /* fnd_gen_kind */
+    uint64_t seed;
+    uint32_t soup_triangles;         /* random soup triangle count */
+    uint32_t subdivision;            /* icosphere face subdivision n: 20*n*n triangles per sphere */
+    uint32_t instances;              /* FND_GEN_INSTANCED_SPHERES */
+    uint32_t octaves;                /* value-noise octaves */
+    float amplitude;                 /* radial displacement amplitude */
+    float albedo;                    /* FND_GEN_FURNACE */
+    float emission;                  /* FND_GEN_FURNACE */
+    uint32_t ggx_fraction_pct;       /* share of GGX materials, percent */
+    uint32_t lights;                 /* quad lights */
+} fnd_gen_params;
+
+typedef struct fnd_generated_scene fnd_generated_scene;
+
+fnd_status fnd_gen_default_params(uint32_t kind, fnd_gen_params* out);
+fnd_status fnd_gen_scene(const fnd_gen_params* params, fnd_generated_scene** out);
+fnd_status fnd_gen_scene_get(const fnd_generated_scene* scene, fnd_scene_desc* out_desc, fnd_camera* out_camera);
+void fnd_gen_scene_free(fnd_generated_scene* scene);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FND_PT_H */

@@ -1,0 +1,102 @@
+/*
+ * pt_oracle.h — CPU oracle for the path-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * or as the timed host baseline.  The product (libfnd_pt.so) never links it.
+ *
+ * The reference (mos9527/Foundation) contains no path tracer: no BVH, ray
+ * intersection, RNG, BSDF or compute dispatch (its only GPU code is the raster
+ * shader src/Renderer/Triangle.slang:23-37).  This oracle therefore restates
+ * the published algorithms the path is specified by, and each function names
+ * its source.  The conventions it keeps from the reference are the camera
+ * (glm::lookAt, Z up, 45 degree vertical field of view,
+ * src/Renderer/Renderer.cpp:373-380) and 32-bit triangle indices
+ * (src/Platform/RHI/Vulkan/Command.cpp:289-305).
+ *
+ * How each part is pinned (what a plausible mistake would fail):
+ *   orc_pcg32_*         PCG reference demo vector (O'Neill 2014) and an
+ *                       independent Python-int restatement in the tests.
+ *   orc_intersect       exact rational orientation tests (Python fractions) on
+ *                       random and constructed cases; watertight fan test;
+ *                       axis-aligned and degenerate special cases.
+ *   orc_scene_build     structural invariants (orc_scene_check).
+ *   orc_trace (BVH)     brute force over every triangle (orc_trace with
+ *                       brute = 1) on scenes up to 50k triangles.
+ *   orc_trace (2-level) brute force over every instance and triangle.
+ *   orc_render          closed forms (furnace), unbiasedness of NEE vs BSDF
+ *                       sampling, pdf normalisation (see the header of
+ *                       pt_oracle_render.c).
+ */
+#ifndef PT_ORACLE_H
+#define PT_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NO_HIT 0xFFFFFFFFu
+
+uint32_t orc_version(void);
+
+/* PCG32 XSH-RR, O'Neill 2014 (pcg-random.org, pcg32_srandom_r / pcg32_random_r). */
+typedef struct orc_pcg32 {
+    uint64_t state;
+    uint64_t inc;
+} orc_pcg32;
+void orc_pcg32_srandom(orc_pcg32* rng, uint64_t initstate, uint64_t initseq);
+uint32_t orc_pcg32_next(orc_pcg32* rng);
+float orc_pcg32_float(orc_pcg32* rng); /* (u >> 8) * 2^-24, exact, in [0, 1) */
+void orc_pcg32_seq(uint64_t initstate, uint64_t initseq, uint32_t* out, uint32_t n);
+
+/* Woop, Benthin, Wald: "Watertight Ray/Triangle Intersection", JCGT 2(1), 2013.
+ * FP32 only: the double-precision re-evaluation of exactly-zero edge functions
+ * is left out; a zero counts as inside and the (t, id) tie rule decides.
+ * ray8 = org[3], tmin, dir[3], tmax; tri9 = v0, v1, v2.  Returns 1 on an
+ * intersection (t not range-checked), out3 = t, u (weight of v1), v (of v2). */
+int orc_intersect(const float* ray8, const float* tri9, float* out3);
+
+typedef struct orc_scene orc_scene;
+
+orc_scene* orc_scene_new(void);
+void orc_scene_free(orc_scene* s);
+/* Unindexed triangles, 9 floats each.  Returns the mesh index or -1. */
+int orc_scene_add_mesh(orc_scene* s, const float* tri_verts, uint32_t num_tris);
+/* Row-major 3x4 maps.  Returns the instance index or -1. */
+int orc_scene_add_instance(orc_scene* s, uint32_t mesh, const float* object_to_world, const float* world_to_object);
+/* Binned SAH (Wald 2007) per mesh, then a top level over instance boxes.
+ * Returns 0 on success. */
+int orc_scene_build(orc_scene* s, int leaf_max, int bins, int nthreads);
+/* Structural invariants of every tree; 0 = valid, else a code naming the
+ * first broken invariant. */
+int orc_scene_check(const orc_scene* s);
+uint64_t orc_scene_num_prims(const orc_scene* s);
+
+/* Closest-hit (anyhit = 0) or any-hit queries.  rays: n * 8 floats (fnd_ray),
+ * hits: n * 4 words (t, prim, u, v as fnd_hit).  brute = 1 tests every
+ * triangle of every instance instead of walking the trees.  counts, if not
+ * NULL, receives per ray (node visits, triangle tests).  nthreads <= 0 uses
+ * one thread per online CPU. */
+void orc_trace(const orc_scene* s, const float* rays, uint32_t n, int anyhit, int brute, int nthreads,
+               float* hits, uint32_t* counts);
+
+/* Host transcription of the GPU walk over the encoded wide-BVH bytes
+ * (fnd_scene_encode output: 64-byte nodes, 48-byte leaf triangles, 64-byte
+ * instance records).  counts, if not NULL, receives per ray (node visits,
+ * triangle tests, instance entries) — the same numbers the kernel counts. */
+void orc_trace_wide(const void* nodes, const void* tris, const void* instances, uint32_t root, int two_level,
+                    uint32_t stack_entries, const float* rays, uint32_t n, int anyhit, int nthreads, float* hits,
+                    uint32_t* counts);
+
+/* Deepest traversal stack any orc_trace_wide walk reached (0xFFFFFFFF if a
+ * walk hit its capacity); reset = 1 clears it. */
+uint32_t orc_wide_max_stack(int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

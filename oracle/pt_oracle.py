@@ -1,0 +1,149 @@
+"""ctypes binding of the CPU oracle (liboracle_pt.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed host baseline.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
+NO_HIT = 0xFFFFFFFF
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: run `make -C oracle`")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.orc_version.restype = C.c_uint32
+    lib.orc_pcg32_seq.argtypes = [C.c_uint64, C.c_uint64, P, C.c_uint32]
+    lib.orc_pcg32_seq.restype = None
+    lib.orc_intersect.argtypes = [P, P, P]
+    lib.orc_intersect.restype = C.c_int
+    lib.orc_scene_new.restype = P
+    lib.orc_scene_free.argtypes = [P]
+    lib.orc_scene_free.restype = None
+    lib.orc_scene_add_mesh.argtypes = [P, P, C.c_uint32]
+    lib.orc_scene_add_mesh.restype = C.c_int
+    lib.orc_scene_add_instance.argtypes = [P, C.c_uint32, P, P]
+    lib.orc_scene_add_instance.restype = C.c_int
+    lib.orc_scene_build.argtypes = [P, C.c_int, C.c_int, C.c_int]
+    lib.orc_scene_build.restype = C.c_int
+    lib.orc_scene_check.argtypes = [P]
+    lib.orc_scene_check.restype = C.c_int
+    lib.orc_scene_num_prims.argtypes = [P]
+    lib.orc_scene_num_prims.restype = C.c_uint64
+    lib.orc_trace.argtypes = [P, P, C.c_uint32, C.c_int, C.c_int, C.c_int, P, P]
+    lib.orc_trace.restype = None
+    lib.orc_trace_wide.argtypes = [P, P, P, C.c_uint32, C.c_int, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int, P, P]
+    lib.orc_trace_wide.restype = None
+    lib.orc_wide_max_stack.argtypes = [C.c_int]
+    lib.orc_wide_max_stack.restype = C.c_uint32
+    _lib = lib
+    return lib
+
+
+def pcg32_seq(initstate, initseq, n):
+    out = np.zeros(n, np.uint32)
+    load().orc_pcg32_seq(initstate, initseq, out.ctypes.data, n)
+    return out
+
+
+def intersect(ray8, tri9):
+    r = np.ascontiguousarray(ray8, np.float32)
+    t = np.ascontiguousarray(tri9, np.float32)
+    out = np.zeros(3, np.float32)
+    hit = load().orc_intersect(r.ctypes.data, t.ctypes.data, out.ctypes.data)
+    return bool(hit), out
+
+
+class OracleScene:
+    """Meshes (unindexed, 9 floats per triangle) + optional instances, binary
+    SAH trees built by the oracle itself."""
+
+    def __init__(self):
+        self.lib = load()
+        self.h = self.lib.orc_scene_new()
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.orc_scene_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_mesh(self, tri_verts):
+        tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+        r = self.lib.orc_scene_add_mesh(self.h, tv.ctypes.data, tv.shape[0])
+        assert r >= 0
+        return r
+
+    def add_instance(self, mesh, o2w, w2o):
+        a = np.ascontiguousarray(o2w, np.float32).reshape(12)
+        b = np.ascontiguousarray(w2o, np.float32).reshape(12)
+        r = self.lib.orc_scene_add_instance(self.h, mesh, a.ctypes.data, b.ctypes.data)
+        assert r >= 0
+        return r
+
+    def build(self, leaf_max=4, bins=16, threads=0):
+        rc = self.lib.orc_scene_build(self.h, leaf_max, bins, threads)
+        assert rc == 0, rc
+        return self
+
+    def check(self):
+        return self.lib.orc_scene_check(self.h)
+
+    def num_prims(self):
+        return self.lib.orc_scene_num_prims(self.h)
+
+    def trace(self, rays, anyhit=False, brute=False, threads=0, counts=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 4), np.float32)
+        cnt = np.zeros((rays.shape[0], 2), np.uint32) if counts else None
+        self.lib.orc_trace(self.h, rays.ctypes.data, rays.shape[0], int(anyhit), int(brute), threads,
+                           hits.ctypes.data, cnt.ctypes.data if counts else None)
+        return (hits, cnt) if counts else hits
+
+
+def scene_from_generated(gen, leaf_max=4, bins=16, threads=0):
+    """Oracle scene holding the same triangles and transforms as a
+    foundation_amd.pt.GeneratedScene."""
+    s = OracleScene()
+    meshes = gen.meshes()
+    for m in meshes:
+        s.add_mesh(m["positions"][m["indices"]].reshape(-1, 9))
+    for inst in gen.instances():
+        s.add_instance(inst["mesh"], inst["o2w"], inst["w2o"])
+    return s.build(leaf_max=leaf_max, bins=bins, threads=threads)
+
+
+def trace_wide(encoded, rays, anyhit=False, threads=0):
+    """Host transcription of the GPU walk over fnd_scene_encode output
+    (foundation_amd.pt.encode_scene).  Returns (hits, counts (N,3))."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    hits = np.zeros((rays.shape[0], 4), np.float32)
+    cnt = np.zeros((rays.shape[0], 3), np.uint32)
+    nodes = np.ascontiguousarray(encoded["nodes"])
+    tris = np.ascontiguousarray(encoded["tris"])
+    insts = np.ascontiguousarray(encoded["instances"])
+    load().orc_wide_max_stack(1)
+    load().orc_trace_wide(nodes.ctypes.data, tris.ctypes.data, insts.ctypes.data if insts.size else None,
+                          encoded["root"], int(encoded["two_level"]), encoded["stack_entries"],
+                          rays.ctypes.data, rays.shape[0], int(anyhit), threads, hits.ctypes.data, cnt.ctypes.data)
+    max_stack = load().orc_wide_max_stack(0)
+    if max_stack > encoded["stack_entries"]:
+        raise AssertionError(f"walk needed {max_stack} stack entries, bound is {encoded['stack_entries']}")
+    return hits, cnt

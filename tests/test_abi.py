@@ -1,0 +1,138 @@
+"""The C ABI library loads, exports every function include/fnd_pt.h declares,
+and its host-only entry points behave (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from foundation_amd import _abi as A
+from foundation_amd import pt
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fnd_pt.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(fnd_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = declared_functions()
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_the_header():
+    assert sorted(A.PROTOTYPES) == declared_functions()
+
+
+def test_abi_version_and_status_strings(lib):
+    assert lib.fnd_pt_abi_version() == 1
+    for s in range(8):
+        assert lib.fnd_pt_status_string(s)
+    assert lib.fnd_pt_status_string(99) == b"unknown status"
+
+
+def test_create_rejects_bad_arguments(lib):
+    assert lib.fnd_pt_create(None, None, None) == A.E_INVALID_ARG
+    bad = A.HostAllocator()  # null callbacks
+    h = C.c_void_p()
+    assert lib.fnd_pt_create(None, C.byref(bad), C.byref(h)) == A.E_INVALID_ARG
+    assert lib.fnd_pt_last_error(None) == b"null context"
+
+
+def test_null_context_calls_fail_cleanly(lib):
+    assert lib.fnd_pt_wait(None) == A.E_INVALID_ARG
+    assert lib.fnd_pt_get_stats(None, None) == A.E_INVALID_ARG
+    assert lib.fnd_pt_trace(None, 0, None, 0, None, 0) == A.E_INVALID_ARG
+    assert lib.fnd_pt_render(None, None, 0) == A.E_INVALID_ARG
+    lib.fnd_pt_destroy(None)
+
+
+def test_allocator_oom_is_reported(lib):
+    """The context's own memory comes from the caller's allocator; a refusing
+    allocator yields FND_E_OOM_HOST, never a crash."""
+    calls = []
+    alloc_t = A.HostAllocator._fields_[1][1]
+    free_t = A.HostAllocator._fields_[3][1]
+
+    @alloc_t
+    def refuse(user, size, align):
+        calls.append(size)
+        return None
+
+    @free_t
+    def nofree(user, p):
+        pass
+
+    a = A.HostAllocator()
+    a.alloc = refuse
+    a.free = nofree
+    h = C.c_void_p()
+    assert lib.fnd_pt_create(None, C.byref(a), C.byref(h)) == A.E_OOM_HOST
+    assert calls and not h.value
+
+
+def test_generator_parameters(lib):
+    p = A.GenParams()
+    assert lib.fnd_gen_default_params(A.GEN_CORNELL_SOUP, C.byref(p)) == A.OK
+    assert p.soup_triangles == 10000 and p.seed == 1
+    assert lib.fnd_gen_default_params(A.GEN_DISPLACED_SPHERE, C.byref(p)) == A.OK
+    assert (p.subdivision, p.soup_triangles, p.octaves) == (256, 200000, 3)
+    assert abs(p.amplitude - 0.1) < 1e-7
+    assert lib.fnd_gen_default_params(A.GEN_INSTANCED_SPHERES, C.byref(p)) == A.OK
+    assert (p.instances, p.subdivision) == (100, 71)
+    assert lib.fnd_gen_default_params(77, C.byref(p)) == A.E_INVALID_ARG
+    p.kind = A.GEN_DISPLACED_SPHERE
+    p.subdivision = 0
+    h = C.c_void_p()
+    assert lib.fnd_gen_scene(C.byref(p), C.byref(h)) == A.E_INVALID_ARG
+
+
+def _tiny_mesh_desc(pos, idx, mats=None):
+    pos = np.ascontiguousarray(pos, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    m = A.MeshDesc()
+    m.positions = pos.ctypes.data_as(C.POINTER(C.c_float))
+    m.num_vertices = pos.shape[0]
+    m.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+    m.num_triangles = idx.shape[0]
+    if mats is not None:
+        m.material_ids = mats.ctypes.data_as(C.POINTER(C.c_uint32))
+    meshes = (A.MeshDesc * 1)(m)
+    d = A.SceneDesc()
+    d.meshes = meshes
+    d.num_meshes = 1
+    return d, (pos, idx, meshes, mats)
+
+
+def test_encode_rejects_bad_scenes():
+    pos = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    d, keep = _tiny_mesh_desc(pos, [[0, 1, 3]])
+    with pytest.raises(pt.FndError, match="out of range"):
+        pt.encode_scene(d)
+    d, keep = _tiny_mesh_desc([[0, 0, 0], [1, 0, np.nan], [0, 1, 0]], [[0, 1, 2]])
+    with pytest.raises(pt.FndError, match="non-finite"):
+        pt.encode_scene(d)
+    mats = np.array([5], np.uint32)
+    d, keep = _tiny_mesh_desc(pos, [[0, 1, 2]], mats)
+    with pytest.raises(pt.FndError, match="material"):
+        pt.encode_scene(d)
+    d = A.SceneDesc()
+    with pytest.raises(pt.FndError, match="no meshes"):
+        pt.encode_scene(d)
+
+
+def test_encode_single_triangle():
+    d, keep = _tiny_mesh_desc([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 2]])
+    enc = pt.encode_scene(d)
+    assert enc["nodes"].shape == (1,) and enc["tris"].shape == (1,)
+    n = enc["nodes"][0]
+    assert n["imask"] == 0 and (n["meta"] & 0xff) == 1  # one leaf slot with one triangle
+    assert enc["stack_entries"] == 3 * enc["max_depth"] + 2
