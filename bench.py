@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Mrays/s of incoherent closest-hit rays at 10M triangles.
+
+Workload (one "step"): one closest-hit launch over the incoherent ray set of
+the 10M-triangle scene — 100 instances of distinct 100,820-triangle displaced
+icospheres (random rotation, scale in [0.5, 2], placement), two-level BVH,
+plus 4 quad lights; the rays are one cosine-weighted bounce ray about the
+geometric normal at every primary hit of a 1920x1080 pinhole view (PCG32,
+seed 1, stream = pixel).  Rays and hits stay in HBM across the timed steps.
+
+Printed (stdout, one JSON line): value = rays * steps / wall time of the timed
+steps; roofline of the traversal kernel from its algorithmic bytes (32 B ray +
+16 B hit + 64 B per visited node + 48 B per tested triangle + 64 B per entered
+instance, counted by the kernel's counting variant over the same rays) over
+its mean launch time from HIP events; the CPU oracle timed on the same rays
+on the host cores (cpu_baseline); and hit-record parity GPU vs oracle.
+Progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_gpu_set(ctx, scene, rays, steps, warmup, flags):
+    import numpy as np
+    from foundation_amd import _abi as A
+    n = rays.shape[0]
+    rb = ctx.buffer_create(n * 32)
+    hb = ctx.buffer_create(n * 16)
+    ctx.buffer_write(rb, rays)
+    for _ in range(warmup):
+        ctx.trace_buffers(scene, rb, hb, n, flags | A.ASYNC)
+    ctx.wait()
+    ctx.reset_stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.trace_buffers(scene, rb, hb, n, flags | A.ASYNC)
+    ctx.wait()
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    kernel_ms = st.trace_kernel_ms_total / max(1, st.trace_launches)
+    hits = ctx.buffer_read(hb, np.zeros((n, 4), np.float32))
+    # algorithmic traffic from the counting variant over the same rays
+    ctx.reset_stats()
+    ctx.trace_buffers(scene, rb, hb, n, flags | A.TRACE_COUNT)
+    cs = ctx.stats()
+    ctx.buffer_destroy(rb)
+    ctx.buffer_destroy(hb)
+    counts = dict(nodes=cs.node_visits, tris=cs.tri_tests, instances=cs.instance_visits)
+    return wall, kernel_ms, hits, counts
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--instances", type=int, default=100)
+    ap.add_argument("--subdivision", type=int, default=71)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, online CPUs)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
+    ap.add_argument("--pmc-summary", default=os.path.join(ROOT, "profiles", "trace_incoherent_10M_pmc.json"))
+    ap.add_argument("--only-incoherent", action="store_true",
+                    help="profiling runs: time only the incoherent set (no coherent/shadow sets, no CPU leg)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="profiling runs: first copy this many MiB device-to-device (a plain copy kernel of "
+                         "known size that scales the FETCH_SIZE / WRITE_SIZE counters)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: every GPU box of this build has one, and the path shards by "
+              "samples or image tiles with no exchange inside a step (see DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    if args.calibrate_copy_mib:
+        import torch
+        n = args.calibrate_copy_mib * 2**20 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy_mib} MiB device-to-device")
+        del src, dst
+
+    from foundation_amd import _abi as A
+    from foundation_amd import pt
+    from foundation_amd import rays as R
+
+    t = time.perf_counter()
+    gen = pt.GeneratedScene(A.GEN_INSTANCED_SPHERES, instances=args.instances, subdivision=args.subdivision)
+    nprims = gen.num_prims()
+    log(f"scene generated: {nprims} primitives in {time.perf_counter() - t:.1f}s")
+    ctx = pt.Context(0)
+    scene = ctx.scene_create(gen.desc)
+    info = ctx.scene_info(scene)
+    log(f"scene built: {info.num_triangles} tris, {info.num_nodes} nodes, depth {info.max_depth}, "
+        f"{info.build_ms:.0f} ms, {info.device_bytes / 2**20:.0f} MiB")
+
+    primary = R.primary_rays(gen.camera, args.width, args.height)
+    prim_hits = ctx.trace(scene, primary)
+    lookup = gen.triangle_lookup()
+    incoherent, _ = R.bounce_rays(primary, prim_hits, lookup, seed=1)
+    shadow, _ = R.shadow_rays(primary, prim_hits, lookup, [0.0, 0.0, 9.0])
+    nrays = incoherent.shape[0]
+    log(f"ray sets: {primary.shape[0]} primary, {nrays} incoherent, {shadow.shape[0]} shadow")
+
+    wall, kernel_ms, gpu_hits, counts = time_gpu_set(ctx, scene, incoherent, args.steps, args.warmup, 0)
+    value = nrays * args.steps / wall / 1e6
+    log(f"incoherent: {value:.1f} Mrays/s, kernel {kernel_ms:.3f} ms/launch, counts {counts}")
+    algo_bytes = nrays * (32 + 16) + 64 * counts["nodes"] + 48 * counts["tris"] + 64 * counts["instances"]
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+
+    coherent = shadow_rate = None
+    ccounts = {"nodes": 0}
+    gpu_shadow = None
+    if not args.only_incoherent:
+        csteps = max(2, args.steps // 2)
+        cwall, ckms, _, ccounts = time_gpu_set(ctx, scene, primary, csteps, 1, 0)
+        swall, skms, gpu_shadow, _ = time_gpu_set(ctx, scene, shadow, csteps, 1, A.TRACE_ANYHIT)
+        coherent = primary.shape[0] * csteps / cwall / 1e6
+        shadow_rate = shadow.shape[0] * csteps / swall / 1e6
+        log(f"coherent: {coherent:.1f} Mrays/s, shadow any-hit: {shadow_rate:.1f} Mrays/s")
+
+    traffic = None
+    if os.path.exists(args.pmc_summary):
+        try:
+            pmc = json.load(open(args.pmc_summary))
+            traffic = pmc.get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    cpu = None
+    parity = None
+    if not args.no_cpu and not args.only_incoherent:
+        from oracle import pt_oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        oscene = O.scene_from_generated(gen, threads=threads)
+        log(f"oracle built in {time.perf_counter() - t:.1f}s")
+        t = time.perf_counter()
+        ref = oscene.trace(incoherent, threads=threads)          # warm-up run, also the parity reference
+        first = time.perf_counter() - t
+        reps = max(1, min(5, int(25.0 / max(first, 1e-3))))
+        times = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            oscene.trace(incoherent, threads=threads)
+            times.append(time.perf_counter() - t)
+        med = statistics.median(times)
+        cpu = {"value": round(nrays / med / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"all {nrays} incoherent rays, binary-BVH oracle, 1 warm-up + median of {reps}"}
+        g = gpu_hits.view(np.uint32)
+        r = ref.view(np.uint32)
+        mism = int(np.any(g != r, axis=1).sum())
+        socc = gpu_shadow.view(np.uint32)[:, 1] != 0xFFFFFFFF
+        oocc = oscene.trace(shadow, anyhit=True, threads=threads).view(np.uint32)[:, 1] != 0xFFFFFFFF
+        parity = {"incoherent_rays": nrays, "closest_hit_record_mismatches": mism,
+                  "shadow_rays": int(shadow.shape[0]), "anyhit_mismatches": int((socc != oocc).sum())}
+        log(f"cpu: {cpu['value']} Mrays/s on {threads} threads; parity {parity}")
+
+    line = {
+        "metric": "Mrays/s incoherent @10M tris",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "incoherent closest-hit rays, 100 instanced displaced spheres (10.08M tris), "
+                               "two-level 64-byte wide BVH", "width": args.width, "height": args.height,
+                   "rays_per_step": nrays, "triangles": int(nprims)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": int(algo_bytes), "kernel_ms": round(kernel_ms, 4)},
+        "cpu_baseline": cpu,
+        "extra": {"coherent_mrays_s": coherent and round(coherent, 2),
+                  "shadow_anyhit_mrays_s": shadow_rate and round(shadow_rate, 2),
+                  "nodes_per_ray": round(counts["nodes"] / nrays, 3), "tris_per_ray": round(counts["tris"] / nrays, 3),
+                  "instances_per_ray": round(counts["instances"] / nrays, 3),
+                  "coherent_nodes_per_ray": round(ccounts["nodes"] / primary.shape[0], 3),
+                  "scene_nodes": int(info.num_nodes), "scene_build_ms": round(info.build_ms, 1),
+                  "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
+                  "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

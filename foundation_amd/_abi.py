@@ -142,6 +142,8 @@ class Stats(C.Structure):
     _fields_ = [
         ("last_kernel_ms", C.c_double),
         ("last_call_ms", C.c_double),
+        ("trace_kernel_ms_total", C.c_double),
+        ("trace_launches", C.c_uint64),
         ("rays_traced", C.c_uint64),
         ("node_visits", C.c_uint64),
         ("tri_tests", C.c_uint64),

@@ -103,6 +103,42 @@ Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h)
     return it == ctx->framebuffers.end() ? nullptr : it->second;
 }
 
+// Resolves the pending (start, end) event pairs of trace launches into the
+// cumulative kernel time.  Call only after the stream has drained.
+void resolve_timings(fnd_pt_ctx* ctx)
+{
+    for (size_t i = 0; i + 1 < ctx->timing_used; i += 2) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ctx->timing_events[i], ctx->timing_events[i + 1]) == hipSuccess) {
+            ctx->stats.trace_kernel_ms_total += ms;
+            ctx->stats.trace_launches += 1;
+            ctx->stats.last_kernel_ms = ms;
+        }
+    }
+    ctx->timing_used = 0;
+}
+
+// A (start, end) pair for the next launch; drains the stream when the ring is full.
+fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end)
+{
+    constexpr size_t kMaxPending = 2048;
+    if (ctx->timing_used >= 2 * kMaxPending) {
+        fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        if (st != FND_OK) return st;
+        resolve_timings(ctx);
+    }
+    while (ctx->timing_events.size() < ctx->timing_used + 2) {
+        hipEvent_t e;
+        fnd_status st = check_hip(ctx, hipEventCreate(&e), "hipEventCreate");
+        if (st != FND_OK) return st;
+        ctx->timing_events.push_back(e);
+    }
+    *start = ctx->timing_events[ctx->timing_used];
+    *end = ctx->timing_events[ctx->timing_used + 1];
+    ctx->timing_used += 2;
+    return FND_OK;
+}
+
 uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block)
 {
     const uint32_t waves = std::max(1u, ctx->num_cus * ctx->tuning.waves_per_cu);
@@ -285,6 +321,8 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
     dev_free(ctx, ctx->d_tmp, ctx->tmp_bytes);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : ctx->timing_events) (void)hipEventDestroy(ev);
+    ctx->timing_events.clear();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     const fnd_host_allocator a = ctx->alloc;
     const bool has = ctx->has_alloc;
@@ -301,7 +339,9 @@ const char* fnd_pt_last_error(const fnd_pt_ctx* ctx)
 fnd_status fnd_pt_wait(fnd_pt_ctx* ctx)
 {
     if (!ctx) return FND_E_INVALID_ARG;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (st == FND_OK) resolve_timings(ctx);
+    return st;
 }
 
 fnd_status fnd_pt_get_stats(fnd_pt_ctx* ctx, fnd_pt_stats* out)
@@ -554,19 +594,20 @@ static fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float
         st = check_hip(ctx, hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream), "reset stats");
         if (st != FND_OK) return st;
     }
-    (void)hipEventRecord(ctx->ev[0], ctx->stream);
+    hipEvent_t e0, e1;
+    st = timing_pair(ctx, &e0, &e1);
+    if (st != FND_OK) return st;
+    (void)hipEventRecord(e0, ctx->stream);
     st = check_hip(ctx, launch_trace(l, a, ctx->stream), "trace launch");
     if (st != FND_OK) return st;
-    (void)hipEventRecord(ctx->ev[1], ctx->stream);
+    (void)hipEventRecord(e1, ctx->stream);
     ctx->stats.last_grid = l.grid;
     ctx->stats.rays_traced += n;
     if (flags & FND_ASYNC) return FND_OK;
-    st = check_hip(ctx, hipEventSynchronize(ctx->ev[1]), "trace");
+    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "trace");
     if (st != FND_OK) return st;
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-    ctx->stats.last_kernel_ms = ms;
-    ctx->stats.last_call_ms = ms;
+    resolve_timings(ctx);
+    ctx->stats.last_call_ms = ctx->stats.last_kernel_ms;
     if (count) {
         unsigned long long c[8];
         st = copy_sync(ctx, c, ctx->d_stats, sizeof c, hipMemcpyDeviceToHost, "read counters");

@@ -56,6 +56,8 @@ struct fnd_pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> timing_events;     // pairs (start, end) per pending trace launch
+    size_t timing_used = 0;                    // events in use (even)
     std::string last_error;
     uint64_t budget = 0;
     uint64_t used = 0;
@@ -87,6 +89,8 @@ fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
 Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
 uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block);
+void resolve_timings(fnd_pt_ctx* ctx);
+fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);
 void render_state_free(fnd_pt_ctx* ctx);
 
 }  // namespace fnd

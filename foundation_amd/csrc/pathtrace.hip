@@ -1,0 +1,327 @@
+// pathtrace.hip — wavefront path tracer stages (generate, shade, shadow
+// resolve, accumulate).  Extension and shadow rays are traced by the
+// persistent traversal kernel (trace_kernel.hpp) between these stages.
+//
+// Estimator (unidirectional path tracing, pbrt-v3 conventions): at the vertex
+// reached after `bounce` segments, emission is added with the power-heuristic
+// weight against light sampling; the path stops there if bounce == max_depth;
+// otherwise one light sample (next-event estimation, shadow ray) and one BSDF
+// sample extend it, with Russian roulette from bounce rr_depth on
+// (survival q = min(max throughput, 0.95)).
+//
+// Appends to the next extension queue and to the shadow queue are compacted
+// per wave: a ballot of the lanes that append, one atomicAdd per wave, and
+// each lane's slot from the popcount of the ballot below it.
+#include "pt_device.hpp"
+#include "pt_shading.hpp"
+#include "wavefront.hpp"
+
+namespace fnd {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lanes_below_mask(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
+}
+
+// Wave-compacted append: returns this lane's slot (valid only where pred).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter, unsigned long long* total)
+{
+    const uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int leader = __builtin_ctzll(mask);
+    uint32_t base = 0;
+    if (int(__lane_id()) == leader) {
+        base = atomicAdd(counter, uint32_t(__popcll(mask)));
+        if (total) atomicAdd(total, (unsigned long long)__popcll(mask));
+    }
+    base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+    return base + lanes_below_mask(mask);
+}
+
+__device__ __forceinline__ uint32_t find_job(const JobDev* jobs, uint32_t num_jobs, uint64_t g)
+{
+    uint32_t lo = 0, hi = num_jobs;  // last job with path_base <= g
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].path_base <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ float ray_eps(V3 p)
+{
+    return 1e-4f * fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+}
+
+__global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_t n)
+{
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool live = false;
+        uint32_t pixel = 0xFFFFFFFFu, sample = 0, j = 0;
+        Ray ray{};
+        if (i < n) {
+            const uint64_t g = g0 + i;
+            j = find_job(a.jobs, a.num_jobs, g);
+            const JobDev& job = a.jobs[j];
+            const uint64_t local = g - job.path_base;
+            const uint32_t s = uint32_t(local / job.padded);
+            const uint32_t k = uint32_t(local % job.padded);
+            const uint32_t T = job.tile;
+            const uint32_t t = k / (T * T), q = k % (T * T);
+            const uint32_t x = (t % job.tiles_x) * T + q % T;
+            const uint32_t y = (t / job.tiles_x) * T + q / T;
+            sample = job.sample_offset + s;
+            if (x < job.width && y < job.height) {
+                pixel = y * job.width + x;
+                Rng rng = rng_for(job.seed, sample, pixel, 0u);
+                const float jx = rng_float(rng), jy = rng_float(rng);
+                const float sx = (2.0f * (float(x) + jx) / float(job.width) - 1.0f) * job.tan_half * job.aspect;
+                const float sy = (1.0f - 2.0f * (float(y) + jy) / float(job.height)) * job.tan_half;
+                const V3 f = v3(job.fwd[0], job.fwd[1], job.fwd[2]);
+                const V3 r = v3(job.right[0], job.right[1], job.right[2]);
+                const V3 u = v3(job.up[0], job.up[1], job.up[2]);
+                const V3 d = normalize(f + r * sx + u * sy);
+                ray.org[0] = job.eye[0]; ray.org[1] = job.eye[1]; ray.org[2] = job.eye[2];
+                ray.tmin = 0.0f;
+                ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+                ray.tmax = 3.0e38f;
+                live = true;
+            }
+            a.b.p_job[i] = j;
+            a.b.p_pixel[i] = pixel;
+            a.b.p_sample[i] = sample;
+            a.b.p_thr[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+            a.b.p_L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const uint32_t slot = wave_append(live, &a.b.counters[kCntExtA], a.b.rays);
+        if (live) {
+            a.b.ext_rays[0][slot] = ray;
+            a.b.ext_path[0][slot] = i;
+            a.b.ext_scene[0][slot] = a.jobs[j].scene_slot;
+        }
+    }
+}
+
+// Zeroes the counters the next bounce appends to and the traversal work counters.
+__global__ void k_prep(uint32_t* counters, uint32_t next)
+{
+    if (threadIdx.x == 0) {
+        counters[next] = 0u;
+        counters[kCntShadow] = 0u;
+        counters[kCntWorkExt] = 0u;
+        counters[kCntWorkShadow] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t bounce)
+{
+    const uint32_t n = a.b.counters[cur];
+    const uint32_t nxt = cur ^ 1u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool cont = false, shadow = false;
+        Ray next_ray{}, sh_ray{};
+        V3 sh_contrib = v3(0, 0, 0);
+        uint32_t path = 0, scene_slot = 0;
+        if (i < n) {
+            path = a.b.ext_path[cur][i];
+            const JobDev& job = a.jobs[a.b.p_job[path]];
+            scene_slot = job.scene_slot;
+            const ShadeScene& sc = a.scenes[scene_slot];
+            const float4 hit = a.b.ext_hits[i];
+            const Ray ray = a.b.ext_rays[cur][i];
+            float4 thr4 = a.b.p_thr[path];
+            float4 L4 = a.b.p_L[path];
+            V3 thr = v3(thr4.x, thr4.y, thr4.z);
+            V3 L = v3(L4.x, L4.y, L4.z);
+            const float pdf_prev = thr4.w;
+            const V3 dir = v3(ray.dir[0], ray.dir[1], ray.dir[2]);
+            const uint32_t tri = __float_as_uint(hit.y);
+            if (tri == 0xFFFFFFFFu) {
+                L = L + thr * v3(sc.background[0], sc.background[1], sc.background[2]);
+            } else {
+                const float t = hit.x;
+                const uint32_t inst = __float_as_uint(hit.z);
+                const float4* tp = sc.tris + 3ull * tri;
+                const float4 A = tp[0], B = tp[1], C = tp[2];
+                const uint32_t mat_id = __float_as_uint(B.w);
+                const int32_t light_id = __float_as_int(C.w);
+                V3 ng = cross(v3(B.x - A.x, B.y - A.y, B.z - A.z), v3(C.x - A.x, C.y - A.y, C.z - A.z));
+                if (inst != 0xFFFFFFFFu) {
+                    const float* m = sc.insts[inst].w2o;
+                    ng = v3(m[0] * ng.x + m[4] * ng.y + m[8] * ng.z, m[1] * ng.x + m[5] * ng.y + m[9] * ng.z,
+                            m[2] * ng.x + m[6] * ng.y + m[10] * ng.z);
+                }
+                ng = normalize(ng);
+                const bool front = dot(ng, dir) < 0.0f;
+                const V3 nf = front ? ng : -ng;  // faces the incoming ray
+                const V3 p = v3(ray.org[0], ray.org[1], ray.org[2]) + dir * t;
+                const DeviceMaterial mat = sc.materials[mat_id];
+                const V3 Le = v3(mat.emission[0], mat.emission[1], mat.emission[2]);
+                const bool emissive = mat.type == 2u;
+                if ((Le.x > 0.f || Le.y > 0.f || Le.z > 0.f) && (!emissive || front)) {
+                    float w = 1.0f;
+                    if (bounce > 0 && pdf_prev > 0.0f && light_id >= 0) {
+                        const DeviceLight& lt = sc.lights[light_id];
+                        const V3 nl = normalize(cross(v3(lt.edge0[0], lt.edge0[1], lt.edge0[2]),
+                                                      v3(lt.edge1[0], lt.edge1[1], lt.edge1[2])));
+                        const float cl = fabsf(dot(nl, dir));
+                        const float pdf_l = cl > 0.0f ? lt.pdf_select * t * t / (lt.area * cl) : 0.0f;
+                        w = power_heuristic(pdf_prev, pdf_l);
+                    }
+                    L = L + thr * Le * w;
+                }
+                if (!emissive && bounce < job.max_depth) {
+                    Rng rng = rng_for(job.seed, a.b.p_sample[path], a.b.p_pixel[path], bounce + 1u);
+                    const Onb frame = onb_from(nf);
+                    const V3 wo = to_local(frame, -dir);
+                    const V3 albedo = v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
+                    const float eps = ray_eps(p);
+                    const V3 org = p + nf * eps;
+                    // next-event estimation
+                    const float ul = rng_float(rng), ua = rng_float(rng), ub = rng_float(rng);
+                    if (sc.num_lights > 0) {
+                        uint32_t li = 0;
+                        while (li + 1 < sc.num_lights && ul >= sc.lights[li].cdf) ++li;
+                        const DeviceLight& lt = sc.lights[li];
+                        const V3 e0 = v3(lt.edge0[0], lt.edge0[1], lt.edge0[2]);
+                        const V3 e1 = v3(lt.edge1[0], lt.edge1[1], lt.edge1[2]);
+                        const V3 x = v3(lt.corner[0], lt.corner[1], lt.corner[2]) + e0 * ua + e1 * ub;
+                        const V3 nl = normalize(cross(e0, e1));
+                        V3 wi = x - p;
+                        const float d2 = dot(wi, wi);
+                        wi = wi * (1.0f / sqrtf(d2));
+                        const float cl = -dot(nl, wi);
+                        if (cl > 0.0f && d2 > 0.0f) {
+                            const float pdf_l = lt.pdf_select * d2 / (lt.area * cl);
+                            const BsdfEval ev = bsdf_eval(mat.type, albedo, mat.roughness, wo, to_local(frame, wi));
+                            if (ev.pdf > 0.0f) {
+                                const float w = power_heuristic(pdf_l, ev.pdf);
+                                sh_contrib = thr * ev.f_cos * v3(lt.radiance[0], lt.radiance[1], lt.radiance[2]) * (w / pdf_l);
+                                if (sh_contrib.x > 0.f || sh_contrib.y > 0.f || sh_contrib.z > 0.f) {
+                                    const V3 to = x - org;
+                                    sh_ray.org[0] = org.x; sh_ray.org[1] = org.y; sh_ray.org[2] = org.z;
+                                    sh_ray.tmin = 0.0f;
+                                    sh_ray.dir[0] = to.x; sh_ray.dir[1] = to.y; sh_ray.dir[2] = to.z;
+                                    sh_ray.tmax = 1.0f - 1e-3f;
+                                    shadow = true;
+                                }
+                            }
+                        }
+                    }
+                    // BSDF sampling
+                    const float u1 = rng_float(rng), u2 = rng_float(rng), urr = rng_float(rng);
+                    const BsdfSample bs = bsdf_sample(mat.type, albedo, mat.roughness, wo, u1, u2);
+                    if (bs.ok) {
+                        V3 nthr = thr * bs.weight;
+                        bool alive = max3(nthr) > 0.0f;
+                        if (alive && bounce >= job.rr_depth) {
+                            const float q = fminf(max3(nthr), 0.95f);
+                            if (urr >= q) alive = false; else nthr = nthr * (1.0f / q);
+                        }
+                        if (alive) {
+                            const V3 wi = normalize(to_world(frame, bs.wi));
+                            next_ray.org[0] = org.x; next_ray.org[1] = org.y; next_ray.org[2] = org.z;
+                            next_ray.tmin = 0.0f;
+                            next_ray.dir[0] = wi.x; next_ray.dir[1] = wi.y; next_ray.dir[2] = wi.z;
+                            next_ray.tmax = 3.0e38f;
+                            thr = nthr;
+                            cont = true;
+                            a.b.p_thr[path] = make_float4(thr.x, thr.y, thr.z, bs.pdf);
+                        }
+                    }
+                }
+            }
+            a.b.p_L[path] = make_float4(L.x, L.y, L.z, 0.f);
+        }
+        const uint32_t es = wave_append(cont, &a.b.counters[nxt], a.b.rays);
+        if (cont) {
+            a.b.ext_rays[nxt][es] = next_ray;
+            a.b.ext_path[nxt][es] = path;
+            a.b.ext_scene[nxt][es] = scene_slot;
+        }
+        const uint32_t ss = wave_append(shadow, &a.b.counters[kCntShadow], a.b.rays);
+        if (shadow) {
+            a.b.sh_rays[ss] = sh_ray;
+            a.b.sh_path[ss] = path;
+            a.b.sh_scene[ss] = scene_slot;
+            a.b.sh_contrib[ss] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, 0.f);
+        }
+    }
+}
+
+// One shadow ray per path per bounce, so the adds below never collide.
+__global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a)
+{
+    const uint32_t n = a.b.counters[kCntShadow];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (__float_as_uint(a.b.sh_hits[i].y) != 0xFFFFFFFFu) continue;
+        const uint32_t path = a.b.sh_path[i];
+        const float4 c = a.b.sh_contrib[i];
+        float4 L = a.b.p_L[path];
+        L.x += c.x; L.y += c.y; L.z += c.z;
+        a.b.p_L[path] = L;
+    }
+}
+
+// Per pixel, in increasing sample order: the thread of a pixel's first path in
+// the batch adds the batch's samples of that pixel one after another, so the
+// framebuffer sums are the same for any batch size or tile order.
+__global__ __launch_bounds__(256) void k_accumulate(WfArgs a, uint64_t g0, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t pixel = a.b.p_pixel[i];
+        if (pixel == 0xFFFFFFFFu) continue;
+        const JobDev& job = a.jobs[a.b.p_job[i]];
+        const uint64_t g = g0 + i;
+        const uint64_t lo = job.path_base > g0 ? job.path_base : g0;
+        if (g >= lo + job.padded) continue;  // not the first sample of this pixel in the batch
+        const uint64_t end = job.path_end < g0 + n ? job.path_end : g0 + n;
+        float* acc = job.accum + 3ull * pixel;
+        float r = acc[0], gr = acc[1], b = acc[2];
+        double r64 = 0, g64 = 0, b64 = 0;
+        double* acc64 = job.accum64 ? job.accum64 + 3ull * pixel : nullptr;
+        if (acc64) { r64 = acc64[0]; g64 = acc64[1]; b64 = acc64[2]; }
+        for (uint64_t h = g; h < end; h += job.padded) {
+            const float4 L = a.b.p_L[h - g0];
+            r += L.x; gr += L.y; b += L.z;
+            if (acc64) { r64 += double(L.x); g64 += double(L.y); b64 += double(L.z); }
+        }
+        acc[0] = r; acc[1] = gr; acc[2] = b;
+        if (acc64) { acc64[0] = r64; acc64[1] = g64; acc64[2] = b64; }
+    }
+}
+
+}  // namespace
+
+void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, g0, n);
+}
+
+void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, next);
+}
+
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t bounce, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a, cur, bounce);
+}
+
+void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, a, g0, n);
+}
+
+}  // namespace fnd

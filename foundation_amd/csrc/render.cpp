@@ -1,13 +1,165 @@
 // render.cpp — framebuffers and the batched wavefront render entry point.
+//
+// fnd_pt_render takes the place of the reference's per-frame raster pass
+// (Renderer::Record + Queue::Submit, src/Renderer/Renderer.cpp:312-366,386-393)
+// and, like Renderer::Draw, returns after the device has finished
+// (src/Renderer/Renderer.cpp:394).
+//
+// All jobs of one call share one sequence of launches: the global path space
+// is the concatenation over jobs of spp x padded pixels (sample-major, pixels
+// in T x T tile order); batches are consecutive ranges of it, so one batch may
+// hold paths of several scenes and resolutions.  Per batch: generate, then per
+// bounce extension trace -> shade -> shadow trace -> shadow resolve, then
+// accumulate.  Stage kernels read their queue lengths from device memory, so
+// the host never waits inside a render.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "ctx.hpp"
+#include "wavefront.hpp"
 
 using namespace fnd;
 
 namespace fnd {
-void render_state_free(fnd_pt_ctx* ctx) { (void)ctx; }
+
+struct RenderState {
+    uint32_t capacity = 0;
+    WfBuffers b{};
+    std::vector<std::pair<void*, uint64_t>> allocs;
+    JobDev* d_jobs = nullptr;
+    uint32_t jobs_cap = 0;
+    ShadeScene* d_shade = nullptr;
+    SceneDev* d_scenes = nullptr;
+    uint32_t scenes_cap = 0;
+};
+
+void render_state_free(fnd_pt_ctx* ctx)
+{
+    auto* rs = static_cast<RenderState*>(ctx->render_state);
+    if (!rs) return;
+    for (auto& a : rs->allocs) dev_free(ctx, a.first, a.second);
+    dev_free(ctx, rs->d_jobs, rs->jobs_cap * sizeof(JobDev));
+    dev_free(ctx, rs->d_shade, rs->scenes_cap * sizeof(ShadeScene));
+    dev_free(ctx, rs->d_scenes, rs->scenes_cap * sizeof(SceneDev));
+    delete rs;
+    ctx->render_state = nullptr;
+}
+
+namespace {
+
+template <class T>
+fnd_status alloc_into(fnd_pt_ctx* ctx, RenderState* rs, T** out, uint64_t count)
+{
+    void* p = nullptr;
+    const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+    fnd_status st = dev_alloc(ctx, &p, bytes);
+    if (st != FND_OK) return st;
+    rs->allocs.emplace_back(p, bytes);
+    *out = static_cast<T*>(p);
+    return FND_OK;
+}
+
+fnd_status ensure_capacity(fnd_pt_ctx* ctx, uint32_t cap)
+{
+    auto* rs = static_cast<RenderState*>(ctx->render_state);
+    if (rs && rs->capacity >= cap) return FND_OK;
+    uint32_t jobs_cap = 0, scenes_cap = 0;
+    JobDev* dj = nullptr;
+    ShadeScene* ds = nullptr;
+    SceneDev* dsc = nullptr;
+    if (rs) {  // keep the small tables, drop the per-path buffers
+        for (auto& a : rs->allocs) dev_free(ctx, a.first, a.second);
+        rs->allocs.clear();
+        jobs_cap = rs->jobs_cap; scenes_cap = rs->scenes_cap;
+        dj = rs->d_jobs; ds = rs->d_shade; dsc = rs->d_scenes;
+        delete rs;
+    }
+    rs = new (std::nothrow) RenderState();
+    if (!rs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+    rs->jobs_cap = jobs_cap; rs->scenes_cap = scenes_cap;
+    rs->d_jobs = dj; rs->d_shade = ds; rs->d_scenes = dsc;
+    ctx->render_state = rs;
+    WfBuffers& b = rs->b;
+    fnd_status st = FND_OK;
+    auto A = [&](auto** p, uint64_t n) {
+        if (st == FND_OK) st = alloc_into(ctx, rs, p, n);
+    };
+    A(&b.p_job, cap); A(&b.p_pixel, cap); A(&b.p_sample, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
+    for (int q = 0; q < 2; ++q) {
+        A(&b.ext_rays[q], cap); A(&b.ext_path[q], cap); A(&b.ext_scene[q], cap);
+    }
+    A(&b.ext_hits, cap);
+    A(&b.sh_rays, cap); A(&b.sh_path, cap); A(&b.sh_scene, cap); A(&b.sh_contrib, cap); A(&b.sh_hits, cap);
+    A(&b.counters, kCntCount);
+    A(&b.rays, 1);
+    if (st != FND_OK) {
+        render_state_free(ctx);
+        return st;
+    }
+    rs->capacity = cap;
+    st = check_hip(ctx, hipMemsetAsync(b.rays, 0, sizeof(unsigned long long), ctx->stream), "clear");
+    return st;
+}
+
+fnd_status ensure_tables(fnd_pt_ctx* ctx, uint32_t njobs, uint32_t nscenes)
+{
+    auto* rs = static_cast<RenderState*>(ctx->render_state);
+    if (njobs > rs->jobs_cap) {
+        dev_free(ctx, rs->d_jobs, rs->jobs_cap * sizeof(JobDev));
+        rs->d_jobs = nullptr;
+        rs->jobs_cap = 0;
+        void* p = nullptr;
+        const uint32_t cap = std::max(njobs, 64u);
+        fnd_status st = dev_alloc(ctx, &p, cap * sizeof(JobDev));
+        if (st != FND_OK) return st;
+        rs->d_jobs = static_cast<JobDev*>(p);
+        rs->jobs_cap = cap;
+    }
+    if (nscenes > rs->scenes_cap) {
+        dev_free(ctx, rs->d_shade, rs->scenes_cap * sizeof(ShadeScene));
+        dev_free(ctx, rs->d_scenes, rs->scenes_cap * sizeof(SceneDev));
+        rs->d_shade = nullptr;
+        rs->d_scenes = nullptr;
+        rs->scenes_cap = 0;
+        const uint32_t cap = std::max(nscenes, 64u);
+        void* p = nullptr;
+        fnd_status st = dev_alloc(ctx, &p, cap * sizeof(ShadeScene));
+        if (st != FND_OK) return st;
+        rs->d_shade = static_cast<ShadeScene*>(p);
+        st = dev_alloc(ctx, &p, cap * sizeof(SceneDev));
+        if (st != FND_OK) return st;
+        rs->d_scenes = static_cast<SceneDev*>(p);
+        rs->scenes_cap = cap;
+    }
+    return FND_OK;
+}
+
+void camera_basis(const fnd_camera& c, uint32_t w, uint32_t h, JobDev& j)
+{
+    double f[3], up[3] = {c.up[0], c.up[1], c.up[2]};
+    for (int i = 0; i < 3; ++i) f[i] = double(c.at[i]) - double(c.eye[i]);
+    auto norm = [](double* v) {
+        const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int i = 0; i < 3; ++i) v[i] /= l;
+    };
+    norm(f);
+    double r[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    norm(r);
+    const double u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+    for (int i = 0; i < 3; ++i) {
+        j.eye[i] = c.eye[i];
+        j.fwd[i] = float(f[i]);
+        j.right[i] = float(r[i]);
+        j.up[i] = float(u[i]);
+    }
+    j.tan_half = float(std::tan(double(c.fov_y_deg) * M_PI / 360.0));
+    j.aspect = float(double(w) / double(h));
+}
+
+}  // namespace
 }  // namespace fnd
 
 extern "C" {
@@ -115,7 +267,167 @@ fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle h, double* rg
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs)
 {
     if (!ctx || (!jobs && num_jobs)) return FND_E_INVALID_ARG;
-    return set_error(ctx, FND_E_UNSUPPORTED, "render is not built yet");
+    if (num_jobs == 0) return FND_OK;
+    const uint32_t T = std::max(1u, ctx->tuning.tile);
+    std::map<Scene*, uint32_t> slot_of;
+    std::vector<Scene*> slots;
+    std::vector<JobDev> hj(num_jobs);
+    std::vector<Framebuffer*> fbs(num_jobs);
+    uint64_t total = 0;
+    uint32_t max_depth = 0, max_stack = 0;
+    for (uint32_t i = 0; i < num_jobs; ++i) {
+        const fnd_render_job& in = jobs[i];
+        Scene* s = find_scene(ctx, in.scene);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown scene handle", i);
+        Framebuffer* fb = find_framebuffer(ctx, in.framebuffer);
+        if (!fb) return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown framebuffer handle", i);
+        for (uint32_t k = 0; k < i; ++k)
+            if (fbs[k] == fb)
+                return set_error(ctx, FND_E_INVALID_ARG, "jobs %u and %u accumulate into one framebuffer", k, i);
+        if (in.spp == 0 || in.max_depth > 64)
+            return set_error(ctx, FND_E_INVALID_ARG, "job %u: spp must be >= 1 and max_depth <= 64", i);
+        if (in.flags & ~(FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE))
+            return set_error(ctx, FND_E_INVALID_ARG, "job %u: unknown flags 0x%x", i, in.flags);
+        if (in.flags & (FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE))
+            return set_error(ctx, FND_E_UNSUPPORTED, "job %u: radiance cache flags need fnd_pt_cache_configure", i);
+        const float dx = in.camera.at[0] - in.camera.eye[0], dy = in.camera.at[1] - in.camera.eye[1],
+                    dz = in.camera.at[2] - in.camera.eye[2];
+        if (!(dx * dx + dy * dy + dz * dz > 0.0f) || !(in.camera.fov_y_deg > 0.0f && in.camera.fov_y_deg < 180.0f))
+            return set_error(ctx, FND_E_INVALID_ARG, "job %u: degenerate camera", i);
+        if (uint64_t(in.sample_offset) + in.spp > 0xFFFFFFFFull)
+            return set_error(ctx, FND_E_INVALID_ARG, "job %u: sample index overflows 32 bits", i);
+        auto it = slot_of.find(s);
+        if (it == slot_of.end()) {
+            it = slot_of.emplace(s, uint32_t(slots.size())).first;
+            slots.push_back(s);
+        }
+        JobDev& j = hj[i];
+        std::memset(&j, 0, sizeof(j));
+        j.scene_slot = it->second;
+        j.width = fb->width;
+        j.height = fb->height;
+        j.tile = T;
+        j.tiles_x = (fb->width + T - 1) / T;
+        const uint64_t padded = uint64_t(j.tiles_x) * ((fb->height + T - 1) / T) * T * T;
+        if (padded >= (1ull << 32)) return set_error(ctx, FND_E_INVALID_ARG, "job %u: image too large", i);
+        j.padded = uint32_t(padded);
+        j.max_depth = in.max_depth;
+        j.rr_depth = in.rr_depth;
+        j.sample_offset = in.sample_offset;
+        j.flags = in.flags;
+        j.seed = in.seed;
+        j.path_base = total;
+        total += padded * in.spp;
+        j.path_end = total;
+        camera_basis(in.camera, fb->width, fb->height, j);
+        j.accum = fb->accum;
+        j.accum64 = fb->accum64;
+        fbs[i] = fb;
+        max_depth = std::max(max_depth, in.max_depth);
+        max_stack = std::max(max_stack, s->info.stack_entries);
+    }
+    const uint32_t cap = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
+    fnd_status st = ensure_capacity(ctx, cap);
+    if (st != FND_OK) return st;
+    st = ensure_tables(ctx, num_jobs, uint32_t(slots.size()));
+    if (st != FND_OK) return st;
+    auto* rs = static_cast<RenderState*>(ctx->render_state);
+    std::vector<ShadeScene> hs(slots.size());
+    std::vector<SceneDev> hd(slots.size());
+    for (size_t k = 0; k < slots.size(); ++k) {
+        const Scene* s = slots[k];
+        hd[k] = SceneDev{s->nodes, s->tris, s->insts, s->root, s->two_level ? 1u : 0u};
+        ShadeScene& z = hs[k];
+        z.tris = s->tris;
+        z.insts = s->insts;
+        z.materials = s->materials;
+        z.lights = s->lights;
+        z.num_lights = s->num_lights;
+        z.two_level = s->two_level ? 1u : 0u;
+        std::memcpy(z.background, s->background, 12);
+        z.pad = 0;
+    }
+    st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice, ctx->stream),
+                   "upload jobs");
+    if (st == FND_OK)
+        st = check_hip(ctx, hipMemcpyAsync(rs->d_shade, hs.data(), hs.size() * sizeof(ShadeScene), hipMemcpyHostToDevice,
+                                           ctx->stream), "upload scenes");
+    if (st == FND_OK)
+        st = check_hip(ctx, hipMemcpyAsync(rs->d_scenes, hd.data(), hd.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
+                                           ctx->stream), "upload scenes");
+    if (st != FND_OK) return st;
+
+    TraceLaunch tl{};
+    tl.two_level = true;
+    tl.multi = true;
+    tl.block = ctx->tuning.block_size;
+    tl.stack_lds = ctx->tuning.stack_lds;
+    tl.grid = trace_grid(ctx, tl.block);
+    st = ensure_spill(ctx, max_stack, tl.stack_lds, tl.grid * tl.block);
+    if (st != FND_OK) return st;
+
+    WfArgs wa{};
+    wa.jobs = rs->d_jobs;
+    wa.num_jobs = num_jobs;
+    wa.scenes = rs->d_shade;
+    wa.b = rs->b;
+    const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
+    st = check_hip(ctx, hipMemsetAsync(rs->b.rays, 0, sizeof(unsigned long long), ctx->stream), "clear");
+    if (st != FND_OK) return st;
+    (void)hipEventRecord(ctx->ev[2], ctx->stream);
+    for (uint64_t g0 = 0; g0 < total; g0 += cap) {
+        const uint32_t n = uint32_t(std::min<uint64_t>(cap, total - g0));
+        const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
+        launch_prep(wa, kCntExtA, ctx->stream);
+        launch_generate(wa, g0, n, grid, ctx->stream);
+        uint32_t cur = kCntExtA;
+        for (uint32_t bounce = 0; bounce <= max_depth; ++bounce) {
+            const uint32_t nxt = cur ^ 1u;
+            launch_prep(wa, nxt, ctx->stream);
+            TraceArgs ta{};
+            ta.scenes = rs->d_scenes;
+            ta.ray_scene = rs->b.ext_scene[cur];
+            ta.rays = rs->b.ext_rays[cur];
+            ta.hits = rs->b.ext_hits;
+            ta.nrays_dev = &rs->b.counters[cur];
+            ta.counter = &rs->b.counters[kCntWorkExt];
+            ta.spill = ctx->d_spill;
+            ta.spill_stride = tl.grid * tl.block;
+            ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
+            tl.mode = kModeClosestInternal;
+            st = check_hip(ctx, launch_trace(tl, ta, ctx->stream), "extension trace");
+            if (st != FND_OK) return st;
+            launch_shade(wa, cur, bounce, grid, ctx->stream);
+            ta.ray_scene = rs->b.sh_scene;
+            ta.rays = rs->b.sh_rays;
+            ta.hits = rs->b.sh_hits;
+            ta.nrays_dev = &rs->b.counters[kCntShadow];
+            ta.counter = &rs->b.counters[kCntWorkShadow];
+            tl.mode = kModeAnyInternal;
+            st = check_hip(ctx, launch_trace(tl, ta, ctx->stream), "shadow trace");
+            if (st != FND_OK) return st;
+            launch_shadow_resolve(wa, grid, ctx->stream);
+            cur = nxt;
+        }
+        launch_accumulate(wa, g0, n, grid, ctx->stream);
+        st = check_hip(ctx, hipGetLastError(), "render launch");
+        if (st != FND_OK) return st;
+    }
+    (void)hipEventRecord(ctx->ev[3], ctx->stream);
+    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "render");
+    if (st != FND_OK) return st;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+    ctx->stats.last_call_ms = ms;
+    unsigned long long rays = 0;
+    st = check_hip(ctx, hipMemcpyAsync(&rays, rs->b.rays, sizeof rays, hipMemcpyDeviceToHost, ctx->stream),
+                   "read ray count");
+    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read ray count");
+    if (st != FND_OK) return st;
+    ctx->stats.render_rays += rays;
+    ctx->stats.paths += total;
+    for (uint32_t i = 0; i < num_jobs; ++i) fbs[i]->samples += jobs[i].spp;
+    return FND_OK;
 }
 
 }  // extern "C"

@@ -381,7 +381,7 @@ void gen_instanced(fnd_generated_scene& s, const fnd_gen_params& p)
         for (auto& th : pool) th.join();
     }
     Pcg rng(p.seed, 0x1257u);
-    const double R = 6.0;
+    const double R = 4.5;
     for (uint32_t i = 0; i < ninst; ++i) {
         MeshData& m = s.meshes[i];
         const bool ggx = rng.uniform() * 100.0 < double(p.ggx_fraction_pct);
@@ -424,7 +424,7 @@ void gen_instanced(fnd_generated_scene& s, const fnd_gen_params& p)
     li.object_to_world[0] = li.object_to_world[5] = li.object_to_world[10] = 1.0f;
     li.world_to_object[0] = li.world_to_object[5] = li.world_to_object[10] = 1.0f;
     s.instances.push_back(li);
-    set_camera(s.camera, 14.0f, 14.0f, 14.0f, 0.0f, 0.0f, 0.0f);
+    set_camera(s.camera, 8.0f, 8.0f, 6.0f, 0.0f, 0.0f, 0.0f);
 }
 
 void gen_furnace(fnd_generated_scene& s, const fnd_gen_params& p)

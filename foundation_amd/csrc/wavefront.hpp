@@ -1,0 +1,91 @@
+// wavefront.hpp — device-side tables and buffers of the wavefront path tracer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pt_layout.hpp"
+#include "scene_encode.hpp"
+
+namespace fnd {
+
+struct JobDev {
+    uint32_t scene_slot;
+    uint32_t width, height;
+    uint32_t tile;            // pixel order: T x T tiles, row-major tiles
+    uint32_t tiles_x;
+    uint32_t padded;          // tiles_x * tiles_y * T * T paths per sample
+    uint32_t max_depth;
+    uint32_t rr_depth;
+    uint32_t sample_offset;
+    uint32_t flags;
+    uint64_t seed;
+    uint64_t path_base;       // first global path index of this job in the render call
+    uint64_t path_end;
+    float eye[3];
+    float tan_half;
+    float fwd[3];
+    float aspect;
+    float right[3];
+    float pad0;
+    float up[3];
+    float pad1;
+    float* accum;             // W*H*3
+    double* accum64;          // W*H*3 or null
+};
+
+struct ShadeScene {
+    const float4* tris;
+    const InstanceRec* insts;
+    const DeviceMaterial* materials;
+    const DeviceLight* lights;
+    uint32_t num_lights;
+    uint32_t two_level;
+    float background[3];
+    float pad;
+};
+
+// Counter slots (u32) of one render call.
+enum WfCounter : uint32_t {
+    kCntExtA = 0,
+    kCntExtB = 1,
+    kCntShadow = 2,
+    kCntWorkExt = 3,
+    kCntWorkShadow = 4,
+    kCntCount = 8
+};
+
+struct WfBuffers {
+    // per path slot
+    uint32_t* p_job;
+    uint32_t* p_pixel;        // y * W + x, or 0xFFFFFFFF for padding
+    uint32_t* p_sample;
+    float4* p_thr;            // throughput, w = solid-angle pdf of the last BSDF sample (0: camera)
+    float4* p_L;              // radiance
+    // extension queues (double-buffered) and hits
+    Ray* ext_rays[2];
+    uint32_t* ext_path[2];
+    uint32_t* ext_scene[2];
+    float4* ext_hits;
+    // shadow queue
+    Ray* sh_rays;
+    uint32_t* sh_path;
+    uint32_t* sh_scene;
+    float4* sh_contrib;
+    float4* sh_hits;
+    uint32_t* counters;       // kCntCount
+    unsigned long long* rays; // rays appended (extension + shadow), cumulative
+};
+
+struct WfArgs {
+    const JobDev* jobs;
+    uint32_t num_jobs;
+    const ShadeScene* scenes;
+    WfBuffers b;
+};
+
+void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
+void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s);
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t bounce, uint32_t grid, hipStream_t s);
+void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);
+void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
+
+}  // namespace fnd

@@ -95,6 +95,30 @@ class GeneratedScene:
         return sum(self.desc.meshes[self.desc.instances[i].mesh].num_triangles
                    for i in range(self.desc.num_instances))
 
+    def triangle_lookup(self):
+        """Callable prim ids -> (n, 3, 3) float64 world-space vertices, without
+        expanding the whole scene."""
+        ms = self.meshes()
+        if self.desc.num_instances == 0:
+            m = ms[0]
+            return lambda prims: m["positions"][m["indices"][prims]].astype(np.float64)
+        insts = self.instances()
+        counts = np.array([ms[i["mesh"]]["indices"].shape[0] for i in insts], np.int64)
+        base = np.concatenate([[0], np.cumsum(counts)[:-1]])
+
+        def lookup(prims):
+            prims = np.asarray(prims, np.int64)
+            inst = np.searchsorted(base, prims, side="right") - 1
+            out = np.zeros((prims.size, 3, 3), np.float64)
+            for k in np.unique(inst):
+                sel = np.nonzero(inst == k)[0]
+                m = ms[insts[k]["mesh"]]
+                v = m["positions"][m["indices"][prims[sel] - base[k]]].astype(np.float64)
+                o2w = insts[k]["o2w"].astype(np.float64)
+                out[sel] = v @ o2w[:, :3].T + o2w[:, 3]
+            return out
+        return lookup
+
     def world_triangles(self):
         """(P, 3, 3) float64 world-space vertices indexed by primitive id."""
         ms = self.meshes()

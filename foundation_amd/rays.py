@@ -86,14 +86,15 @@ def onb_duff(n):
 
 def hit_frames(rays, hits, world_tris):
     """For the rays that hit: indices, hit points, unit geometric normals
-    facing the incoming ray (float64)."""
+    facing the incoming ray (float64).  world_tris: (P,3,3) array or a
+    callable prim ids -> (n,3,3)."""
     prim = hits[:, 1].view(np.uint32)
     idx = np.nonzero(prim != 0xFFFFFFFF)[0]
     o = rays[idx, 0:3].astype(np.float64)
     d = rays[idx, 4:7].astype(np.float64)
     t = hits[idx, 0].astype(np.float64)
     p = o + t[:, None] * d
-    v = world_tris[prim[idx]]
+    v = world_tris(prim[idx]) if callable(world_tris) else world_tris[prim[idx]]
     n = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
     n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-300)
     flip = np.sum(n * d, axis=1) > 0

@@ -208,6 +208,8 @@ typedef struct fnd_render_job {
 typedef struct fnd_pt_stats {
     double last_kernel_ms;          /* device time of the last trace launch (HIP events on the context stream) */
     double last_call_ms;            /* device time of the whole last trace/render call */
+    double trace_kernel_ms_total;   /* cumulative device time of trace launches, resolved at each wait */
+    uint64_t trace_launches;        /* trace launches whose time is in trace_kernel_ms_total */
     uint64_t rays_traced;           /* cumulative */
     uint64_t node_visits;           /* cumulative, only with FND_TRACE_COUNT */
     uint64_t tri_tests;             /* cumulative, only with FND_TRACE_COUNT */

@@ -12,6 +12,7 @@
  */
 #define _GNU_SOURCE
 #include "pt_oracle.h"
+#include "pt_oracle_internal.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -163,43 +164,6 @@ static void xform_ray(const float* m, const float* o, const float* d, float* oo,
 }
 
 /* ------------------------------------------------------------------ BVH */
-
-typedef struct onode {
-    float lo[3], hi[3];
-    uint32_t first; /* internal: left child (right = first + 1); leaf: first entry in prim_idx */
-    uint32_t count; /* 0 = internal */
-} onode;
-
-typedef struct obvh {
-    onode* nodes;
-    uint32_t num_nodes;
-    uint32_t* prim_idx;
-    uint32_t num_prims;
-    uint32_t max_depth;
-} obvh;
-
-typedef struct omesh {
-    float* tris; /* 9 floats per triangle */
-    uint32_t num_tris;
-    obvh bvh;
-} omesh;
-
-typedef struct oinst {
-    uint32_t mesh;
-    float o2w[12];
-    float w2o[12];
-    uint32_t id_base;
-    float lo[3], hi[3];
-} oinst;
-
-struct orc_scene {
-    omesh* meshes;
-    uint32_t num_meshes, cap_meshes;
-    oinst* insts;
-    uint32_t num_insts, cap_insts;
-    obvh tlas;
-    int built;
-};
 
 typedef struct obox { float lo[3], hi[3]; } obox;
 
@@ -365,10 +329,14 @@ void orc_scene_free(orc_scene* s)
     if (!s) return;
     for (uint32_t i = 0; i < s->num_meshes; ++i) {
         free(s->meshes[i].tris);
+        free(s->meshes[i].mat);
+        free(s->meshes[i].light);
         obvh_free(&s->meshes[i].bvh);
     }
     free(s->meshes);
     free(s->insts);
+    free(s->materials);
+    free(s->lights);
     obvh_free(&s->tlas);
     free(s);
 }
@@ -811,4 +779,155 @@ void orc_trace(const orc_scene* s, const float* rays, uint32_t n, int anyhit, in
     for (int i = 1; i < nthreads; ++i) pthread_create(&th[i], NULL, trace_worker, &j);
     trace_worker(&j);
     for (int i = 1; i < nthreads; ++i) pthread_join(th[i], NULL);
+}
+
+/* ------------------------------------------------------------------ shading data */
+
+int orc_scene_set_mesh_attrs(orc_scene* s, uint32_t mesh, const uint32_t* mat_ids, const int32_t* light_ids)
+{
+    if (!s || mesh >= s->num_meshes) return -1;
+    omesh* m = &s->meshes[mesh];
+    free(m->mat);
+    free(m->light);
+    m->mat = NULL;
+    m->light = NULL;
+    if (mat_ids) {
+        m->mat = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(m->num_tris ? m->num_tris : 1));
+        if (!m->mat) return -1;
+        memcpy(m->mat, mat_ids, sizeof(uint32_t) * (size_t)m->num_tris);
+    }
+    if (light_ids) {
+        m->light = (int32_t*)malloc(sizeof(int32_t) * (size_t)(m->num_tris ? m->num_tris : 1));
+        if (!m->light) return -1;
+        memcpy(m->light, light_ids, sizeof(int32_t) * (size_t)m->num_tris);
+    }
+    return 0;
+}
+
+/* 8 floats per material as fnd_material: type (as uint32 bits), albedo[3],
+ * roughness, emission[3].  Roughness is clamped to [0.02, 1]. */
+int orc_scene_set_materials(orc_scene* s, const float* mats8, uint32_t n)
+{
+    if (!s || (!mats8 && n)) return -1;
+    free(s->materials);
+    s->materials = (orc_material*)calloc(n ? n : 1, sizeof(orc_material));
+    if (!s->materials) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        orc_material* m = &s->materials[i];
+        memcpy(&m->type, &mats8[8 * (size_t)i], 4);
+        for (int c = 0; c < 3; ++c) {
+            m->albedo[c] = mats8[8 * (size_t)i + 1 + c];
+            m->emission[c] = mats8[8 * (size_t)i + 5 + c];
+        }
+        m->roughness = fminf(1.0f, fmaxf(0.02f, mats8[8 * (size_t)i + 4]));
+    }
+    s->num_materials = n;
+    return 0;
+}
+
+/* 12 floats per light as fnd_quad_light: corner, edge0, edge1, radiance.
+ * Lights are chosen in proportion to mean radiance times area. */
+int orc_scene_set_lights(orc_scene* s, const float* l12, uint32_t n)
+{
+    if (!s || (!l12 && n)) return -1;
+    free(s->lights);
+    s->lights = (orc_light*)calloc(n ? n : 1, sizeof(orc_light));
+    if (!s->lights) return -1;
+    double total = 0.0;
+    double* pw = (double*)malloc(sizeof(double) * (n ? n : 1));
+    if (!pw) return -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        orc_light* L = &s->lights[i];
+        const float* q = l12 + 12 * (size_t)i;
+        memcpy(L->corner, q, 12);
+        memcpy(L->e0, q + 3, 12);
+        memcpy(L->e1, q + 6, 12);
+        memcpy(L->radiance, q + 9, 12);
+        const double cx = (double)L->e0[1] * L->e1[2] - (double)L->e0[2] * L->e1[1];
+        const double cy = (double)L->e0[2] * L->e1[0] - (double)L->e0[0] * L->e1[2];
+        const double cz = (double)L->e0[0] * L->e1[1] - (double)L->e0[1] * L->e1[0];
+        const double a = sqrt(cx * cx + cy * cy + cz * cz);
+        if (!(a > 0.0)) { free(pw); return -1; }
+        L->area = (float)a;
+        L->n[0] = (float)(cx / a); L->n[1] = (float)(cy / a); L->n[2] = (float)(cz / a);
+        pw[i] = (((double)L->radiance[0] + L->radiance[1] + L->radiance[2]) / 3.0) * a;
+        if (pw[i] < 1e-12) pw[i] = 1e-12;
+        total += pw[i];
+    }
+    double acc = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        acc += pw[i];
+        s->lights[i].cdf = (i + 1 == n) ? 1.0f : (float)(acc / total);
+        s->lights[i].pdf_select = (float)(pw[i] / total);
+    }
+    free(pw);
+    s->num_lights = n;
+    return 0;
+}
+
+void orc_scene_set_background(orc_scene* s, const float* rgb)
+{
+    if (s && rgb) memcpy(s->background, rgb, 12);
+}
+
+/* One ray through the trees with stacks on the C stack (depth-checked). */
+static int trace_core(const orc_scene* s, const float* o, const float* d, float tmax, int anyhit, obest* b)
+{
+    uint32_t md = 1;
+    for (uint32_t i = 0; i < s->num_meshes; ++i)
+        if (s->meshes[i].bvh.max_depth > md) md = s->meshes[i].bvh.max_depth;
+    if (md + 4 > 512 || s->tlas.max_depth + 4 > 512) return -1;
+    uint32_t sn[512], tn[512];
+    float st[512], tt[512];
+    float ray8[8] = {o[0], o[1], o[2], 0.0f, d[0], d[1], d[2], tmax};
+    float hit4[4];
+    (void)b;
+    trace_one(s, ray8, anyhit, 0, hit4, NULL, sn, st, tn, tt);
+    uint32_t id;
+    memcpy(&id, &hit4[1], 4);
+    b->t = hit4[0];
+    b->id = id;
+    return id != ORC_NO_HIT;
+}
+
+int orc_closest_shade(const orc_scene* s, v3f_in o, v3f_in d, orc_shade_hit* out)
+{
+    obest b;
+    const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    if (trace_core(s, of, df, 3.0e38f, 0, &b) != 1) return 0;
+    uint32_t mesh = 0, local = b.id;
+    const float* w2o = NULL;
+    if (s->num_insts) {
+        uint32_t lo = 0, hi = s->num_insts;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (s->insts[mid].id_base <= b.id) lo = mid; else hi = mid;
+        }
+        mesh = s->insts[lo].mesh;
+        local = b.id - s->insts[lo].id_base;
+        w2o = s->insts[lo].w2o;
+    }
+    const omesh* m = &s->meshes[mesh];
+    const float* v = m->tris + 9 * (size_t)local;
+    const float e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+    const float e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+    float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    if (w2o) {
+        const float nn[3] = {w2o[0] * n[0] + w2o[4] * n[1] + w2o[8] * n[2],
+                             w2o[1] * n[0] + w2o[5] * n[1] + w2o[9] * n[2],
+                             w2o[2] * n[0] + w2o[6] * n[1] + w2o[10] * n[2]};
+        memcpy(n, nn, sizeof n);
+    }
+    out->t = b.t;
+    memcpy(out->n, n, sizeof n);
+    out->material = m->mat ? m->mat[local] : 0u;
+    out->light = m->light ? m->light[local] : -1;
+    return 1;
+}
+
+int orc_occluded(const orc_scene* s, v3f_in o, v3f_in d, float tmax)
+{
+    obest b;
+    const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
+    return trace_core(s, of, df, tmax, 1, &b) == 1;
 }

@@ -83,6 +83,27 @@ uint64_t orc_scene_num_prims(const orc_scene* s);
 void orc_trace(const orc_scene* s, const float* rays, uint32_t n, int anyhit, int brute, int nthreads,
                float* hits, uint32_t* counts);
 
+/* Shading data for orc_render.  Materials: 8 floats each laid out as
+ * fnd_material (type as uint32 bits, albedo[3], roughness, emission[3]);
+ * lights: 12 floats each as fnd_quad_light. */
+int orc_scene_set_mesh_attrs(orc_scene* s, uint32_t mesh, const uint32_t* mat_ids, const int32_t* light_ids);
+int orc_scene_set_materials(orc_scene* s, const float* mats8, uint32_t n);
+int orc_scene_set_lights(orc_scene* s, const float* lights12, uint32_t n);
+void orc_scene_set_background(orc_scene* s, const float* rgb);
+
+/* Adds spp samples per pixel to accum (W*H*3 running sums, row 0 at the top,
+ * samples added in order) and to accum64 if not NULL.  cam10 = eye[3], at[3],
+ * up[3], fov_y_deg.  mode 0 = MIS (next-event + BSDF), 1 = BSDF sampling
+ * only, 2 = next-event only for light sources.  Returns rays traced. */
+uint64_t orc_render(const orc_scene* s, const float* cam10, uint32_t width, uint32_t height, uint32_t spp,
+                    uint32_t sample_offset, uint32_t max_depth, uint32_t rr_depth, uint64_t seed, uint32_t mode,
+                    int nthreads, float* accum, double* accum64);
+
+/* Local-frame BSDF sampling / evaluation (type 0 Lambert, 1 GGX), for the
+ * sampling tests.  out7 = wi[3], weight[3], pdf; out4 = f*cos[3], pdf. */
+int orc_bsdf_sample(int type, const float* albedo3, float rough, const float* wo3, float u1, float u2, float* out7);
+void orc_bsdf_eval(int type, const float* albedo3, float rough, const float* wo3, const float* wi3, float* out4);
+
 /* Host transcription of the GPU walk over the encoded wide-BVH bytes
  * (fnd_scene_encode output: 64-byte nodes, 48-byte leaf triangles, 64-byte
  * instance records).  counts, if not NULL, receives per ray (node visits,

@@ -45,6 +45,21 @@ def load():
     lib.orc_trace.restype = None
     lib.orc_trace_wide.argtypes = [P, P, P, C.c_uint32, C.c_int, C.c_uint32, P, C.c_uint32, C.c_int, C.c_int, P, P]
     lib.orc_trace_wide.restype = None
+    lib.orc_scene_set_mesh_attrs.argtypes = [P, C.c_uint32, P, P]
+    lib.orc_scene_set_mesh_attrs.restype = C.c_int
+    lib.orc_scene_set_materials.argtypes = [P, P, C.c_uint32]
+    lib.orc_scene_set_materials.restype = C.c_int
+    lib.orc_scene_set_lights.argtypes = [P, P, C.c_uint32]
+    lib.orc_scene_set_lights.restype = C.c_int
+    lib.orc_scene_set_background.argtypes = [P, P]
+    lib.orc_scene_set_background.restype = None
+    lib.orc_render.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                               C.c_uint64, C.c_uint32, C.c_int, P, P]
+    lib.orc_render.restype = C.c_uint64
+    lib.orc_bsdf_sample.argtypes = [C.c_int, P, C.c_float, P, C.c_float, C.c_float, P]
+    lib.orc_bsdf_sample.restype = C.c_int
+    lib.orc_bsdf_eval.argtypes = [C.c_int, P, C.c_float, P, P, P]
+    lib.orc_bsdf_eval.restype = None
     lib.orc_wide_max_stack.argtypes = [C.c_int]
     lib.orc_wide_max_stack.restype = C.c_uint32
     _lib = lib
@@ -109,6 +124,35 @@ class OracleScene:
     def num_prims(self):
         return self.lib.orc_scene_num_prims(self.h)
 
+    def set_mesh_attrs(self, mesh, material, light):
+        m = np.ascontiguousarray(material, np.uint32)
+        li = np.ascontiguousarray(light, np.int32)
+        assert self.lib.orc_scene_set_mesh_attrs(self.h, mesh, m.ctypes.data, li.ctypes.data) == 0
+
+    def set_materials(self, mats8):
+        a = np.ascontiguousarray(mats8, np.float32).reshape(-1, 8)
+        assert self.lib.orc_scene_set_materials(self.h, a.ctypes.data, a.shape[0]) == 0
+
+    def set_lights(self, lights12):
+        a = np.ascontiguousarray(lights12, np.float32).reshape(-1, 12)
+        assert self.lib.orc_scene_set_lights(self.h, a.ctypes.data if a.size else None, a.shape[0]) == 0
+
+    def set_background(self, rgb):
+        a = np.ascontiguousarray(rgb, np.float32)
+        self.lib.orc_scene_set_background(self.h, a.ctypes.data)
+
+    def render(self, cam10, width, height, spp, max_depth, rr_depth=None, seed=1, sample_offset=0, mode=0,
+               threads=0, accum=None, fp64=False):
+        """Adds spp samples per pixel; returns (sum image (H,W,3) float32,
+        sum image float64 or None, rays traced)."""
+        cam = np.ascontiguousarray(cam10, np.float32)
+        acc = np.zeros((height, width, 3), np.float32) if accum is None else accum
+        acc64 = np.zeros((height, width, 3), np.float64) if fp64 else None
+        rays = self.lib.orc_render(self.h, cam.ctypes.data, width, height, spp, sample_offset, max_depth,
+                                   max_depth if rr_depth is None else rr_depth, seed, mode, threads,
+                                   acc.ctypes.data, acc64.ctypes.data if fp64 else None)
+        return acc, acc64, rays
+
     def trace(self, rays, anyhit=False, brute=False, threads=0, counts=False):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         hits = np.zeros((rays.shape[0], 4), np.float32)
@@ -118,16 +162,39 @@ class OracleScene:
         return (hits, cnt) if counts else hits
 
 
-def scene_from_generated(gen, leaf_max=4, bins=16, threads=0):
+def scene_from_generated(gen, leaf_max=4, bins=16, threads=0, shading=False):
     """Oracle scene holding the same triangles and transforms as a
-    foundation_amd.pt.GeneratedScene."""
+    foundation_amd.pt.GeneratedScene (and its materials and lights if
+    shading)."""
     s = OracleScene()
     meshes = gen.meshes()
-    for m in meshes:
+    for i, m in enumerate(meshes):
         s.add_mesh(m["positions"][m["indices"]].reshape(-1, 9))
+        if shading:
+            s.set_mesh_attrs(i, m["material"], m["light"])
     for inst in gen.instances():
         s.add_instance(inst["mesh"], inst["o2w"], inst["w2o"])
+    if shading:
+        d = gen.desc
+        mats = np.zeros((max(1, d.num_materials), 8), np.float32)
+        for k in range(d.num_materials):
+            mt = d.materials[k]
+            mats[k, 0] = np.array([mt.type], np.uint32).view(np.float32)[0]
+            mats[k, 1:4] = mt.albedo[:]
+            mats[k, 4] = mt.roughness
+            mats[k, 5:8] = mt.emission[:]
+        s.set_materials(mats)
+        lights = np.zeros((d.num_lights, 12), np.float32)
+        for k in range(d.num_lights):
+            L = d.lights[k]
+            lights[k] = list(L.corner) + list(L.edge0) + list(L.edge1) + list(L.radiance)
+        s.set_lights(lights)
+        s.set_background(np.array(d.background[:], np.float32))
     return s.build(leaf_max=leaf_max, bins=bins, threads=threads)
+
+
+def camera10(cam):
+    return np.array(list(cam.eye) + list(cam.at) + list(cam.up) + [cam.fov_y_deg], np.float32)
 
 
 def trace_wide(encoded, rays, anyhit=False, threads=0):

@@ -1,0 +1,157 @@
+"""GPU wavefront path tracer vs closed forms and the CPU oracle.
+
+* Furnace: inside a closed Lambert sphere of albedo a and emission e (no light
+  sampling), every path of depth D returns e * sum_{k<=D} a^k.  With a = 0.5
+  every partial sum is exact in FP32, so the image must equal it bit for bit.
+* Cornell box + soup (the CPU configuration, reduced in size): the GPU and the
+  oracle use the same random-number keys per (seed, sample, pixel, bounce) and
+  the same estimator, so their images agree far better than two independent
+  renders do; the test bounds relMSE(GPU, oracle) by a fraction of the
+  seed-to-seed relMSE of the GPU itself.
+* Images do not depend on queue capacity, tile order, batching with other
+  jobs, or splitting the samples over several calls (bitwise).
+"""
+import numpy as np
+import pytest
+
+from foundation_amd import _abi as A
+from foundation_amd import pt
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_mse(x, y):
+    return float(np.mean((x.astype(np.float64) - y) ** 2 / (y.astype(np.float64) ** 2 + 0.01)))
+
+
+def render(ctx, scene, cam, w, h, spp, depth, rr=None, seed=1, offset=0, fb=None, fp64=False):
+    own = fb is None
+    if own:
+        fb = ctx.framebuffer_create(w, h, fp64=fp64)
+    ctx.render([pt.make_job(scene, fb, cam, spp, depth, rr_depth=rr, seed=seed, sample_offset=offset)])
+    img, n = ctx.framebuffer_read(fb, w, h)
+    img64 = ctx.framebuffer_read(fb, w, h, fp64=True)[0] if fp64 else None
+    if own:
+        ctx.framebuffer_destroy(fb)
+    return img, img64
+
+
+@pytest.fixture(scope="module")
+def furnace(ctx):
+    g = pt.GeneratedScene(A.GEN_FURNACE, subdivision=8)
+    s = ctx.scene_create(g.desc)
+    yield g, s
+    ctx.scene_destroy(s)
+
+
+@pytest.fixture(scope="module")
+def cornell(ctx):
+    g = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    s = ctx.scene_create(g.desc)
+    yield g, s
+    ctx.scene_destroy(s)
+
+
+@pytest.mark.parametrize("depth", [0, 1, 3, 6])
+def test_furnace_is_exact(ctx, furnace, depth):
+    g, s = furnace
+    img, _ = render(ctx, s, g.camera, 48, 32, 4, depth)
+    expect = np.float32(2.0 - 0.5 ** depth)
+    assert np.mean(img == expect) > 0.999, (img.min(), img.max(), expect)
+
+
+def test_furnace_with_russian_roulette_is_unbiased(ctx, furnace):
+    g, s = furnace
+    img, _ = render(ctx, s, g.camera, 64, 48, 64, 8, rr=1, seed=5)
+    expect = 2.0 - 0.5 ** 8
+    se = img.std() / np.sqrt(img.size / 3)
+    assert abs(img.mean() - expect) < 5 * se + 1e-4, (img.mean(), expect, se)
+
+
+def test_cornell_matches_oracle(ctx, cornell, oracle):
+    g, s = cornell
+    w = h = 64
+    gpu1, _ = render(ctx, s, g.camera, w, h, 16, 4, seed=1)
+    gpu2, _ = render(ctx, s, g.camera, w, h, 16, 4, seed=2)
+    osc = oracle.scene_from_generated(g, shading=True)
+    acc, _, _ = osc.render(oracle.camera10(g.camera), w, h, 16, 4, seed=1)
+    ref = acc / 16
+    noise = rel_mse(gpu1, gpu2)
+    diff = rel_mse(gpu1, ref)
+    print(f"relMSE gpu/oracle {diff:.3e}, seed-to-seed {noise:.3e}", flush=True)
+    assert noise > 1e-4
+    assert diff < 0.2 * noise
+    assert abs(gpu1.mean() - ref.mean()) < 0.01 * ref.mean()
+
+
+def test_images_do_not_depend_on_queue_or_tiles(ctx, cornell):
+    g, s = cornell
+    base, _ = render(ctx, s, g.camera, 40, 24, 4, 4, seed=3)
+    old = ctx.tuning()
+    try:
+        for cap, tile in [(1024, 8), (3000, 4), (1 << 20, 16), (1 << 20, 1)]:
+            ctx.set_tuning(queue_capacity=cap, tile=tile)
+            img, _ = render(ctx, s, g.camera, 40, 24, 4, 4, seed=3)
+            assert np.array_equal(img.view(np.uint32), base.view(np.uint32)), (cap, tile)
+    finally:
+        ctx.set_tuning(queue_capacity=old.queue_capacity, tile=old.tile)
+
+
+def test_progressive_equals_single_call(ctx, cornell):
+    g, s = cornell
+    one, _ = render(ctx, s, g.camera, 32, 32, 8, 3, seed=4)
+    fb = ctx.framebuffer_create(32, 32)
+    ctx.render([pt.make_job(s, fb, g.camera, 3, 3, seed=4, sample_offset=0)])
+    ctx.render([pt.make_job(s, fb, g.camera, 5, 3, seed=4, sample_offset=3)])
+    two, n = ctx.framebuffer_read(fb, 32, 32)
+    ctx.framebuffer_destroy(fb)
+    assert n == 8
+    assert np.array_equal(one.view(np.uint32), two.view(np.uint32))
+
+
+def test_batched_jobs_equal_jobs_alone(ctx, cornell, furnace):
+    gc, sc = cornell
+    gf, sf = furnace
+    gi = pt.GeneratedScene(A.GEN_INSTANCED_SPHERES, instances=16, subdivision=12)
+    si = ctx.scene_create(gi.desc)
+    try:
+        specs = [(sc, gc.camera, 48, 32, 4, 4, 11), (sf, gf.camera, 16, 16, 2, 3, 12),
+                 (si, gi.camera, 64, 40, 3, 5, 13), (sc, gc.camera, 20, 36, 5, 2, 14)]
+        alone = [render(ctx, s, cam, w, h, spp, d, seed=seed)[0] for s, cam, w, h, spp, d, seed in specs]
+        fbs = [ctx.framebuffer_create(w, h) for _, _, w, h, _, _, _ in specs]
+        jobs = [pt.make_job(s, fb, cam, spp, d, seed=seed) for (s, cam, w, h, spp, d, seed), fb in zip(specs, fbs)]
+        old = ctx.tuning()
+        ctx.set_tuning(queue_capacity=4096)
+        try:
+            ctx.render(jobs)
+        finally:
+            ctx.set_tuning(queue_capacity=old.queue_capacity)
+        for (s, cam, w, h, spp, d, seed), fb, ref in zip(specs, fbs, alone):
+            img, n = ctx.framebuffer_read(fb, w, h)
+            assert n == spp
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+            ctx.framebuffer_destroy(fb)
+    finally:
+        ctx.scene_destroy(si)
+
+
+def test_fp64_accumulation_tracks_fp32(ctx, cornell):
+    g, s = cornell
+    img, img64 = render(ctx, s, g.camera, 32, 24, 16, 4, seed=6, fp64=True)
+    assert img64 is not None
+    np.testing.assert_allclose(img, img64, rtol=2e-5, atol=1e-6)
+
+
+def test_render_rejects_bad_jobs(ctx, cornell):
+    g, s = cornell
+    fb = ctx.framebuffer_create(8, 8)
+    try:
+        with pytest.raises(pt.FndError):
+            ctx.render([pt.make_job(s, fb, g.camera, 0, 4)])
+        with pytest.raises(pt.FndError):
+            ctx.render([pt.make_job(s, A.INVALID_HANDLE, g.camera, 1, 4)])
+        bad = A.Camera()
+        with pytest.raises(pt.FndError):
+            ctx.render([pt.make_job(s, fb, bad, 1, 4)])
+    finally:
+        ctx.framebuffer_destroy(fb)
