@@ -83,9 +83,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: every GPU box of this build has one, and the path shards by "
-              "samples or image tiles with no exchange inside a step (see DESIGN.md); run with --gpus 1",
-              file=sys.stderr)
+        print("bench.py measures one GPU only; the path shards by rays, samples or image tiles with no "
+              "exchange inside a step, so N GPUs would each run this same single-GPU workload. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

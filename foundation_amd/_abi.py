@@ -183,6 +183,26 @@ class GenParams(C.Structure):
     ]
 
 
+class CacheDesc(C.Structure):
+    _fields_ = [
+        ("enable", C.c_uint32),
+        ("levels", C.c_uint32),
+        ("features", C.c_uint32),
+        ("log2_table", C.c_uint32),
+        ("base_resolution", C.c_uint32),
+        ("max_resolution", C.c_uint32),
+        ("hidden", C.c_uint32),
+        ("hidden_layers", C.c_uint32),
+        ("learning_rate", C.c_float),
+        ("train_paths", C.c_uint32),
+        ("train_steps", C.c_uint32),
+        ("train_depth", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("bounds_lo", C.c_float * 3),
+        ("bounds_hi", C.c_float * 3),
+    ]
+
+
 class HostSceneView(C.Structure):
     _fields_ = [
         ("nodes", C.c_void_p),
@@ -237,6 +257,14 @@ PROTOTYPES = {
     "fnd_pt_framebuffer_read": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_read_f64": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_render": (_S, [_P, C.POINTER(RenderJob), C.c_uint32]),
+    "fnd_pt_cache_default_desc": (_S, [C.POINTER(CacheDesc)]),
+    "fnd_pt_cache_configure": (_S, [_P, C.POINTER(CacheDesc)]),
+    "fnd_pt_cache_param_count": (_S, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "fnd_pt_cache_set_params": (_S, [_P, _P, _P]),
+    "fnd_pt_cache_get_params": (_S, [_P, _P, _P]),
+    "fnd_pt_cache_infer": (_S, [_P, _P, C.c_uint32, _P]),
+    "fnd_pt_cache_gradients": (_S, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float), _P, _P]),
+    "fnd_pt_cache_train": (_S, [_P, _P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "fnd_scene_encode": (_S, [C.POINTER(SceneDesc), C.POINTER(_P), C.c_char_p, C.c_size_t]),
     "fnd_host_scene_get": (_S, [_P, C.POINTER(HostSceneView)]),
     "fnd_host_scene_free": (None, [_P]),

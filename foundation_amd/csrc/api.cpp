@@ -315,6 +315,7 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
     }
     ctx->framebuffers.clear();
     render_state_free(ctx);
+    cache_free(ctx);
     dev_free(ctx, ctx->d_counters, 0);
     dev_free(ctx, ctx->d_stats, 0);
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);

@@ -77,6 +77,7 @@ struct fnd_pt_ctx {
     void* d_tmp = nullptr;                     // host-array trace staging
     uint64_t tmp_bytes = 0;
     void* render_state = nullptr;              // wavefront buffers (render.cpp)
+    void* cache = nullptr;                     // radiance cache (cache_api.cpp)
 };
 
 namespace fnd {
@@ -92,5 +93,6 @@ uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block);
 void resolve_timings(fnd_pt_ctx* ctx);
 fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);
 void render_state_free(fnd_pt_ctx* ctx);
+void cache_free(fnd_pt_ctx* ctx);
 
 }  // namespace fnd

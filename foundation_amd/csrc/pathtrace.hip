@@ -70,11 +70,22 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             const uint64_t local = g - job.path_base;
             const uint32_t s = uint32_t(local / job.padded);
             const uint32_t k = uint32_t(local % job.padded);
-            const uint32_t T = job.tile;
-            const uint32_t t = k / (T * T), q = k % (T * T);
-            const uint32_t x = (t % job.tiles_x) * T + q % T;
-            const uint32_t y = (t / job.tiles_x) * T + q / T;
-            sample = job.sample_offset + s;
+            uint32_t x, y;
+            if (job.flags & kJobTrainRecord) {
+                // training path k of frame sample_offset: a random pixel, its own RNG stream
+                Rng pick = rng_for(job.seed ^ 0x5EED7EA1ull, job.sample_offset, k, 0u);
+                const uint32_t pix = rng_next(pick) % (job.width * job.height);
+                x = pix % job.width;
+                y = pix / job.width;
+                sample = job.sample_offset * job.padded + k;
+                a.b.rec_count[i] = 0u;
+            } else {
+                const uint32_t T = job.tile;
+                const uint32_t t = k / (T * T), q = k % (T * T);
+                x = (t % job.tiles_x) * T + q % T;
+                y = (t / job.tiles_x) * T + q / T;
+                sample = job.sample_offset + s;
+            }
             if (x < job.width && y < job.height) {
                 pixel = y * job.width + x;
                 Rng rng = rng_for(job.seed, sample, pixel, 0u);
@@ -106,6 +117,12 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
     }
 }
 
+// Zeroes the counter slots whose bit is set in mask.
+__global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
+{
+    if (threadIdx.x < kCntCount && ((mask >> threadIdx.x) & 1u)) counters[threadIdx.x] = 0u;
+}
+
 // Zeroes the counters the next bounce appends to and the traversal work counters.
 __global__ void k_prep(uint32_t* counters, uint32_t next)
 {
@@ -124,9 +141,11 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        bool cont = false, shadow = false;
+        bool cont = false, shadow = false, query = false;
         Ray next_ray{}, sh_ray{};
         V3 sh_contrib = v3(0, 0, 0);
+        CacheQuery cq{};
+        V3 q_w = v3(0, 0, 0);
         uint32_t path = 0, scene_slot = 0;
         if (i < n) {
             path = a.b.ext_path[cur][i];
@@ -141,6 +160,16 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             V3 L = v3(L4.x, L4.y, L4.z);
             const float pdf_prev = thr4.w;
             const V3 dir = v3(ray.dir[0], ray.dir[1], ray.dir[2]);
+            const bool record = (job.flags & kJobTrainRecord) != 0u;
+            uint32_t rc = 0;
+            if (record) {
+                // the previous diffuse vertex's target starts after its own light sample
+                rc = a.b.rec_count[path];
+                if (rc > 0) {
+                    float4* last = a.b.rec + (size_t(path) * a.b.rec_max + rc - 1) * 4;
+                    if (last[3].w == 0.0f) last[3] = make_float4(L.x, L.y, L.z, 1.0f);
+                }
+            }
             const uint32_t tri = __float_as_uint(hit.y);
             if (tri == 0xFFFFFFFFu) {
                 L = L + thr * v3(sc.background[0], sc.background[1], sc.background[2]);
@@ -214,9 +243,26 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                             }
                         }
                     }
+                    const bool diffuse = mat.type == 0u;
+                    if (record && diffuse && rc < a.b.rec_max) {
+                        float4* r = a.b.rec + (size_t(path) * a.b.rec_max + rc) * 4;
+                        r[0] = make_float4(p.x, p.y, p.z, 0.0f);
+                        r[1] = make_float4(nf.x, nf.y, nf.z, 0.0f);
+                        r[2] = make_float4(thr.x * albedo.x, thr.y * albedo.y, thr.z * albedo.z, 0.0f);
+                        r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        a.b.rec_count[path] = rc + 1;
+                    }
+                    if (diffuse && (job.flags & FND_JOB_USE_CACHE)) {
+                        // terminate into the cache: it stands for the indirect light leaving p
+                        query = true;
+                        cq.pos = make_float4(p.x, p.y, p.z, 0.0f);
+                        cq.nrm = make_float4(nf.x, nf.y, nf.z, 0.0f);
+                        q_w = thr * albedo;
+                    }
                     // BSDF sampling
                     const float u1 = rng_float(rng), u2 = rng_float(rng), urr = rng_float(rng);
-                    const BsdfSample bs = bsdf_sample(mat.type, albedo, mat.roughness, wo, u1, u2);
+                    const BsdfSample bs = query ? BsdfSample{v3(0, 0, 1), v3(0, 0, 0), 0.0f, false}
+                                                : bsdf_sample(mat.type, albedo, mat.roughness, wo, u1, u2);
                     if (bs.ok) {
                         V3 nthr = thr * bs.weight;
                         bool alive = max3(nthr) > 0.0f;
@@ -244,6 +290,12 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             a.b.ext_rays[nxt][es] = next_ray;
             a.b.ext_path[nxt][es] = path;
             a.b.ext_scene[nxt][es] = scene_slot;
+        }
+        const uint32_t qs = wave_append(query, &a.b.counters[kCntQuery], nullptr);
+        if (query) {
+            a.b.q_query[qs] = cq;
+            a.b.q_path[qs] = path;
+            a.b.q_weight[qs] = make_float4(q_w.x, q_w.y, q_w.z, 0.f);
         }
         const uint32_t ss = wave_append(shadow, &a.b.counters[kCntShadow], a.b.rays);
         if (shadow) {
@@ -278,6 +330,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfArgs a, uint64_t g0, uint3
         const uint32_t pixel = a.b.p_pixel[i];
         if (pixel == 0xFFFFFFFFu) continue;
         const JobDev& job = a.jobs[a.b.p_job[i]];
+        if (!job.accum) continue;
         const uint64_t g = g0 + i;
         const uint64_t lo = job.path_base > g0 ? job.path_base : g0;
         if (g >= lo + job.padded) continue;  // not the first sample of this pixel in the batch
@@ -297,7 +350,64 @@ __global__ __launch_bounds__(256) void k_accumulate(WfArgs a, uint64_t g0, uint3
     }
 }
 
+// Cache predictions replace the rest of the path: L += throughput * albedo * max(pred, 0).
+__global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
+{
+    const uint32_t n = a.b.counters[kCntQuery];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t path = a.b.q_path[i];
+        const float4 w = a.b.q_weight[i], p = a.b.q_pred[i];
+        float4 L = a.b.p_L[path];
+        L.x += w.x * fmaxf(p.x, 0.0f);
+        L.y += w.y * fmaxf(p.y, 0.0f);
+        L.z += w.z * fmaxf(p.z, 0.0f);
+        a.b.p_L[path] = L;
+    }
+}
+
+// Training targets from finished training paths: at a recorded diffuse vertex
+// with weight = throughput * albedo, target = (L_path - L_mark) / weight, the
+// albedo-demodulated radiance the path gathered after that vertex's own light
+// sample (per channel; vertices with a vanishing weight are skipped).
+__global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t rc = a.b.rec_count[i];
+        if (rc == 0) continue;
+        const float4 L = a.b.p_L[i];
+        for (uint32_t k = 0; k < rc; ++k) {
+            const float4* r = a.b.rec + (size_t(i) * a.b.rec_max + k) * 4;
+            const float4 w = r[2];
+            if (!(w.x > 1e-4f && w.y > 1e-4f && w.z > 1e-4f)) continue;
+            const float4 m = r[3].w != 0.0f ? r[3] : L;
+            const uint32_t slot = atomicAdd(&a.b.counters[kCntTrain], 1u);
+            if (slot >= a.b.t_cap) return;
+            CacheQuery q;
+            q.pos = r[0];
+            q.nrm = r[1];
+            a.b.t_query[slot] = q;
+            a.b.t_target[slot] = make_float4(fmaxf(L.x - m.x, 0.0f) / w.x, fmaxf(L.y - m.y, 0.0f) / w.y,
+                                             fmaxf(L.z - m.z, 0.0f) / w.z, 0.0f);
+        }
+    }
+}
+
 }  // namespace
+
+void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_slots, dim3(1), dim3(64), 0, s, a.b.counters, mask);
+}
+
+void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(256), 0, s, a);
+}
+
+void launch_cache_collect(const WfArgs& a, uint32_t n, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cache_collect, dim3(grid), dim3(256), 0, s, a, n);
+}
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)
 {

@@ -18,6 +18,7 @@
 #include <map>
 #include <vector>
 
+#include "cache_state.hpp"
 #include "ctx.hpp"
 #include "wavefront.hpp"
 
@@ -95,6 +96,7 @@ fnd_status ensure_capacity(fnd_pt_ctx* ctx, uint32_t cap)
     A(&b.sh_rays, cap); A(&b.sh_path, cap); A(&b.sh_scene, cap); A(&b.sh_contrib, cap); A(&b.sh_hits, cap);
     A(&b.counters, kCntCount);
     A(&b.rays, 1);
+    A(&b.q_query, cap); A(&b.q_path, cap); A(&b.q_weight, cap); A(&b.q_pred, cap);
     if (st != FND_OK) {
         render_state_free(ctx);
         return st;
@@ -275,6 +277,8 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     std::vector<Framebuffer*> fbs(num_jobs);
     uint64_t total = 0;
     uint32_t max_depth = 0, max_stack = 0;
+    bool any_use = false, any_train = false;
+    Scene* cache_scene = nullptr;
     for (uint32_t i = 0; i < num_jobs; ++i) {
         const fnd_render_job& in = jobs[i];
         Scene* s = find_scene(ctx, in.scene);
@@ -288,8 +292,13 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             return set_error(ctx, FND_E_INVALID_ARG, "job %u: spp must be >= 1 and max_depth <= 64", i);
         if (in.flags & ~(FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE))
             return set_error(ctx, FND_E_INVALID_ARG, "job %u: unknown flags 0x%x", i, in.flags);
-        if (in.flags & (FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE))
-            return set_error(ctx, FND_E_UNSUPPORTED, "job %u: radiance cache flags need fnd_pt_cache_configure", i);
+        if (in.flags & (FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE)) {
+            if (!cache_get(ctx))
+                return set_error(ctx, FND_E_INVALID_ARG, "job %u: radiance cache flags need fnd_pt_cache_configure", i);
+            if (!cache_scene) cache_scene = s;
+            any_use |= (in.flags & FND_JOB_USE_CACHE) != 0;
+            any_train |= (in.flags & FND_JOB_TRAIN_CACHE) != 0;
+        }
         const float dx = in.camera.at[0] - in.camera.eye[0], dy = in.camera.at[1] - in.camera.eye[1],
                     dz = in.camera.at[2] - in.camera.eye[2];
         if (!(dx * dx + dy * dy + dz * dz > 0.0f) || !(in.camera.fov_y_deg > 0.0f && in.camera.fov_y_deg < 180.0f))
@@ -314,7 +323,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         j.max_depth = in.max_depth;
         j.rr_depth = in.rr_depth;
         j.sample_offset = in.sample_offset;
-        j.flags = in.flags;
+        j.flags = in.flags & FND_JOB_USE_CACHE;
         j.seed = in.seed;
         j.path_base = total;
         total += padded * in.spp;
@@ -326,8 +335,12 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         max_depth = std::max(max_depth, in.max_depth);
         max_stack = std::max(max_stack, s->info.stack_entries);
     }
+    CacheState* cs = cache_get(ctx);
+    if (cache_scene && !cs->bounds_set) cache_set_bounds(cs, cache_scene->info.bounds_lo, cache_scene->info.bounds_hi, true);
     const uint32_t cap = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
-    fnd_status st = ensure_capacity(ctx, cap);
+    uint32_t train_cap = 0;
+    if (any_train) train_cap = std::min(std::max(1024u, ctx->tuning.queue_capacity), cs->desc.train_paths);
+    fnd_status st = ensure_capacity(ctx, std::max(cap, train_cap));
     if (st != FND_OK) return st;
     st = ensure_tables(ctx, num_jobs, uint32_t(slots.size()));
     if (st != FND_OK) return st;
@@ -347,11 +360,8 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         std::memcpy(z.background, s->background, 12);
         z.pad = 0;
     }
-    st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice, ctx->stream),
-                   "upload jobs");
-    if (st == FND_OK)
-        st = check_hip(ctx, hipMemcpyAsync(rs->d_shade, hs.data(), hs.size() * sizeof(ShadeScene), hipMemcpyHostToDevice,
-                                           ctx->stream), "upload scenes");
+    st = check_hip(ctx, hipMemcpyAsync(rs->d_shade, hs.data(), hs.size() * sizeof(ShadeScene), hipMemcpyHostToDevice,
+                                       ctx->stream), "upload scenes");
     if (st == FND_OK)
         st = check_hip(ctx, hipMemcpyAsync(rs->d_scenes, hd.data(), hd.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
                                            ctx->stream), "upload scenes");
@@ -368,51 +378,122 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
 
     WfArgs wa{};
     wa.jobs = rs->d_jobs;
-    wa.num_jobs = num_jobs;
     wa.scenes = rs->d_shade;
     wa.b = rs->b;
+    if (cs) {
+        wa.b.rec = cs->rec;
+        wa.b.rec_count = cs->rec_count;
+        wa.b.rec_max = std::min(cs->desc.train_depth, kCacheMaxRecords);
+        wa.b.t_query = cs->s_query;
+        wa.b.t_target = cs->s_target;
+        wa.b.t_cap = cs->sample_cap;
+    }
     const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
+
+    // One pass of the wavefront over a job table already on the device.
+    auto run = [&](uint32_t njobs, uint64_t paths, uint32_t batch, uint32_t depth, bool apply_cache,
+                   bool collect) -> fnd_status {
+        wa.num_jobs = njobs;
+        for (uint64_t g0 = 0; g0 < paths; g0 += batch) {
+            const uint32_t n = uint32_t(std::min<uint64_t>(batch, paths - g0));
+            const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
+            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery), ctx->stream);
+            launch_generate(wa, g0, n, grid, ctx->stream);
+            uint32_t cur = kCntExtA;
+            for (uint32_t bounce = 0; bounce <= depth; ++bounce) {
+                const uint32_t nxt = cur ^ 1u;
+                launch_prep(wa, nxt, ctx->stream);
+                TraceArgs ta{};
+                ta.scenes = rs->d_scenes;
+                ta.ray_scene = rs->b.ext_scene[cur];
+                ta.rays = rs->b.ext_rays[cur];
+                ta.hits = rs->b.ext_hits;
+                ta.nrays_dev = &rs->b.counters[cur];
+                ta.counter = &rs->b.counters[kCntWorkExt];
+                ta.spill = ctx->d_spill;
+                ta.spill_stride = tl.grid * tl.block;
+                ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
+                TraceLaunch l = tl;
+                l.mode = kModeClosestInternal;
+                fnd_status e = check_hip(ctx, launch_trace(l, ta, ctx->stream), "extension trace");
+                if (e != FND_OK) return e;
+                launch_shade(wa, cur, bounce, grid, ctx->stream);
+                ta.ray_scene = rs->b.sh_scene;
+                ta.rays = rs->b.sh_rays;
+                ta.hits = rs->b.sh_hits;
+                ta.nrays_dev = &rs->b.counters[kCntShadow];
+                ta.counter = &rs->b.counters[kCntWorkShadow];
+                l.mode = kModeAnyInternal;
+                e = check_hip(ctx, launch_trace(l, ta, ctx->stream), "shadow trace");
+                if (e != FND_OK) return e;
+                launch_shadow_resolve(wa, grid, ctx->stream);
+                cur = nxt;
+            }
+            if (apply_cache) {
+                const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
+                launch_cache_forward(cs->view, rs->b.q_query, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
+                                     cg, ctx->stream);
+                launch_cache_apply(wa, grid, ctx->stream);
+            }
+            if (collect) launch_cache_collect(wa, n, grid, ctx->stream);
+            launch_accumulate(wa, g0, n, grid, ctx->stream);
+            fnd_status e = check_hip(ctx, hipGetLastError(), "render launch");
+            if (e != FND_OK) return e;
+        }
+        return FND_OK;
+    };
+
     st = check_hip(ctx, hipMemsetAsync(rs->b.rays, 0, sizeof(unsigned long long), ctx->stream), "clear");
     if (st != FND_OK) return st;
     (void)hipEventRecord(ctx->ev[2], ctx->stream);
-    for (uint64_t g0 = 0; g0 < total; g0 += cap) {
-        const uint32_t n = uint32_t(std::min<uint64_t>(cap, total - g0));
-        const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
-        launch_prep(wa, kCntExtA, ctx->stream);
-        launch_generate(wa, g0, n, grid, ctx->stream);
-        uint32_t cur = kCntExtA;
-        for (uint32_t bounce = 0; bounce <= max_depth; ++bounce) {
-            const uint32_t nxt = cur ^ 1u;
-            launch_prep(wa, nxt, ctx->stream);
-            TraceArgs ta{};
-            ta.scenes = rs->d_scenes;
-            ta.ray_scene = rs->b.ext_scene[cur];
-            ta.rays = rs->b.ext_rays[cur];
-            ta.hits = rs->b.ext_hits;
-            ta.nrays_dev = &rs->b.counters[cur];
-            ta.counter = &rs->b.counters[kCntWorkExt];
-            ta.spill = ctx->d_spill;
-            ta.spill_stride = tl.grid * tl.block;
-            ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
-            tl.mode = kModeClosestInternal;
-            st = check_hip(ctx, launch_trace(tl, ta, ctx->stream), "extension trace");
-            if (st != FND_OK) return st;
-            launch_shade(wa, cur, bounce, grid, ctx->stream);
-            ta.ray_scene = rs->b.sh_scene;
-            ta.rays = rs->b.sh_rays;
-            ta.hits = rs->b.sh_hits;
-            ta.nrays_dev = &rs->b.counters[kCntShadow];
-            ta.counter = &rs->b.counters[kCntWorkShadow];
-            tl.mode = kModeAnyInternal;
-            st = check_hip(ctx, launch_trace(tl, ta, ctx->stream), "shadow trace");
-            if (st != FND_OK) return st;
-            launch_shadow_resolve(wa, grid, ctx->stream);
-            cur = nxt;
-        }
-        launch_accumulate(wa, g0, n, grid, ctx->stream);
-        st = check_hip(ctx, hipGetLastError(), "render launch");
+
+    // ---- training frames: trace training paths, gather targets, optimise
+    for (uint32_t i = 0; i < num_jobs && any_train; ++i) {
+        if (!(jobs[i].flags & FND_JOB_TRAIN_CACHE)) continue;
+        JobDev tj = hj[i];
+        tj.flags = kJobTrainRecord;
+        tj.padded = cs->desc.train_paths;
+        tj.tile = 1;
+        tj.tiles_x = 1;
+        tj.path_base = 0;
+        tj.path_end = cs->desc.train_paths;
+        tj.max_depth = cs->desc.train_depth;
+        tj.seed = jobs[i].seed ^ 0xC0FFEE5EEDull;
+        tj.accum = nullptr;
+        tj.accum64 = nullptr;
+        st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, &tj, sizeof(JobDev), hipMemcpyHostToDevice, ctx->stream),
+                       "upload training job");
         if (st != FND_OK) return st;
+        launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
+        st = run(1, cs->desc.train_paths, train_cap, tj.max_depth, false, true);
+        if (st != FND_OK) return st;
+        uint32_t nsamp = 0;
+        st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[kCntTrain], 4, hipMemcpyDeviceToHost, ctx->stream),
+                       "read sample count");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "training paths");
+        if (st != FND_OK) return st;
+        nsamp = std::min(nsamp, cs->sample_cap);
+        if (nsamp) {
+            const uint32_t steps = cs->desc.train_steps;
+            const uint32_t per = std::min(kCacheBatch, (nsamp + steps - 1) / steps);
+            for (uint32_t k = 0; k < steps; ++k) {
+                const uint32_t off = uint32_t((uint64_t(k) * per) % nsamp);
+                const uint32_t m = std::min(per, nsamp - off);
+                float* lp = (k + 1 == steps) ? &cs->train_loss_host : nullptr;
+                st = cache_step(ctx, cs, cs->s_query + off, cs->s_target + off, m, true, nullptr, nullptr, lp);
+                if (st != FND_OK) return st;
+            }
+            cs->last_samples = nsamp;
+        }
+        cs->frames += 1;
     }
+
+    // ---- the frame itself
+    st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice,
+                                       ctx->stream), "upload jobs");
+    if (st != FND_OK) return st;
+    st = run(num_jobs, total, cap, max_depth, any_use, false);
+    if (st != FND_OK) return st;
     (void)hipEventRecord(ctx->ev[3], ctx->stream);
     st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "render");
     if (st != FND_OK) return st;

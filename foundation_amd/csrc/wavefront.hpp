@@ -2,10 +2,14 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "cache.hpp"
 #include "pt_layout.hpp"
 #include "scene_encode.hpp"
 
 namespace fnd {
+
+// JobDev.flags: FND_JOB_USE_CACHE (1) from the caller, and internally:
+constexpr uint32_t kJobTrainRecord = 0x10000u;  // training paths: random pixels, record diffuse vertices
 
 struct JobDev {
     uint32_t scene_slot;
@@ -50,6 +54,8 @@ enum WfCounter : uint32_t {
     kCntShadow = 2,
     kCntWorkExt = 3,
     kCntWorkShadow = 4,
+    kCntQuery = 5,
+    kCntTrain = 6,
     kCntCount = 8
 };
 
@@ -73,6 +79,18 @@ struct WfBuffers {
     float4* sh_hits;
     uint32_t* counters;       // kCntCount
     unsigned long long* rays; // rays appended (extension + shadow), cumulative
+    // radiance-cache queries (at most one per path per batch)
+    CacheQuery* q_query;
+    uint32_t* q_path;
+    float4* q_weight;         // throughput * albedo at the query vertex
+    float4* q_pred;
+    // training records: per path slot, rec_max x (pos, normal, weight, L mark (w = 1 once set))
+    float4* rec;
+    uint32_t* rec_count;
+    uint32_t rec_max;
+    CacheQuery* t_query;      // gathered training samples
+    float4* t_target;
+    uint32_t t_cap;
 };
 
 struct WfArgs {
@@ -87,5 +105,8 @@ void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s);
 void launch_shade(const WfArgs& a, uint32_t cur, uint32_t bounce, uint32_t grid, hipStream_t s);
 void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
+void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);
+void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s);
+void launch_cache_collect(const WfArgs& a, uint32_t n, uint32_t grid, hipStream_t s);
 
 }  // namespace fnd

@@ -288,6 +288,53 @@ fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle fb, double* r
 /* Renders all jobs in one batched wavefront launch sequence. */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
 
+/* ---------------------------------------------------------------- radiance cache */
+
+/* Online-trained radiance cache (no counterpart in the reference).  A
+ * multiresolution hash grid (Mueller et al. 2022) feeds a ReLU MLP without
+ * biases (Mueller et al. 2021) that predicts the albedo-demodulated indirect
+ * radiance leaving a diffuse point.  The kernels are compiled for exactly:
+ * 16 levels x 2 features, 2^19 entries per level, resolutions 16..2048,
+ * 5 hidden layers of 64; any other shape returns FND_E_UNSUPPORTED.
+ * Inputs per query: world position and oriented unit normal.  Loss: relative
+ * L2, sum over samples and RGB of (p - y)^2 / (p^2 + 0.01) with p held
+ * constant in the denominator.  Optimiser: Adam (beta1 0.9, beta2 0.99,
+ * eps 1e-15) on FP32 master weights; the kernels read FP16 copies. */
+typedef struct fnd_cache_desc {
+    uint32_t enable;            /* 0 releases the cache */
+    uint32_t levels;            /* 16 */
+    uint32_t features;          /* 2 */
+    uint32_t log2_table;        /* 19 */
+    uint32_t base_resolution;   /* 16 */
+    uint32_t max_resolution;    /* 2048 */
+    uint32_t hidden;            /* 64 */
+    uint32_t hidden_layers;     /* 5 */
+    float learning_rate;        /* 1e-2 */
+    uint32_t train_paths;       /* training paths traced per frame (FND_JOB_TRAIN_CACHE), <= 2^16 */
+    uint32_t train_steps;       /* optimiser steps per frame */
+    uint32_t train_depth;       /* bounces of a training path */
+    uint64_t seed;              /* initialisation */
+    float bounds_lo[3];         /* region the grid covers; lo == hi takes the bounds of the */
+    float bounds_hi[3];         /* first scene rendered with the cache */
+} fnd_cache_desc;
+
+fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* out);
+fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* desc);
+/* Parameter counts: MLP weights (plain layout: W0 [48][64], W1..W4
+ * [64][64], W5 [64][16], row = input) and grid features ([level][entry][2]). */
+fnd_status fnd_pt_cache_param_count(fnd_pt_ctx* ctx, uint64_t* mlp, uint64_t* grid);
+fnd_status fnd_pt_cache_set_params(fnd_pt_ctx* ctx, const float* mlp, const float* grid);
+fnd_status fnd_pt_cache_get_params(fnd_pt_ctx* ctx, float* mlp, float* grid);
+/* queries: n * 6 floats (position, normal); out: n * 3 */
+fnd_status fnd_pt_cache_infer(fnd_pt_ctx* ctx, const float* queries, uint32_t n, float* out);
+/* Loss and gradients of the loss for one batch, without an optimiser step.
+ * dmlp: fnd_pt_cache_param_count(mlp) floats, dgrid: (grid) floats or NULL. */
+fnd_status fnd_pt_cache_gradients(fnd_pt_ctx* ctx, const float* queries, const float* targets, uint32_t n,
+                                  float* loss, float* dmlp, float* dgrid);
+/* `steps` optimiser steps on one batch; loss of the last step. */
+fnd_status fnd_pt_cache_train(fnd_pt_ctx* ctx, const float* queries, const float* targets, uint32_t n,
+                              uint32_t steps, float* loss);
+
 /* ---------------------------------------------------------------- host encoding */
 
 /* The host half of fnd_pt_scene_create without a device: validates the

@@ -112,6 +112,16 @@ void orc_trace_wide(const void* nodes, const void* tris, const void* instances, 
                     uint32_t stack_entries, const float* rays, uint32_t n, int anyhit, int nthreads, float* hits,
                     uint32_t* counts);
 
+/* Radiance cache in FP64 (see pt_oracle_cache.c).  mlp: 20480 floats in the
+ * plain layout W0 [48][64], W1..W4 [64][64], W5 [64][16] (row = input);
+ * grid: 16 * 2^19 * 2 floats.  Returns the summed relative-L2 loss. */
+void orc_cache_levels(int* res16, uint32_t* dense_mask);
+double orc_cache_loss_grad(const float* mlp, const float* grid, const float* queries, const float* targets, uint32_t n,
+                           const float* lo3, float inv_extent, int round_fp16, int plain_l2, float* pred, double* dmlp,
+                           double* dgrid);
+void orc_adam_step(double* p, double* m, double* v, const double* g, uint64_t n, double lr, double b1, double b2,
+                   double eps, uint32_t t);
+
 /* Deepest traversal stack any orc_trace_wide walk reached (0xFFFFFFFF if a
  * walk hit its capacity); reset = 1 clears it. */
 uint32_t orc_wide_max_stack(int reset);

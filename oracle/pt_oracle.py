@@ -60,6 +60,12 @@ def load():
     lib.orc_bsdf_sample.restype = C.c_int
     lib.orc_bsdf_eval.argtypes = [C.c_int, P, C.c_float, P, P, P]
     lib.orc_bsdf_eval.restype = None
+    lib.orc_cache_levels.argtypes = [P, P]
+    lib.orc_cache_levels.restype = None
+    lib.orc_cache_loss_grad.argtypes = [P, P, P, P, C.c_uint32, P, C.c_float, C.c_int, C.c_int, P, P, P]
+    lib.orc_cache_loss_grad.restype = C.c_double
+    lib.orc_adam_step.argtypes = [P, P, P, P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint32]
+    lib.orc_adam_step.restype = None
     lib.orc_wide_max_stack.argtypes = [C.c_int]
     lib.orc_wide_max_stack.restype = C.c_uint32
     _lib = lib
@@ -214,3 +220,38 @@ def trace_wide(encoded, rays, anyhit=False, threads=0):
     if max_stack > encoded["stack_entries"]:
         raise AssertionError(f"walk needed {max_stack} stack entries, bound is {encoded['stack_entries']}")
     return hits, cnt
+
+
+CACHE_MLP_PARAMS = 20480
+CACHE_GRID_PARAMS = 16 * (1 << 19) * 2
+
+
+def cache_levels():
+    res = np.zeros(16, np.int32)
+    mask = np.zeros(1, np.uint32)
+    load().orc_cache_levels(res.ctypes.data, mask.ctypes.data)
+    return res, int(mask[0])
+
+
+def cache_loss_grad(mlp, grid, queries, targets, lo, inv_extent, round_fp16=True, want_grid=True, plain_l2=False):
+    """FP64 loss, prediction and gradients of the radiance cache."""
+    mlp = np.ascontiguousarray(mlp, np.float32)
+    grid = np.ascontiguousarray(grid, np.float32)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+    t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+    lo = np.ascontiguousarray(lo, np.float32)
+    n = q.shape[0]
+    pred = np.zeros((n, 3), np.float32)
+    dmlp = np.zeros(CACHE_MLP_PARAMS, np.float64)
+    dgrid = np.zeros(CACHE_GRID_PARAMS, np.float64) if want_grid else None
+    loss = load().orc_cache_loss_grad(mlp.ctypes.data, grid.ctypes.data, q.ctypes.data, t.ctypes.data, n,
+                                      lo.ctypes.data, float(inv_extent), int(round_fp16), int(plain_l2),
+                                      pred.ctypes.data,
+                                      dmlp.ctypes.data, dgrid.ctypes.data if want_grid else None)
+    return loss, pred, dmlp, dgrid
+
+
+def adam_step(p, m, v, g, lr, t, b1=0.9, b2=0.99, eps=1e-15):
+    for a in (p, m, v, g):
+        assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    load().orc_adam_step(p.ctypes.data, m.ctypes.data, v.ctypes.data, g.ctypes.data, p.size, lr, b1, b2, eps, t)

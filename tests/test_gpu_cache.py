@@ -1,0 +1,164 @@
+"""Radiance cache on the GPU vs the FP64 CPU oracle, and what it gains.
+
+Tolerances: the GPU forward stores weights, grid values and activations in
+FP16 (the oracle rounds the same quantities with round_fp16=True); the
+backward runs in BF16 (8-bit mantissa) with FP32 accumulation, so gradients
+are compared as whole vectors per layer, |g_gpu - g_ref| / |g_ref| < 5e-2.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from foundation_amd import _abi as A
+from foundation_amd import pt
+
+pytestmark = pytest.mark.gpu
+
+LAYERS = [(0, 3072), (3072, 7168), (7168, 11264), (11264, 15360), (15360, 19456), (19456, 20480)]
+
+
+def configure(ctx, lo=(0, 0, 0), hi=(1, 1, 1), **kw):
+    d = A.CacheDesc()
+    assert ctx.lib.fnd_pt_cache_default_desc(C.byref(d)) == 0
+    for i in range(3):
+        d.bounds_lo[i] = lo[i]
+        d.bounds_hi[i] = hi[i]
+    for k, v in kw.items():
+        setattr(d, k, v)
+    ctx._ck(ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d)))
+
+
+def set_params(ctx, mlp, grid):
+    ctx._ck(ctx.lib.fnd_pt_cache_set_params(ctx.h, mlp.ctypes.data, grid.ctypes.data))
+
+
+def gradients(ctx, q, t):
+    loss = C.c_float()
+    dm = np.zeros(20480, np.float32)
+    dg = np.zeros(16 * (1 << 19) * 2, np.float32)
+    ctx._ck(ctx.lib.fnd_pt_cache_gradients(ctx.h, q.ctypes.data, t.ctypes.data, q.shape[0], C.byref(loss),
+                                           dm.ctypes.data, dg.ctypes.data))
+    return loss.value, dm, dg
+
+
+def infer(ctx, q):
+    out = np.zeros((q.shape[0], 3), np.float32)
+    ctx._ck(ctx.lib.fnd_pt_cache_infer(ctx.h, q.ctypes.data, q.shape[0], out.ctypes.data))
+    return out
+
+
+@pytest.fixture(scope="module")
+def params(oracle):
+    rng = np.random.default_rng(1)
+    mlp = rng.uniform(-0.25, 0.25, 20480).astype(np.float32)
+    grid = rng.uniform(-0.5, 0.5, oracle.CACHE_GRID_PARAMS).astype(np.float32)
+    n = 2048
+    pos = rng.uniform(0.02, 0.98, (n, 3))
+    nrm = rng.normal(size=(n, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    q = np.concatenate([pos, nrm], 1).astype(np.float32)
+    t = rng.uniform(0, 2, (n, 3)).astype(np.float32)
+    return mlp, grid, q, t
+
+
+def test_forward_matches_oracle(ctx, oracle, params):
+    mlp, grid, q, t = params
+    configure(ctx)
+    set_params(ctx, mlp, grid)
+    got = infer(ctx, q)
+    _, ref, _, _ = oracle.cache_loss_grad(mlp, grid, q, t, [0, 0, 0], 1.0, round_fp16=True, want_grid=False)
+    err = np.abs(got - ref)
+    print(f"forward max err {err.max():.3e}, ref scale {np.abs(ref).mean():.3e}", flush=True)
+    assert np.all(err <= 2e-2 * (np.abs(ref) + 0.1))
+
+
+def test_gradients_match_oracle(ctx, oracle, params):
+    mlp, grid, q, t = params
+    configure(ctx)
+    set_params(ctx, mlp, grid)
+    loss, dm, dg = gradients(ctx, q, t)
+    rloss, _, rdm, rdg = oracle.cache_loss_grad(mlp, grid, q, t, [0, 0, 0], 1.0, round_fp16=True)
+    print(f"loss gpu {loss:.6g} oracle {rloss:.6g}", flush=True)
+    assert abs(loss - rloss) <= 1e-2 * abs(rloss)
+    for a, b in LAYERS:
+        rel = np.linalg.norm(dm[a:b] - rdm[a:b]) / np.linalg.norm(rdm[a:b])
+        print(f"layer [{a}:{b}) relative gradient error {rel:.3e}", flush=True)
+        assert rel < 5e-2, (a, b, rel)
+    rel = np.linalg.norm(dg - rdg) / np.linalg.norm(rdg)
+    print(f"grid relative gradient error {rel:.3e}", flush=True)
+    assert rel < 5e-2
+    # no gradient outside the touched entries
+    assert np.all(dg[rdg == 0] == 0)
+
+
+def test_adam_first_step(ctx, params):
+    mlp, grid, q, t = params
+    configure(ctx, learning_rate=1e-2)
+    set_params(ctx, mlp, grid)
+    _, dm, dg = gradients(ctx, q, t)
+    loss = C.c_float()
+    ctx._ck(ctx.lib.fnd_pt_cache_train(ctx.h, q.ctypes.data, t.ctypes.data, q.shape[0], 1, C.byref(loss)))
+    new_m = np.zeros(20480, np.float32)
+    new_g = np.zeros_like(grid)
+    ctx._ck(ctx.lib.fnd_pt_cache_get_params(ctx.h, new_m.ctypes.data, new_g.ctypes.data))
+    sel = np.abs(dm) > 1e-6
+    np.testing.assert_allclose((new_m - mlp)[sel], -1e-2 * np.sign(dm[sel]), rtol=0, atol=2e-6)
+    assert np.all(new_m[dm == 0] == mlp[dm == 0])
+    gs = np.abs(dg) > 1e-6
+    np.testing.assert_allclose((new_g - grid)[gs], -1e-2 * np.sign(dg[gs]), rtol=0, atol=2e-6)
+
+
+def test_training_fits_a_smooth_field(ctx):
+    rng = np.random.default_rng(3)
+    configure(ctx, seed=7)
+    n = 1 << 16
+    pos = rng.uniform(0, 1, (n, 3))
+    nrm = np.tile([0.0, 0.0, 1.0], (n, 1))
+    q = np.concatenate([pos, nrm], 1).astype(np.float32)
+    # targets in [0.5, 2.5] so the relative error below is well defined
+    t = np.stack([1.5 + np.sin(6 * pos[:, 0]), 1.5 + np.cos(5 * pos[:, 1]), 1 + pos[:, 2] ** 2], 1).astype(np.float32)
+    loss = C.c_float()
+    ctx._ck(ctx.lib.fnd_pt_cache_train(ctx.h, q.ctypes.data, t.ctypes.data, n, 1, C.byref(loss)))
+    first = loss.value
+    ctx._ck(ctx.lib.fnd_pt_cache_train(ctx.h, q.ctypes.data, t.ctypes.data, n, 200, C.byref(loss)))
+    last = loss.value
+    print(f"loss {first:.4g} -> {last:.4g}", flush=True)
+    assert last < 0.05 * first
+    pred = infer(ctx, q[:4096])
+    assert np.mean(np.abs(pred - t[:4096]) / t[:4096]) < 0.05
+
+
+def rel_mse(x, y):
+    return float(np.mean((x.astype(np.float64) - y) ** 2 / (y.astype(np.float64) ** 2 + 0.01)))
+
+
+def test_cache_beats_plain_path_tracing_at_equal_samples(ctx):
+    """Cornell box + soup: 32 frames of 1 spp with the cache (trained every
+    frame on paths the tracer generates) against 32 spp of plain path
+    tracing, both judged by a 1024-spp plain render in FP64."""
+    g = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    scene = ctx.scene_create(g.desc)
+    w = h = 96
+    try:
+        ref_fb = ctx.framebuffer_create(w, h, fp64=True)
+        ctx.render([pt.make_job(scene, ref_fb, g.camera, 1024, 6, seed=99)])
+        ref = ctx.framebuffer_read(ref_fb, w, h, fp64=True)[0]
+        ctx.framebuffer_destroy(ref_fb)
+        plain_fb = ctx.framebuffer_create(w, h)
+        ctx.render([pt.make_job(scene, plain_fb, g.camera, 32, 6, seed=5)])
+        plain = ctx.framebuffer_read(plain_fb, w, h)[0]
+        ctx.framebuffer_destroy(plain_fb)
+        configure(ctx, lo=(0, 0, 0), hi=(0, 0, 0), train_paths=1 << 14, train_depth=5)  # bounds from the scene
+        cache_fb = ctx.framebuffer_create(w, h)
+        for f in range(32):
+            ctx.render([pt.make_job(scene, cache_fb, g.camera, 1, 6, seed=5, sample_offset=f,
+                                    flags=A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE)])
+        cached = ctx.framebuffer_read(cache_fb, w, h)[0]
+        ctx.framebuffer_destroy(cache_fb)
+        e_plain, e_cache = rel_mse(plain, ref), rel_mse(cached, ref)
+        print(f"relMSE plain 32 spp {e_plain:.4e}, cache 32 frames {e_cache:.4e}", flush=True)
+        assert np.all(np.isfinite(cached))
+        assert e_cache < e_plain
+    finally:
+        ctx.scene_destroy(scene)
