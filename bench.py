@@ -118,7 +118,8 @@ def main():
     prim_hits = ctx.trace(scene, primary)
     lookup = gen.triangle_lookup()
     incoherent, _ = R.bounce_rays(primary, prim_hits, lookup, seed=1)
-    shadow, _ = R.shadow_rays(primary, prim_hits, lookup, [0.0, 0.0, 9.0])
+    # shadow rays toward the centre of the plane that holds the scene's four quad lights (z = 7.5)
+    shadow, _ = R.shadow_rays(primary, prim_hits, lookup, [0.0, 0.0, 7.5])
     nrays = incoherent.shape[0]
     log(f"ray sets: {primary.shape[0]} primary, {nrays} incoherent, {shadow.shape[0]} shadow")
 
