@@ -1,0 +1,236 @@
+// PathTracerPass.hpp — the C++ face of the path-tracing pass, over the
+// extern "C" fnd_pt ABI (include/fnd_pt.h).  Header-only, no HIP types.
+//
+// Shaped after the reference's Renderer so it can stand in its place:
+//  * constructed from a device choice and a borrowed host allocator, as
+//    Renderer(RHIApplicationObjectHandle<RHIDevice>, Core::Allocator*)
+//    (src/Renderer/Renderer.hpp:41, src/Renderer/Renderer.cpp:34-37); any
+//    type with the Allocate(size, alignment) / Reallocate(ptr, size,
+//    alignment) / Deallocate(ptr) members of Core::Allocator
+//    (src/Core/Allocator/Allocator.hpp:27-31) plugs in;
+//  * Execute() records and submits one frame and waits for it, in place of
+//    Record + Submit + the fence wait in Renderer::Draw
+//    (src/Renderer/Renderer.cpp:382-394);
+//  * the destructor waits for the device to go idle first, as
+//    Renderer::~Renderer does (src/Renderer/Renderer.cpp:402-406);
+//  * a failed call is fatal, as CHECK -> BugCheck -> std::terminate is
+//    (src/Core/Core.hpp:17, src/Core/Core.cpp:7-10), unless the caller installs
+//    a handler that throws instead.
+//
+// Single-threaded, like the reference's one render loop
+// (src/Editor/Editor.cpp:20-23).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <exception>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fnd_pt.h"
+
+namespace fnd {
+
+// What happens on a non-OK status.  The default prints the status and the
+// context's last error and terminates.
+using FailureHandler = void (*)(fnd_status status, const char* call, const char* detail);
+
+inline void terminate_on_failure(fnd_status status, const char* call, const char* detail)
+{
+    std::fprintf(stderr, "fnd_pt: %s failed: %s (%d): %s\n", call, fnd_pt_status_string(status), int(status),
+                 detail ? detail : "");
+    std::terminate();
+}
+
+inline void throw_on_failure(fnd_status status, const char* call, const char* detail)
+{
+    throw std::runtime_error(std::string(call) + ": " + fnd_pt_status_string(status) + ": " + (detail ? detail : ""));
+}
+
+inline FailureHandler& failure_handler()
+{
+    static FailureHandler handler = terminate_on_failure;
+    return handler;
+}
+
+// C callbacks over a Core::Allocator-shaped object.
+template <class Alloc>
+fnd_host_allocator make_host_allocator(Alloc* a)
+{
+    fnd_host_allocator h{};
+    h.user = a;
+    h.alloc = [](void* u, size_t size, size_t align) -> void* { return static_cast<Alloc*>(u)->Allocate(size, align); };
+    h.realloc = [](void* u, void* p, size_t size, size_t align) -> void* {
+        return static_cast<Alloc*>(u)->Reallocate(p, size, align);
+    };
+    h.free = [](void* u, void* p) { static_cast<Alloc*>(u)->Deallocate(p); };
+    return h;
+}
+
+class PathTracerPass {
+public:
+    struct Desc {
+        int device = 0;
+        uint64_t device_memory_budget = 0;  // 0 = unlimited
+        uint32_t width = 1920;              // the reference's swapchain size (src/Renderer/Renderer.cpp:40-41)
+        uint32_t height = 1080;
+        uint32_t spp = 1;                   // samples added per Execute()
+        uint32_t max_depth = 8;
+        uint32_t rr_depth = 3;
+        uint64_t seed = 1;
+        bool use_cache = false;             // terminate paths into the radiance cache
+        bool train_cache = false;           // train the cache on this pass's own paths every frame
+        bool fp64 = false;                  // also accumulate in double (reference images)
+    };
+
+    explicit PathTracerPass(const Desc& desc) : desc_(desc) { init(nullptr); }
+
+    template <class Alloc>
+    PathTracerPass(const Desc& desc, Alloc* allocator) : desc_(desc), host_alloc_(make_host_allocator(allocator))
+    {
+        init(&host_alloc_);
+    }
+
+    PathTracerPass(const PathTracerPass&) = delete;
+    PathTracerPass& operator=(const PathTracerPass&) = delete;
+
+    ~PathTracerPass()
+    {
+        if (!ctx_) return;
+        (void)fnd_pt_wait(ctx_);
+        fnd_pt_destroy(ctx_);  // also releases scenes, buffers and framebuffers
+    }
+
+    // ---- scene and view
+    fnd_handle CreateScene(const fnd_scene_desc& desc)
+    {
+        fnd_handle h = FND_INVALID_HANDLE;
+        check(fnd_pt_scene_create(ctx_, &desc, &h), "fnd_pt_scene_create");
+        return h;
+    }
+    void DestroyScene(fnd_handle scene)
+    {
+        if (scene == scene_) scene_ = FND_INVALID_HANDLE;
+        check(fnd_pt_scene_destroy(ctx_, scene), "fnd_pt_scene_destroy");
+    }
+    fnd_scene_info SceneInfo(fnd_handle scene)
+    {
+        fnd_scene_info info{};
+        check(fnd_pt_scene_info(ctx_, scene, &info), "fnd_pt_scene_info");
+        return info;
+    }
+    // Select what Execute() renders.  A new scene or camera restarts the
+    // accumulation.
+    void SetView(fnd_handle scene, const fnd_camera& camera)
+    {
+        scene_ = scene;
+        camera_ = camera;
+        Reset();
+    }
+
+    void ConfigureCache(const fnd_cache_desc* desc)
+    {
+        fnd_cache_desc d{};
+        if (!desc) {
+            check(fnd_pt_cache_default_desc(&d), "fnd_pt_cache_default_desc");
+            desc = &d;
+        }
+        check(fnd_pt_cache_configure(ctx_, desc), "fnd_pt_cache_configure");
+    }
+
+    // ---- one frame
+    void Execute()
+    {
+        if (scene_ == FND_INVALID_HANDLE) check(FND_E_INVALID_ARG, "PathTracerPass::Execute (no scene set)");
+        fnd_render_job job{};
+        job.scene = scene_;
+        job.framebuffer = fb_;
+        job.camera = camera_;
+        job.spp = desc_.spp;
+        job.max_depth = desc_.max_depth;
+        job.rr_depth = desc_.rr_depth;
+        job.flags = (desc_.use_cache ? FND_JOB_USE_CACHE : 0u) | (desc_.train_cache ? FND_JOB_TRAIN_CACHE : 0u);
+        job.seed = desc_.seed;
+        job.sample_offset = samples_;
+        check(fnd_pt_render(ctx_, &job, 1), "fnd_pt_render");
+        samples_ += desc_.spp;
+    }
+
+    void Reset()
+    {
+        check(fnd_pt_framebuffer_clear(ctx_, fb_), "fnd_pt_framebuffer_clear");
+        samples_ = 0;
+    }
+
+    // ---- output
+    // Mean radiance, width * height * 3 floats, row 0 at the top.
+    void ReadRGB(std::vector<float>& rgb)
+    {
+        rgb.resize(size_t(desc_.width) * desc_.height * 3);
+        uint32_t n = 0;
+        check(fnd_pt_framebuffer_read(ctx_, fb_, rgb.data(), &n), "fnd_pt_framebuffer_read");
+    }
+    void ReadRGB64(std::vector<double>& rgb)
+    {
+        rgb.resize(size_t(desc_.width) * desc_.height * 3);
+        uint32_t n = 0;
+        check(fnd_pt_framebuffer_read_f64(ctx_, fb_, rgb.data(), &n), "fnd_pt_framebuffer_read_f64");
+    }
+    // R8G8B8A8_UNORM, the reference's presentation format
+    // (src/Renderer/Renderer.cpp:39-44): clamp(x, 0, 1)^(1/2.2) * 255,
+    // rounded to nearest, alpha 255.
+    void ReadRGBA8(std::vector<uint8_t>& rgba)
+    {
+        std::vector<float> rgb;
+        ReadRGB(rgb);
+        const size_t n = size_t(desc_.width) * desc_.height;
+        rgba.resize(n * 4);
+        for (size_t i = 0; i < n; ++i) {
+            for (int c = 0; c < 3; ++c) {
+                float x = rgb[3 * i + c];
+                x = x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f;
+                rgba[4 * i + c] = uint8_t(std::pow(x, 1.0f / 2.2f) * 255.0f + 0.5f);
+            }
+            rgba[4 * i + 3] = 255;
+        }
+    }
+
+    uint32_t Samples() const { return samples_; }
+    const Desc& Description() const { return desc_; }
+    fnd_pt_ctx* Handle() const { return ctx_; }
+    fnd_pt_stats Stats()
+    {
+        fnd_pt_stats s{};
+        check(fnd_pt_get_stats(ctx_, &s), "fnd_pt_get_stats");
+        return s;
+    }
+
+private:
+    void init(const fnd_host_allocator* alloc)
+    {
+        fnd_pt_create_desc cd{};
+        cd.device = desc_.device;
+        cd.device_memory_budget = desc_.device_memory_budget;
+        check(fnd_pt_create(&cd, alloc, &ctx_), "fnd_pt_create");
+        check(fnd_pt_framebuffer_create(ctx_, desc_.width, desc_.height, desc_.fp64 ? FND_FB_FP64 : 0u, &fb_),
+              "fnd_pt_framebuffer_create");
+    }
+
+    void check(fnd_status st, const char* call)
+    {
+        if (st != FND_OK) failure_handler()(st, call, ctx_ ? fnd_pt_last_error(ctx_) : "");
+    }
+
+    Desc desc_;
+    fnd_host_allocator host_alloc_{};
+    fnd_pt_ctx* ctx_ = nullptr;
+    fnd_handle fb_ = FND_INVALID_HANDLE;
+    fnd_handle scene_ = FND_INVALID_HANDLE;
+    fnd_camera camera_{};
+    uint32_t samples_ = 0;
+};
+
+}  // namespace fnd
