@@ -145,20 +145,6 @@ uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block)
     return std::max(1u, waves * 64u / block);
 }
 
-}  // namespace fnd
-
-namespace {
-
-// Copies on the context stream and waits: the stream is non-blocking, so a
-// plain hipMemcpy on the null stream would not order against its kernels.
-fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipMemcpyKind kind, const char* what)
-{
-    if (!bytes) return FND_OK;
-    fnd_status st = check_hip(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream), what);
-    if (st != FND_OK) return st;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), what);
-}
-
 void* host_alloc(const fnd_host_allocator* a, size_t size, size_t align)
 {
     if (a && a->alloc) return a->alloc(a->user, size, align);
@@ -174,19 +160,18 @@ void host_free(const fnd_host_allocator* a, void* p)
         std::free(p);
 }
 
-template <class T>
-T* ctx_new(fnd_pt_ctx* ctx)
-{
-    void* p = host_alloc(ctx->has_alloc ? &ctx->alloc : nullptr, sizeof(T), alignof(T) < 16 ? 16 : alignof(T));
-    return p ? new (p) T() : nullptr;
-}
+}  // namespace fnd
 
-template <class T>
-void ctx_delete(fnd_pt_ctx* ctx, T* p)
+namespace {
+
+// Copies on the context stream and waits: the stream is non-blocking, so a
+// plain hipMemcpy on the null stream would not order against its kernels.
+fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipMemcpyKind kind, const char* what)
 {
-    if (!p) return;
-    p->~T();
-    host_free(ctx->has_alloc ? &ctx->alloc : nullptr, p);
+    if (!bytes) return FND_OK;
+    fnd_status st = check_hip(ctx, hipMemcpyAsync(dst, src, bytes, kind, ctx->stream), what);
+    if (st != FND_OK) return st;
+    return check_hip(ctx, hipStreamSynchronize(ctx->stream), what);
 }
 
 void default_tuning(fnd_tuning& t)
@@ -569,6 +554,8 @@ static fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float
 {
     const bool any = (flags & FND_TRACE_ANYHIT) != 0;
     const bool count = (flags & FND_TRACE_COUNT) != 0;
+    // ray indices are formed as grid threads + a 32-bit work counter
+    if (n > kMaxRaysPerTrace) return set_error(ctx, FND_E_INVALID_ARG, "at most %u rays per call", kMaxRaysPerTrace);
     TraceLaunch l{};
     l.mode = any ? kModeAny : kModeClosest;
     l.two_level = s->two_level;

@@ -58,7 +58,7 @@ void cache_free(fnd_pt_ctx* ctx)
     if (!cs) return;
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& a : cs->allocs) dev_free(ctx, a.first, a.second);
-    delete cs;
+    ctx_delete(ctx, cs);
     ctx->cache = nullptr;
 }
 
@@ -189,7 +189,7 @@ fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* d)
     fnd_status st = validate(ctx, *d);
     if (st != FND_OK) return st;
     cache_free(ctx);
-    auto* cs = new (std::nothrow) CacheState();
+    auto* cs = ctx_new<CacheState>(ctx);
     if (!cs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
     ctx->cache = cs;
     cs->desc = *d;

@@ -1,5 +1,6 @@
 // ctx.hpp — internal state behind the fnd_pt_* C ABI.
 #pragma once
+#include <new>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -94,5 +95,25 @@ void resolve_timings(fnd_pt_ctx* ctx);
 fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);
 void render_state_free(fnd_pt_ctx* ctx);
 void cache_free(fnd_pt_ctx* ctx);
+
+// Host memory of a context comes from the caller's allocator (malloc/free
+// when none was given); every context-owned object is made and released here.
+void* host_alloc(const fnd_host_allocator* a, size_t size, size_t align);
+void host_free(const fnd_host_allocator* a, void* p);
+
+template <class T>
+T* ctx_new(fnd_pt_ctx* ctx)
+{
+    void* p = host_alloc(ctx->has_alloc ? &ctx->alloc : nullptr, sizeof(T), alignof(T) < 16 ? 16 : alignof(T));
+    return p ? new (p) T() : nullptr;
+}
+
+template <class T>
+void ctx_delete(fnd_pt_ctx* ctx, T* p)
+{
+    if (!p) return;
+    p->~T();
+    host_free(ctx->has_alloc ? &ctx->alloc : nullptr, p);
+}
 
 }  // namespace fnd

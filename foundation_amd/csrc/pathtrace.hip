@@ -61,7 +61,7 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         bool live = false;
-        uint32_t pixel = 0xFFFFFFFFu, sample = 0, j = 0;
+        uint32_t pixel = 0xFFFFFFFFu, sample = 0, j = 0, train = kNoTrainSlot;
         Ray ray{};
         if (i < n) {
             const uint64_t g = g0 + i;
@@ -70,24 +70,23 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             const uint64_t local = g - job.path_base;
             const uint32_t s = uint32_t(local / job.padded);
             const uint32_t k = uint32_t(local % job.padded);
-            uint32_t x, y;
-            if (job.flags & kJobTrainRecord) {
-                // training path k of frame sample_offset: a random pixel, its own RNG stream
-                Rng pick = rng_for(job.seed ^ 0x5EED7EA1ull, job.sample_offset, k, 0u);
-                const uint32_t pix = rng_next(pick) % (job.width * job.height);
-                x = pix % job.width;
-                y = pix / job.width;
-                sample = job.sample_offset * job.padded + k;
-                a.b.rec_count[i] = 0u;
-            } else {
-                const uint32_t T = job.tile;
-                const uint32_t t = k / (T * T), q = k % (T * T);
-                x = (t % job.tiles_x) * T + q % T;
-                y = (t / job.tiles_x) * T + q / T;
-                sample = job.sample_offset + s;
-            }
+            const uint32_t T = job.tile;
+            const uint32_t t = k / (T * T), q = k % (T * T);
+            const uint32_t x = (t % job.tiles_x) * T + q % T;
+            const uint32_t y = (t / job.tiles_x) * T + q / T;
+            sample = job.sample_offset + s;
             if (x < job.width && y < job.height) {
                 pixel = y * job.width + x;
+                if (job.train_count) {
+                    const uint64_t qi = uint64_t(s) * (uint64_t(job.width) * job.height) + pixel;
+                    if (qi < job.train_mod) {
+                        const uint64_t ti = (qi * job.train_mul + job.train_add) % job.train_mod;
+                        if (ti < job.train_count) {
+                            train = job.train_base + uint32_t(ti);
+                            a.b.rec_count[train] = 0u;
+                        }
+                    }
+                }
                 Rng rng = rng_for(job.seed, sample, pixel, 0u);
                 const float jx = rng_float(rng), jy = rng_float(rng);
                 const float sx = (2.0f * (float(x) + jx) / float(job.width) - 1.0f) * job.tan_half * job.aspect;
@@ -107,6 +106,7 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             a.b.p_sample[i] = sample;
             a.b.p_thr[i] = make_float4(1.f, 1.f, 1.f, 0.f);
             a.b.p_L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            a.b.p_train[i] = train;
         }
         const uint32_t slot = wave_append(live, &a.b.counters[kCntExtA], a.b.rays);
         if (live) {
@@ -160,13 +160,14 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             V3 L = v3(L4.x, L4.y, L4.z);
             const float pdf_prev = thr4.w;
             const V3 dir = v3(ray.dir[0], ray.dir[1], ray.dir[2]);
-            const bool record = (job.flags & kJobTrainRecord) != 0u;
+            const uint32_t tslot = a.b.p_train[path];
+            const bool record = tslot != kNoTrainSlot;
             uint32_t rc = 0;
             if (record) {
                 // the previous diffuse vertex's target starts after its own light sample
-                rc = a.b.rec_count[path];
+                rc = a.b.rec_count[tslot];
                 if (rc > 0) {
-                    float4* last = a.b.rec + (size_t(path) * a.b.rec_max + rc - 1) * 4;
+                    float4* last = a.b.rec + (size_t(tslot) * a.b.rec_max + rc - 1) * 4;
                     if (last[3].w == 0.0f) last[3] = make_float4(L.x, L.y, L.z, 1.0f);
                 }
             }
@@ -245,15 +246,16 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                     }
                     const bool diffuse = mat.type == 0u;
                     if (record && diffuse && rc < a.b.rec_max) {
-                        float4* r = a.b.rec + (size_t(path) * a.b.rec_max + rc) * 4;
+                        float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + rc) * 4;
                         r[0] = make_float4(p.x, p.y, p.z, 0.0f);
                         r[1] = make_float4(nf.x, nf.y, nf.z, 0.0f);
                         r[2] = make_float4(thr.x * albedo.x, thr.y * albedo.y, thr.z * albedo.z, 0.0f);
                         r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                        a.b.rec_count[path] = rc + 1;
+                        a.b.rec_count[tslot] = rc + 1;
                     }
-                    if (diffuse && (job.flags & FND_JOB_USE_CACHE)) {
-                        // terminate into the cache: it stands for the indirect light leaving p
+                    if (diffuse && (job.flags & FND_JOB_USE_CACHE) && !record) {
+                        // terminate into the cache: it stands for the indirect light leaving p.
+                        // Training paths go on (their pixel gets the plain estimate).
                         query = true;
                         cq.pos = make_float4(p.x, p.y, p.z, 0.0f);
                         cq.nrm = make_float4(nf.x, nf.y, nf.z, 0.0f);
@@ -365,18 +367,21 @@ __global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
     }
 }
 
-// Training targets from finished training paths: at a recorded diffuse vertex
-// with weight = throughput * albedo, target = (L_path - L_mark) / weight, the
-// albedo-demodulated radiance the path gathered after that vertex's own light
-// sample (per channel; vertices with a vanishing weight are skipped).
+// Training targets from the finished training paths of a batch: at a
+// recorded diffuse vertex with weight = throughput * albedo, target =
+// (L_path - L_mark) / weight, the albedo-demodulated radiance the path
+// gathered after that vertex's own light sample (per channel; vertices with a
+// vanishing weight are skipped).
 __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint32_t n)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t rc = a.b.rec_count[i];
+        const uint32_t tslot = a.b.p_train[i];
+        if (tslot == kNoTrainSlot) continue;
+        const uint32_t rc = a.b.rec_count[tslot];
         if (rc == 0) continue;
         const float4 L = a.b.p_L[i];
         for (uint32_t k = 0; k < rc; ++k) {
-            const float4* r = a.b.rec + (size_t(i) * a.b.rec_max + k) * 4;
+            const float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + k) * 4;
             const float4 w = r[2];
             if (!(w.x > 1e-4f && w.y > 1e-4f && w.z > 1e-4f)) continue;
             const float4 m = r[3].w != 0.0f ? r[3] : L;

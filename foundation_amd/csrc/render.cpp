@@ -45,7 +45,7 @@ void render_state_free(fnd_pt_ctx* ctx)
     dev_free(ctx, rs->d_jobs, rs->jobs_cap * sizeof(JobDev));
     dev_free(ctx, rs->d_shade, rs->scenes_cap * sizeof(ShadeScene));
     dev_free(ctx, rs->d_scenes, rs->scenes_cap * sizeof(SceneDev));
-    delete rs;
+    ctx_delete(ctx, rs);
     ctx->render_state = nullptr;
 }
 
@@ -76,9 +76,9 @@ fnd_status ensure_capacity(fnd_pt_ctx* ctx, uint32_t cap)
         rs->allocs.clear();
         jobs_cap = rs->jobs_cap; scenes_cap = rs->scenes_cap;
         dj = rs->d_jobs; ds = rs->d_shade; dsc = rs->d_scenes;
-        delete rs;
+        ctx_delete(ctx, rs);
     }
-    rs = new (std::nothrow) RenderState();
+    rs = ctx_new<RenderState>(ctx);
     if (!rs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
     rs->jobs_cap = jobs_cap; rs->scenes_cap = scenes_cap;
     rs->d_jobs = dj; rs->d_shade = ds; rs->d_scenes = dsc;
@@ -89,6 +89,7 @@ fnd_status ensure_capacity(fnd_pt_ctx* ctx, uint32_t cap)
         if (st == FND_OK) st = alloc_into(ctx, rs, p, n);
     };
     A(&b.p_job, cap); A(&b.p_pixel, cap); A(&b.p_sample, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
+    A(&b.p_train, cap);
     for (int q = 0; q < 2; ++q) {
         A(&b.ext_rays[q], cap); A(&b.ext_path[q], cap); A(&b.ext_scene[q], cap);
     }
@@ -139,6 +140,33 @@ fnd_status ensure_tables(fnd_pt_ctx* ctx, uint32_t njobs, uint32_t nscenes)
     return FND_OK;
 }
 
+constexpr uint64_t kTrainShare = 4;
+
+uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Training-path selection of one job (see JobDev): `count` (at most a quarter)
+// of the job's first ts samples per pixel, ts = min(spp, (2^32 - 1) / (W * H)) so every product
+// below stays under 2^64.  The multipliers are primes near 2^32 / golden
+// ratio (Fibonacci hashing spreads consecutive t over the image); at most one
+// of them can divide the modulus.
+void select_training(JobDev& j, uint32_t spp, uint32_t base, uint32_t count, uint64_t frame)
+{
+    const uint64_t P = uint64_t(j.width) * j.height;
+    const uint64_t ts = std::max<uint64_t>(1, std::min<uint64_t>(spp, 0xFFFFFFFFull / P));
+    j.train_mod = P * ts;
+    j.train_mul = (j.train_mod % 2654435789ull) != 0 ? 2654435789ull : 2654435803ull;
+    j.train_add = mix64(j.seed ^ mix64(uint64_t(j.sample_offset) ^ (frame << 32))) % j.train_mod;
+    j.train_base = base;
+    // at most one path in kTrainShare trains, so most pixels keep the cached estimate
+    j.train_count = uint32_t(std::min<uint64_t>(count, std::max<uint64_t>(1, j.train_mod / kTrainShare)));
+}
+
 void camera_basis(const fnd_camera& c, uint32_t w, uint32_t h, JobDev& j)
 {
     double f[3], up[3] = {c.up[0], c.up[1], c.up[2]};
@@ -173,7 +201,7 @@ fnd_status fnd_pt_framebuffer_create(fnd_pt_ctx* ctx, uint32_t width, uint32_t h
     if (width == 0 || height == 0 || width > 16384 || height > 16384)
         return set_error(ctx, FND_E_INVALID_ARG, "framebuffer size %ux%u out of range", width, height);
     if (flags & ~FND_FB_FP64) return set_error(ctx, FND_E_INVALID_ARG, "unknown framebuffer flags");
-    auto* f = new (std::nothrow) Framebuffer();
+    auto* f = ctx_new<Framebuffer>(ctx);
     if (!f) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
     f->width = width;
     f->height = height;
@@ -182,7 +210,7 @@ fnd_status fnd_pt_framebuffer_create(fnd_pt_ctx* ctx, uint32_t width, uint32_t h
     void* p = nullptr;
     fnd_status st = dev_alloc(ctx, &p, n * sizeof(float));
     if (st != FND_OK) {
-        delete f;
+        ctx_delete(ctx, f);
         return st;
     }
     f->accum = static_cast<float*>(p);
@@ -190,7 +218,7 @@ fnd_status fnd_pt_framebuffer_create(fnd_pt_ctx* ctx, uint32_t width, uint32_t h
         st = dev_alloc(ctx, &p, n * sizeof(double));
         if (st != FND_OK) {
             dev_free(ctx, f->accum, n * sizeof(float));
-            delete f;
+            ctx_delete(ctx, f);
             return st;
         }
         f->accum64 = static_cast<double*>(p);
@@ -210,7 +238,7 @@ fnd_status fnd_pt_framebuffer_destroy(fnd_pt_ctx* ctx, fnd_handle h)
     const uint64_t n = uint64_t(f->width) * f->height * 3;
     dev_free(ctx, f->accum, n * sizeof(float));
     dev_free(ctx, f->accum64, n * sizeof(double));
-    delete f;
+    ctx_delete(ctx, f);
     ctx->framebuffers.erase(h);
     return FND_OK;
 }
@@ -337,10 +365,20 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     }
     CacheState* cs = cache_get(ctx);
     if (cache_scene && !cs->bounds_set) cache_set_bounds(cs, cache_scene->info.bounds_lo, cache_scene->info.bounds_hi, true);
+    if (any_train) {
+        // the frame's training paths: desc.train_paths split over the training jobs
+        uint32_t ntrain = 0;
+        for (uint32_t i = 0; i < num_jobs; ++i) ntrain += (jobs[i].flags & FND_JOB_TRAIN_CACHE) ? 1u : 0u;
+        const uint32_t per = std::max(1u, cs->desc.train_paths / ntrain);
+        uint32_t base = 0;
+        for (uint32_t i = 0; i < num_jobs; ++i) {
+            if (!(jobs[i].flags & FND_JOB_TRAIN_CACHE) || base >= kCacheTrainMaxPaths) continue;
+            select_training(hj[i], jobs[i].spp, base, std::min(per, kCacheTrainMaxPaths - base), cs->frames);
+            base += hj[i].train_count;
+        }
+    }
     const uint32_t cap = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
-    uint32_t train_cap = 0;
-    if (any_train) train_cap = std::min(std::max(1024u, ctx->tuning.queue_capacity), cs->desc.train_paths);
-    fnd_status st = ensure_capacity(ctx, std::max(cap, train_cap));
+    fnd_status st = ensure_capacity(ctx, cap);
     if (st != FND_OK) return st;
     st = ensure_tables(ctx, num_jobs, uint32_t(slots.size()));
     if (st != FND_OK) return st;
@@ -447,26 +485,16 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     if (st != FND_OK) return st;
     (void)hipEventRecord(ctx->ev[2], ctx->stream);
 
-    // ---- training frames: trace training paths, gather targets, optimise
-    for (uint32_t i = 0; i < num_jobs && any_train; ++i) {
-        if (!(jobs[i].flags & FND_JOB_TRAIN_CACHE)) continue;
-        JobDev tj = hj[i];
-        tj.flags = kJobTrainRecord;
-        tj.padded = cs->desc.train_paths;
-        tj.tile = 1;
-        tj.tiles_x = 1;
-        tj.path_base = 0;
-        tj.path_end = cs->desc.train_paths;
-        tj.max_depth = cs->desc.train_depth;
-        tj.seed = jobs[i].seed ^ 0xC0FFEE5EEDull;
-        tj.accum = nullptr;
-        tj.accum64 = nullptr;
-        st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, &tj, sizeof(JobDev), hipMemcpyHostToDevice, ctx->stream),
-                       "upload training job");
-        if (st != FND_OK) return st;
-        launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
-        st = run(1, cs->desc.train_paths, train_cap, tj.max_depth, false, true);
-        if (st != FND_OK) return st;
+    // ---- the frame; training paths among its own paths record their diffuse vertices
+    if (any_train) launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
+    st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice,
+                                       ctx->stream), "upload jobs");
+    if (st != FND_OK) return st;
+    st = run(num_jobs, total, cap, max_depth, any_use, any_train);
+    if (st != FND_OK) return st;
+
+    // ---- optimise on the samples the frame gathered (the next frame reads the result)
+    if (any_train) {
         uint32_t nsamp = 0;
         st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[kCntTrain], 4, hipMemcpyDeviceToHost, ctx->stream),
                        "read sample count");
@@ -487,13 +515,6 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         }
         cs->frames += 1;
     }
-
-    // ---- the frame itself
-    st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice,
-                                       ctx->stream), "upload jobs");
-    if (st != FND_OK) return st;
-    st = run(num_jobs, total, cap, max_depth, any_use, false);
-    if (st != FND_OK) return st;
     (void)hipEventRecord(ctx->ev[3], ctx->stream);
     st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "render");
     if (st != FND_OK) return st;

@@ -21,6 +21,10 @@
 
 namespace fnd {
 
+// Rays per trace launch: ray indices are formed as (grid threads + a 32-bit
+// work counter) and must not wrap.
+constexpr uint32_t kMaxRaysPerTrace = 0x7FFFFFFFu;
+
 struct SceneDev {
     const WideNode* nodes;
     const float4* tris;
@@ -86,6 +90,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
         ++sp;
     };
 
+    // The first fetch is static (lane gtid takes ray gtid), so a launch whose
+    // queue is short or empty retires its surplus waves without touching the
+    // shared counter; later fetches take ray blocks past the grid's first
+    // gtid range from that counter.
+    const uint32_t grid_threads = gridDim.x * BLOCK;
+    bool first_fetch = true;
     for (;;) {
         const uint64_t idle = __ballot(!active && !retired);
         const uint64_t act = __ballot(active);
@@ -93,8 +103,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const uint32_t nidle = uint32_t(__popcll(idle));
             const int leader = __builtin_ctzll(idle);
             uint32_t base = 0;
-            if (int(__lane_id()) == leader) base = atomicAdd(args.counter, nidle);
-            base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+            if (first_fetch) {
+                base = gtid - __lane_id();  // every lane is idle: idx = gtid
+                first_fetch = false;
+            } else {
+                if (int(__lane_id()) == leader) base = grid_threads + atomicAdd(args.counter, nidle);
+                base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+            }
             if (!active && !retired) {
                 const uint32_t idx = base + lanes_below(idle);
                 if (idx < nrays) {

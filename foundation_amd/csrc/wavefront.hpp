@@ -8,8 +8,16 @@
 
 namespace fnd {
 
-// JobDev.flags: FND_JOB_USE_CACHE (1) from the caller, and internally:
-constexpr uint32_t kJobTrainRecord = 0x10000u;  // training paths: random pixels, record diffuse vertices
+// JobDev.flags: FND_JOB_USE_CACHE (1) from the caller.
+//
+// Radiance-cache training paths are a subset of a job's own paths, chosen by
+// an affine bijection of the path's index q = sample_in_call * W * H + pixel:
+// t = (q * train_mul + train_add) mod train_mod with gcd(train_mul,
+// train_mod) = 1, and the path trains iff t < train_count.  The set has
+// exactly train_count members, and t is the path's record slot (plus
+// train_base), so selection needs no atomics and the image does not depend on
+// scheduling.
+constexpr uint32_t kNoTrainSlot = 0xFFFFFFFFu;
 
 struct JobDev {
     uint32_t scene_slot;
@@ -22,6 +30,11 @@ struct JobDev {
     uint32_t sample_offset;
     uint32_t flags;
     uint64_t seed;
+    uint32_t train_base;      // first record slot of this job's training paths
+    uint32_t train_count;     // training paths of this job in this call (0: none)
+    uint64_t train_mod;       // W * H * spp
+    uint64_t train_mul;       // coprime with train_mod
+    uint64_t train_add;       // < train_mod, changes every call
     uint64_t path_base;       // first global path index of this job in the render call
     uint64_t path_end;
     float eye[3];
@@ -66,6 +79,7 @@ struct WfBuffers {
     uint32_t* p_sample;
     float4* p_thr;            // throughput, w = solid-angle pdf of the last BSDF sample (0: camera)
     float4* p_L;              // radiance
+    uint32_t* p_train;        // training record slot, or kNoTrainSlot
     // extension queues (double-buffered) and hits
     Ray* ext_rays[2];
     uint32_t* ext_path[2];
@@ -84,7 +98,7 @@ struct WfBuffers {
     uint32_t* q_path;
     float4* q_weight;         // throughput * albedo at the query vertex
     float4* q_pred;
-    // training records: per path slot, rec_max x (pos, normal, weight, L mark (w = 1 once set))
+    // training records: per training slot, rec_max x (pos, normal, weight, L mark (w = 1 once set))
     float4* rec;
     uint32_t* rec_count;
     uint32_t rec_max;

@@ -269,7 +269,8 @@ fnd_status fnd_pt_buffer_read(fnd_pt_ctx* ctx, fnd_handle buffer, uint64_t offse
 
 /* ---------------------------------------------------------------- ray queries */
 
-/* Host arrays in and out (uploads, traces, downloads). */
+/* At most 2^31 - 1 rays per call (FND_E_INVALID_ARG beyond).
+ * Host arrays in and out (uploads, traces, downloads). */
 fnd_status fnd_pt_trace(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, uint32_t num_rays,
                         fnd_hit* hits, uint32_t flags);
 /* Device-resident rays and hits (fnd_ray / fnd_hit arrays in buffers). */
