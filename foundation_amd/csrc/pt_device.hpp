@@ -94,9 +94,13 @@ __device__ __forceinline__ bool isect_tri(const RayK& r, const float4 a, const f
     const float By = fmaf(-r.Sy, Bkz, Bky);
     const float Cx = fmaf(-r.Sx, Ckz, Ckx);
     const float Cy = fmaf(-r.Sy, Ckz, Cky);
-    const float U = fmaf(Cx, By, -(Cy * Bx));
-    const float V = fmaf(Ax, Cy, -(Ay * Cx));
-    const float W = fmaf(Bx, Ay, -(By * Ax));
+    // Plain products (no fma): the edge function of an edge seen from either of
+    // its triangles is then the exact negation (products commute, x - y =
+    // -(y - x)), which keeps shared edges watertight and gives det == 0 for a
+    // triangle with two equal vertices.
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
     const float det = (U + V) + W;
     if (det == 0.0f) return false;

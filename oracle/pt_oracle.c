@@ -113,9 +113,12 @@ static int oray_isect(const oray* r, const float* v0, const float* v1, const flo
     const float By = fmaf(-r->Sy, B[kz], B[ky]);
     const float Cx = fmaf(-r->Sx, C[kz], C[kx]);
     const float Cy = fmaf(-r->Sy, C[kz], C[ky]);
-    const float U = fmaf(Cx, By, -(Cy * Bx));
-    const float V = fmaf(Ax, Cy, -(Ay * Cx));
-    const float W = fmaf(Bx, Ay, -(By * Ax));
+    /* Plain products, no fma (-ffp-contract=off): the edge function of an
+     * edge seen from either of its triangles is the exact negation, which keeps
+     * shared edges watertight and gives det == 0 for two equal vertices. */
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
     const float det = (U + V) + W;
     if (det == 0.0f) return 0;

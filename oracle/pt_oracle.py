@@ -86,6 +86,26 @@ def intersect(ray8, tri9):
     return bool(hit), out
 
 
+def bsdf_sample(mtype, albedo, rough, wo, u1, u2):
+    """Local frame (z = normal).  Returns (ok, wi[3], weight = f cos / pdf [3], pdf)."""
+    a = np.ascontiguousarray(albedo, np.float32)
+    w = np.ascontiguousarray(wo, np.float32)
+    out = np.zeros(7, np.float32)
+    ok = load().orc_bsdf_sample(int(mtype), a.ctypes.data, float(rough), w.ctypes.data, float(u1), float(u2),
+                                out.ctypes.data)
+    return bool(ok), out[0:3].copy(), out[3:6].copy(), float(out[6])
+
+
+def bsdf_eval(mtype, albedo, rough, wo, wi):
+    """Local frame.  Returns (f * cos(theta_i) [3], pdf of bsdf_sample)."""
+    a = np.ascontiguousarray(albedo, np.float32)
+    o = np.ascontiguousarray(wo, np.float32)
+    i = np.ascontiguousarray(wi, np.float32)
+    out = np.zeros(4, np.float32)
+    load().orc_bsdf_eval(int(mtype), a.ctypes.data, float(rough), o.ctypes.data, i.ctypes.data, out.ctypes.data)
+    return out[0:3].copy(), float(out[3])
+
+
 class OracleScene:
     """Meshes (unindexed, 9 floats per triangle) + optional instances, binary
     SAH trees built by the oracle itself."""

@@ -93,9 +93,10 @@ static int wray_isect(const wray* r, const float* a, const float* b, const float
     const float Ax = fmaf(-r->Sx, Akz, Akx), Ay = fmaf(-r->Sy, Akz, Aky);
     const float Bx = fmaf(-r->Sx, Bkz, Bkx), By = fmaf(-r->Sy, Bkz, Bky);
     const float Cx = fmaf(-r->Sx, Ckz, Ckx), Cy = fmaf(-r->Sy, Ckz, Cky);
-    const float U = fmaf(Cx, By, -(Cy * Bx));
-    const float V = fmaf(Ax, Cy, -(Ay * Cx));
-    const float W = fmaf(Bx, Ay, -(By * Ax));
+    /* plain products, as on the device (exact negation across a shared edge) */
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
     if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return 0;
     const float det = (U + V) + W;
     if (det == 0.0f) return 0;

@@ -1,0 +1,337 @@
+"""Pins the oracle's geometry (intersection, BVH traversal, wide-BVH walk) and
+the product's host encoder against checks that do not restate their formulas:
+
+* intersection: an exact rational barycentric solve (Cramer's rule over
+  Python fractions), a different method than the sheared edge functions;
+  a watertight fan whose shared edges and vertex are hit by rays built in
+  exact dyadic arithmetic; degenerate triangles;
+* traversal: brute force over every triangle (and every instance), which needs
+  no tree; the committed fixture of 4096 brute-force hit records;
+* the wide walk over the encoder's 64-byte nodes against the binary-tree walk;
+* encoder: every dequantised child box contains the exact bounds of what lies
+  below it; every triangle sits in exactly one leaf;
+* generators: closed-form counts and committed SHA-256 fixtures.
+"""
+import hashlib
+import json
+import os
+from fractions import Fraction as F
+
+import numpy as np
+import pytest
+
+from foundation_amd import _abi as A
+from foundation_amd import pt
+from foundation_amd import rays as R
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def isect(oracle, o, d, tri, tmin=0.0, tmax=3e38):
+    r = np.array(list(o) + [tmin] + list(d) + [tmax], np.float32)
+    return oracle.intersect(r, np.asarray(tri, np.float32).reshape(9))
+
+
+def exact_bary(o, d, tri):
+    """Solve o + t d = (1-u-v) v0 + u v1 + v v2 exactly; None if singular."""
+    o = [F(float(x)) for x in o]
+    d = [F(float(x)) for x in d]
+    v0, v1, v2 = ([F(float(x)) for x in tri[3 * k:3 * k + 3]] for k in range(3))
+    e1 = [v1[i] - v0[i] for i in range(3)]
+    e2 = [v2[i] - v0[i] for i in range(3)]
+    rhs = [o[i] - v0[i] for i in range(3)]
+    # columns: u * e1 + v * e2 - t * d = rhs
+    m = [[e1[i], e2[i], -d[i]] for i in range(3)]
+
+    def det3(a):
+        return (a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0])
+                + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]))
+
+    D = det3(m)
+    if D == 0:
+        return None
+    sol = []
+    for c in range(3):
+        mc = [[rhs[i] if j == c else m[i][j] for j in range(3)] for i in range(3)]
+        sol.append(det3(mc) / D)
+    return sol  # u, v, t
+
+
+def test_intersection_matches_exact_barycentric_solve(oracle):
+    rng = np.random.default_rng(7)
+    checked = hits = 0
+    for k in range(3000):
+        tri = rng.normal(size=9).astype(np.float32)
+        o = (rng.normal(size=3) * 3).astype(np.float32)
+        # aim at a point with barycentrics in [-0.3, 1.3] so both outcomes occur
+        b = rng.uniform(-0.3, 1.3, 2)
+        p = tri[0:3] + b[0] * (tri[3:6] - tri[0:3]) + b[1] * (tri[6:9] - tri[0:3])
+        d = (p - o).astype(np.float32)
+        if k % 7 == 0:
+            d[rng.integers(3)] = 0.0  # axis-parallel components
+        ex = exact_bary(o, d, tri)
+        got, out = isect(oracle, o, d, tri)
+        if ex is None:
+            continue
+        u, v, t = ex
+        w = 1 - u - v
+        margin = min(abs(u), abs(v), abs(w))
+        if margin < F(1, 10**4):
+            continue  # within rounding of an edge: FP32 may decide either way
+        inside = u > 0 and v > 0 and w > 0
+        assert got == inside, (k, float(u), float(v), float(t))
+        checked += 1
+        if inside:
+            hits += 1
+            # FP32 error of T / det scales with the distances involved, not with t
+            scale = max(np.linalg.norm(tri[3 * j:3 * j + 3] - o) for j in range(3)) / np.linalg.norm(d)
+            assert abs(float(out[0]) - float(t)) <= 1e-4 * scale, (k, float(out[0]), float(t), scale)
+            np.testing.assert_allclose(out[1], float(u), atol=1e-4)
+            np.testing.assert_allclose(out[2], float(v), atol=1e-4)
+    assert checked > 2500 and 0.1 * checked < hits < 0.9 * checked
+
+
+def test_degenerate_triangles_never_hit(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(500):
+        a = rng.normal(size=3).astype(np.float32)
+        b = rng.normal(size=3).astype(np.float32)
+        o = (rng.normal(size=3) * 3).astype(np.float32)
+        d = ((a + b) / 2 - o).astype(np.float32)
+        for tri in (np.concatenate([a, a, b]), np.concatenate([a, b, b]), np.concatenate([a, a, a])):
+            assert not isect(oracle, o, d, tri)[0]
+
+
+def _fan(n, rng):
+    """A closed fan of n triangles around the origin in the plane z = 0 with
+    vertex coordinates on a 1/64 grid, so halves and quarters of them and
+    sums with 1/64-grid offsets are exact in FP32."""
+    ang = np.sort(rng.uniform(0, 2 * np.pi, n)) if n > 3 else np.array([0.3, 2.4, 4.4])
+    rad = rng.uniform(0.5, 1.0, n)
+    vs = np.stack([np.round(rad * np.cos(ang) * 64) / 64, np.round(rad * np.sin(ang) * 64) / 64, np.zeros(n)], 1)
+    tris = [np.concatenate([[0, 0, 0], vs[i], vs[(i + 1) % n]]) for i in range(n)]
+    return vs.astype(np.float32), np.array(tris, np.float32)
+
+
+@pytest.mark.parametrize("n", [3, 5, 8, 13])
+def test_watertight_fan(oracle, n):
+    """Rays through the shared vertex and through points of the shared edges
+    hit at least one triangle of the fan (and never pass between them)."""
+    rng = np.random.default_rng(n)
+    vs, tris = _fan(n, rng)
+    targets = [np.zeros(3, np.float32)]
+    for v in vs:
+        targets += [v / 2, v / 4, 3 * v / 4]
+    scene = oracle.OracleScene()
+    scene.add_mesh(tris)
+    scene.build()
+    rays = []
+    for p in targets:
+        for _ in range(12):
+            off = np.round(rng.uniform(-2, 2, 3) * 64) / 64
+            off[2] = abs(off[2]) + 0.25 if rng.uniform() < 0.5 else -(abs(off[2]) + 0.25)
+            o = (p + off).astype(np.float32)
+            assert np.all(o - off == p)  # exact in FP32
+            rays.append(list(o) + [0.0] + list((-off).astype(np.float32)) + [3e38])
+    rays = np.array(rays, np.float32)
+    for r in rays:
+        assert any(isect(oracle, r[0:3], r[4:7], t)[0] for t in tris), r
+    hits = scene.trace(rays)
+    assert np.all(hits[:, 1].view(np.uint32) != 0xFFFFFFFF)
+    np.testing.assert_allclose(hits[:, 0], 1.0, rtol=1e-6)
+
+
+def _soup(rng, n, scale=1.0):
+    c = rng.uniform(-1, 1, (n, 1, 3))
+    e = rng.normal(size=(n, 3, 3)) * rng.uniform(0.005, 0.2, (n, 1, 1))
+    return ((c + e) * scale).reshape(n, 9).astype(np.float32)
+
+
+def _random_rays(rng, n, spread=2.0):
+    o = rng.uniform(-spread, spread, (n, 3))
+    d = rng.normal(size=(n, 3))
+    tmax = np.where(rng.uniform(size=n) < 0.2, rng.uniform(0.1, 3.0, n), 3e38)
+    return np.concatenate([o, np.zeros((n, 1)), d, tmax[:, None]], 1).astype(np.float32)
+
+
+def test_bvh_walk_equals_brute_force(oracle):
+    rng = np.random.default_rng(11)
+    s = oracle.OracleScene()
+    s.add_mesh(_soup(rng, 4000))
+    s.add_mesh(_soup(rng, 1500, 0.5))
+    s.build()
+    assert s.check() == 0
+    rays = _random_rays(rng, 3000)
+    a = s.trace(rays)
+    b = s.trace(rays, brute=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (a[:, 1].view(np.uint32) != 0xFFFFFFFF).mean() > 0.15
+    occ_a = s.trace(rays, anyhit=True)[:, 1].view(np.uint32) != 0xFFFFFFFF
+    occ_b = b[:, 1].view(np.uint32) != 0xFFFFFFFF
+    assert np.array_equal(occ_a, occ_b)
+
+
+def _rot(rng):
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_two_level_walk_equals_brute_force(oracle):
+    rng = np.random.default_rng(12)
+    s = oracle.OracleScene()
+    meshes = [_soup(rng, 600, 0.5), _soup(rng, 300, 0.3)]
+    for m in meshes:
+        s.add_mesh(m)
+    for i in range(12):
+        M = _rot(rng) * rng.uniform(0.5, 2.0)
+        t = rng.uniform(-2, 2, 3)
+        o2w = np.concatenate([M, t[:, None]], 1).astype(np.float32)
+        Mi = np.linalg.inv(M)
+        w2o = np.concatenate([Mi, (-Mi @ t)[:, None]], 1).astype(np.float32)
+        s.add_instance(i % 2, o2w.reshape(12), w2o.reshape(12))
+    s.build()
+    assert s.check() == 0
+    assert s.num_prims() == 6 * 600 + 6 * 300
+    rays = _random_rays(rng, 2000, 3.0)
+    a = s.trace(rays)
+    b = s.trace(rays, brute=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (a[:, 1].view(np.uint32) != 0xFFFFFFFF).mean() > 0.1
+
+
+def test_golden_cornell_hits(oracle):
+    z = np.load(os.path.join(GOLDEN, "cornell_hits.npz"))
+    gen = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    s = oracle.scene_from_generated(gen)
+    hits = s.trace(z["rays"])
+    assert np.array_equal(hits.view(np.uint32), z["hits"].view(np.uint32))
+
+
+def _encode(gen):
+    return pt.encode_scene(gen.desc)
+
+
+def _dequant(node):
+    e = np.asarray(node["e"], np.float64)
+    p = np.asarray(node["p"], np.float64)
+    lo = np.zeros((4, 3))
+    hi = np.zeros((4, 3))
+    for a in range(3):
+        for k in range(4):
+            lo[k, a] = p[a] + float((int(node["qlo"][a]) >> (8 * k)) & 0xFF) * 2.0 ** e[a]
+            hi[k, a] = p[a] + float((int(node["qhi"][a]) >> (8 * k)) & 0xFF) * 2.0 ** e[a]
+    return lo, hi
+
+
+@pytest.mark.parametrize("kind,kw", [(A.GEN_CORNELL_SOUP, {"soup_triangles": 3000}),
+                                     (A.GEN_DISPLACED_SPHERE, {"subdivision": 24, "soup_triangles": 2000})])
+def test_encoder_boxes_are_conservative_and_leaves_partition(kind, kw):
+    gen = pt.GeneratedScene(kind, **kw)
+    enc = _encode(gen)
+    nodes, tris = enc["nodes"], enc["tris"]
+    tv = np.stack([tris["v0"], tris["v1"], tris["v2"]], 1).astype(np.float64)  # (T, 3, 3)
+    seen = np.zeros(len(tris), np.int64)
+
+    def subtree(ni):
+        """(lo, hi) exact bounds of everything under node ni, checking its slots."""
+        node = nodes[ni]
+        lo, hi = _dequant(node)
+        imask = int(node["imask"])
+        meta = int(node["meta"])
+        child = int(node["child_base"])
+        prim = int(node["prim_base"])
+        rank = 0
+        blo = np.full(3, np.inf)
+        bhi = np.full(3, -np.inf)
+        for k in range(4):
+            if (imask >> k) & 1:
+                clo, chi = subtree(child + rank)
+                rank += 1
+            else:
+                m = (meta >> (8 * k)) & 0xFF
+                if m == 0:
+                    continue
+                first, count = prim + (m >> 3), m & 7
+                assert 1 <= count <= 4
+                seen[first:first + count] += 1
+                v = tv[first:first + count].reshape(-1, 3)
+                clo, chi = v.min(0), v.max(0)
+            assert np.all(lo[k] <= clo) and np.all(hi[k] >= chi), (ni, k)
+            blo = np.minimum(blo, clo)
+            bhi = np.maximum(bhi, chi)
+        return blo, bhi
+
+    subtree(int(enc["root"]))
+    assert np.all(seen == 1)
+
+
+def test_wide_walk_equals_binary_walk(oracle):
+    rng = np.random.default_rng(5)
+    for kind, kw in [(A.GEN_CORNELL_SOUP, {}), (A.GEN_INSTANCED_SPHERES, {"instances": 6, "subdivision": 8})]:
+        gen = pt.GeneratedScene(kind, **kw)
+        enc = _encode(gen)
+        binary = oracle.scene_from_generated(gen)
+        wt = np.asarray(gen.world_triangles(), np.float64).reshape(-1, 3)
+        lo, hi = wt.min(0), wt.max(0)
+        n = 4000
+        o = rng.uniform(lo, hi, (n, 3))
+        d = rng.normal(size=(n, 3))
+        rays = np.concatenate([o, np.zeros((n, 1)), d, np.full((n, 1), 3e38)], 1).astype(np.float32)
+        prim = R.primary_rays(gen.camera, 48, 32)
+        rays = np.concatenate([rays, prim], 0)
+        for anyhit in (False, True):
+            w, _ = oracle.trace_wide(enc, rays, anyhit=anyhit)  # also checks the stack bound
+            b = binary.trace(rays, anyhit=anyhit)
+            if anyhit:
+                assert np.array_equal(w[:, 1].view(np.uint32) != 0xFFFFFFFF, b[:, 1].view(np.uint32) != 0xFFFFFFFF)
+            else:
+                assert np.array_equal(w.view(np.uint32), b.view(np.uint32))
+
+
+def _scene_hash(gen):
+    h = hashlib.sha256()
+    for m in gen.meshes():
+        for k in ("positions", "indices", "material", "light"):
+            h.update(np.ascontiguousarray(m[k]).tobytes())
+    for inst in gen.instances():
+        h.update(inst["o2w"].tobytes())
+        h.update(inst["w2o"].tobytes())
+    return h.hexdigest()
+
+
+def test_generators_match_fixtures_and_closed_forms():
+    want = json.load(open(os.path.join(GOLDEN, "generator_hashes.json")))
+    cases = {
+        "cornell_soup": (A.GEN_CORNELL_SOUP, {}),
+        "sphere_n32_soup5k": (A.GEN_DISPLACED_SPHERE, {"subdivision": 32, "soup_triangles": 5000}),
+        "instanced_8x_n10": (A.GEN_INSTANCED_SPHERES, {"instances": 8, "subdivision": 10}),
+        "furnace_n8": (A.GEN_FURNACE, {"subdivision": 8}),
+    }
+    for name, (kind, kw) in cases.items():
+        gen = pt.GeneratedScene(kind, **kw)
+        assert _scene_hash(gen) == want[name], name
+    # Cornell: 5 quads + 2 boxes (10 + 24 triangles) + light quad + soup
+    gen = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    m = gen.meshes()
+    ntri = sum(len(x["indices"]) for x in m)
+    assert ntri == 34 + 2 + 10000
+    # icosphere with n x n face subdivision: 20 n^2 triangles, |r - 1| <= amplitude
+    gen = pt.GeneratedScene(A.GEN_DISPLACED_SPHERE, subdivision=16, soup_triangles=0)
+    sph = gen.meshes()[0]
+    assert len(sph["indices"]) >= 20 * 16 * 16
+    pos = sph["positions"][np.unique(sph["indices"][: 20 * 16 * 16])]
+    r = np.linalg.norm(pos, axis=1)
+    assert np.all(np.abs(r - 1.0) <= 0.1 + 1e-5)
+    # V - E + F = 2 on the sphere part
+    f = sph["indices"][: 20 * 16 * 16]
+    edges = {tuple(sorted(e)) for t in f for e in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0]))}
+    assert len(np.unique(f)) - len(edges) + len(f) == 2
+    # instances: rotation part orthonormal up to the uniform scale in [0.5, 2]
+    gen = pt.GeneratedScene(A.GEN_INSTANCED_SPHERES, instances=10, subdivision=6)
+    for inst in gen.instances()[:-1]:
+        M = inst["o2w"].reshape(3, 4)[:, :3].astype(np.float64)
+        s = np.cbrt(np.linalg.det(M))
+        assert 0.5 - 1e-5 <= s <= 2 + 1e-5
+        np.testing.assert_allclose(M.T @ M / s ** 2, np.eye(3), atol=1e-5)
+        W = inst["w2o"].reshape(3, 4)[:, :3].astype(np.float64)
+        np.testing.assert_allclose(W @ M, np.eye(3), atol=1e-5)
