@@ -14,6 +14,14 @@ steps; roofline of the traversal kernel from its algorithmic bytes (32 B ray +
 instance, counted by the kernel's counting variant over the same rays) over
 its mean launch time from HIP events; the CPU oracle timed on the same rays
 on the host cores (cpu_baseline); and hit-record parity GPU vs oracle.
+
+`value` is the first half of the metric (Mrays/s).  The second half is in
+extra.time_to_rel_mse_0_01: progressive 1 spp/frame path tracing of the
+displaced-sphere scene (1.31M-triangle icosphere + 200k soup, 4 quad lights,
+1920x1080, depth 8, RR from 3) until relMSE <= 0.01 against a 4096-spp FP64
+GPU reference, plain and with the radiance cache trained every frame; device
+seconds of the frames are summed.  extra.path_tracing_10M is samples/s of
+full path tracing on the 10M-triangle scene.  --no-render skips both legs.
 Progress goes to stderr.
 """
 import argparse
@@ -62,6 +70,78 @@ def time_gpu_set(ctx, scene, rays, steps, warmup, flags):
     return wall, kernel_ms, hits, counts
 
 
+def render_throughput(ctx, scene, camera, width, height, spp, depth, rr):
+    """Samples/s of full path tracing (NEE + MIS + RR) on a scene already built."""
+    from foundation_amd import pt
+    fb = ctx.framebuffer_create(width, height)
+    ctx.render([pt.make_job(scene, fb, camera, 1, depth, rr_depth=rr, seed=1)])  # warm-up
+    ctx.reset_stats()
+    t = time.perf_counter()
+    ctx.render([pt.make_job(scene, fb, camera, spp, depth, rr_depth=rr, seed=1, sample_offset=1)])
+    wall = time.perf_counter() - t
+    st = ctx.stats()
+    ctx.framebuffer_destroy(fb)
+    ms = st.last_call_ms
+    return {"workload": f"{width}x{height}, {spp} spp, depth {depth}, RR from {rr}, NEE+MIS on 4 quad lights",
+            "msamples_s": round(width * height * spp / ms / 1e3, 2), "ms_per_spp": round(ms / spp, 3),
+            "render_mrays_s": round(st.render_rays / ms / 1e3, 1), "wall_ms": round(wall * 1e3, 1)}
+
+
+def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
+    """Progressive 1 spp/frame rendering of the displaced-sphere scene, plain and
+    with the radiance cache (trained every frame on the frame's own paths),
+    until relMSE <= target against an FP64 GPU reference of ref_spp samples.
+    Device time of the frames is summed (relMSE evaluation excluded)."""
+    import ctypes as C
+    from foundation_amd import _abi as A
+    from foundation_amd import pt
+    from foundation_amd.metrics import rel_mse
+    gen = pt.GeneratedScene(A.GEN_DISPLACED_SPHERE)
+    scene = ctx.scene_create(gen.desc)
+    out = {"scene": "displaced icosphere (1.31M tris) + 200k soup, 4 quad lights, 50/50 Lambert/GGX",
+           "width": width, "height": height, "max_depth": depth, "rr_depth": rr, "target": target}
+    try:
+        ref_fb = ctx.framebuffer_create(width, height, fp64=True)
+        t = time.perf_counter()
+        chunk = 256
+        for k in range(ref_spp // chunk):
+            ctx.render([pt.make_job(scene, ref_fb, gen.camera, chunk, depth, rr_depth=rr, seed=99,
+                                    sample_offset=k * chunk)])
+        ref = ctx.framebuffer_read(ref_fb, width, height, fp64=True)[0]
+        ctx.framebuffer_destroy(ref_fb)
+        out["reference"] = {"spp": ref_spp, "accumulation": "fp64", "seconds": round(time.perf_counter() - t, 1)}
+        log(f"relMSE reference: {ref_spp} spp FP64 in {out['reference']['seconds']} s")
+        for mode in ("plain", "cache"):
+            flags = 0
+            if mode == "cache":
+                d = A.CacheDesc()
+                ctx.lib.fnd_pt_cache_default_desc(C.byref(d))
+                ctx._ck(ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d)))
+                flags = A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE
+            fb = ctx.framebuffer_create(width, height)
+            gpu_ms = 0.0
+            e = float("inf")
+            f = 0
+            while f < max_frames and e > target:
+                ctx.render([pt.make_job(scene, fb, gen.camera, 1, depth, rr_depth=rr, seed=5, sample_offset=f,
+                                        flags=flags)])
+                gpu_ms += ctx.stats().last_call_ms
+                f += 1
+                e = rel_mse(ctx.framebuffer_read(fb, width, height)[0], ref)
+            ctx.framebuffer_destroy(fb)
+            out[mode] = {"frames": f, "seconds": round(gpu_ms / 1e3, 3), "ms_per_frame": round(gpu_ms / f, 3),
+                         "rel_mse": round(e, 5), "reached": bool(e <= target)}
+            log(f"relMSE {mode}: {e:.4g} after {f} frames, {gpu_ms / 1e3:.2f} s")
+        if out["plain"]["reached"] and out["cache"]["reached"]:
+            out["speedup_cache_vs_plain"] = round(out["plain"]["seconds"] / out["cache"]["seconds"], 3)
+    finally:
+        d = A.CacheDesc()
+        d.enable = 0
+        ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d))
+        ctx.scene_destroy(scene)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +153,11 @@ def main():
     ap.add_argument("--subdivision", type=int, default=71)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, online CPUs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
+    ap.add_argument("--no-render", action="store_true",
+                    help="skip the path-tracing legs (samples/s and time to relMSE 0.01, plain vs cache)")
+    ap.add_argument("--relmse-width", type=int, default=1920)
+    ap.add_argument("--relmse-height", type=int, default=1080)
+    ap.add_argument("--relmse-ref-spp", type=int, default=4096)
     ap.add_argument("--pmc-summary", default=os.path.join(ROOT, "profiles", "trace_incoherent_10M_pmc.json"))
     ap.add_argument("--only-incoherent", action="store_true",
                     help="profiling runs: time only the incoherent set (no coherent/shadow sets, no CPU leg)")
@@ -177,8 +262,16 @@ def main():
                   "shadow_rays": int(shadow.shape[0]), "anyhit_mismatches": int((socc != oocc).sum())}
         log(f"cpu: {cpu['value']} Mrays/s on {threads} threads; parity {parity}")
 
+    render = relmse = None
+    if not args.no_render and not args.only_incoherent:
+        render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 4, 8, 3)
+        log(f"path tracing @10M tris: {render}")
+        ctx.scene_destroy(scene)  # free the 10M-triangle scene before the next one
+        scene = None
+        relmse = time_to_rel_mse(ctx, args.relmse_width, args.relmse_height, 8, 3, 0.01, args.relmse_ref_spp, 1500)
+
     line = {
-        "metric": "Mrays/s incoherent @10M tris",
+        "metric": "Mrays/s incoherent @10M tris; time-to-relMSE 0.01 (cache vs plain)",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -204,7 +297,8 @@ def main():
                   "coherent_nodes_per_ray": round(ccounts["nodes"] / primary.shape[0], 3),
                   "scene_nodes": int(info.num_nodes), "scene_build_ms": round(info.build_ms, 1),
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
-                  "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None},
+                  "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+                  "path_tracing_10M": render, "time_to_rel_mse_0_01": relmse},
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -412,13 +412,22 @@ __global__ __launch_bounds__(256) void k_cache_dw(const uint16_t* act, const uin
 
 // Adam (Kingma & Ba 2015) with bias-correction factors bc1 = 1 - b1^t,
 // bc2 = 1 - b2^t; optionally writes the FP16 copy the forward pass reads.
-__global__ __launch_bounds__(256) void k_adam(float* p, const float* gr, float* m, float* v, uint64_t n, float lr,
+// LAZY (the hash grid): a pair of entries whose gradients are both exactly 0
+// is left alone, moments included (Mueller et al. 2022, section 4: updates of
+// zero-gradient table entries are skipped), and every gradient read is
+// cleared, so the next step starts from zero without a separate pass.
+template <bool LAZY>
+__global__ __launch_bounds__(256) void k_adam(float* p, float* gr, float* m, float* v, uint64_t n, float lr,
                                               float b1, float b2, float eps, float bc1, float bc2, uint32_t* out_half2)
 {
     const uint64_t pairs = n / 2;
     for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < pairs; i += uint64_t(gridDim.x) * 256u) {
-        float2 pp = reinterpret_cast<float2*>(p)[i];
         const float2 gg = reinterpret_cast<const float2*>(gr)[i];
+        if (LAZY) {
+            if (gg.x == 0.0f && gg.y == 0.0f) continue;
+            reinterpret_cast<float2*>(gr)[i] = make_float2(0.0f, 0.0f);
+        }
+        float2 pp = reinterpret_cast<float2*>(p)[i];
         float2 mm = reinterpret_cast<float2*>(m)[i];
         float2 vv = reinterpret_cast<float2*>(v)[i];
         mm.x = b1 * mm.x + (1.0f - b1) * gg.x;
@@ -516,12 +525,15 @@ void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32
     hipLaunchKernelGGL(k_cache_dw, dim3(chunks, 6), dim3(256), 0, s, act, dz, n, stride, chunk, grad_mlp);
 }
 
-void launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
-                 float bc1, float bc2, uint32_t* out_half2, hipStream_t s)
+void launch_adam(float* p, float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+                 float bc1, float bc2, uint32_t* out_half2, bool lazy, hipStream_t s)
 {
     const uint64_t blocks = std::min<uint64_t>(8192u, (n / 2 + 255u) / 256u);
-    hipLaunchKernelGGL(k_adam, dim3(uint32_t(std::max<uint64_t>(1, blocks))), dim3(256), 0, s, p, g, m, v, n, lr, b1,
-                       b2, eps, bc1, bc2, out_half2);
+    const dim3 grid(uint32_t(std::max<uint64_t>(1, blocks)));
+    if (lazy)
+        hipLaunchKernelGGL(k_adam<true>, grid, dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2, out_half2);
+    else
+        hipLaunchKernelGGL(k_adam<false>, grid, dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2, out_half2);
 }
 
 void launch_arrange(const float* w, _Float16* wfwd, uint16_t* wbwd, hipStream_t s)

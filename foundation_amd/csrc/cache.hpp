@@ -79,8 +79,9 @@ void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t n, 
                            uint32_t stride, float* grad_grid, hipStream_t s);
 void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32_t stride, float* grad_mlp,
                      hipStream_t s);
-void launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
-                 float bc1, float bc2, uint32_t* out_half2, hipStream_t s);
+// lazy: skip pairs whose gradients are both 0 and clear the gradients read
+void launch_adam(float* p, float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+                 float bc1, float bc2, uint32_t* out_half2, bool lazy, hipStream_t s);
 void launch_arrange(const float* w, _Float16* wfwd, uint16_t* wbwd, hipStream_t s);
 void launch_zero(float* p, uint64_t n, hipStream_t s);
 

@@ -85,7 +85,10 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
     if (n > cs->tb.cap) return set_error(ctx, FND_E_INVALID_ARG, "cache batch of %u exceeds %u", n, cs->tb.cap);
     hipStream_t s = ctx->stream;
     launch_zero(cs->tb.grad_mlp, kMlpParams, s);
-    launch_zero(cs->tb.grad_grid, kGridParams, s);
+    // the lazy grid update clears what it reads; only a step without update
+    // (gradients for the caller) leaves the grid gradient to be cleared here
+    if (cs->grid_grad_dirty) launch_zero(cs->tb.grad_grid, kGridParams, s);
+    cs->grid_grad_dirty = !apply;
     launch_zero(cs->tb.loss, 1, s);
     const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
     launch_cache_forward(cs->view, q, n, nullptr, cs->tb.pred, cs->tb.act, cs->tb.cap, grid, s);
@@ -99,9 +102,9 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
         const float b1 = 0.9f, b2 = 0.99f, eps = 1e-15f;
         const float bc1 = 1.0f - std::pow(b1, float(cs->step)), bc2 = 1.0f - std::pow(b2, float(cs->step));
         launch_adam(cs->w_master, cs->tb.grad_mlp, cs->w_m, cs->w_v, kMlpParams, cs->desc.learning_rate, b1, b2, eps,
-                    bc1, bc2, nullptr, s);
+                    bc1, bc2, nullptr, false, s);
         launch_adam(cs->g_master, cs->tb.grad_grid, cs->g_m, cs->g_v, kGridParams, cs->desc.learning_rate, b1, b2,
-                    eps, bc1, bc2, cs->grid16, s);
+                    eps, bc1, bc2, cs->grid16, true, s);
         launch_arrange(cs->w_master, cs->wfwd, cs->wbwd, s);
         st = check_hip(ctx, hipGetLastError(), "cache update");
         if (st != FND_OK) return st;

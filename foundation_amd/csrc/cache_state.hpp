@@ -36,6 +36,7 @@ struct CacheState {
     uint32_t* rec_count = nullptr;
     CacheView view{};
     bool bounds_set = false;
+    bool grid_grad_dirty = false;     // grid gradient holds values no update has consumed
     uint32_t step = 0;
     float train_loss_host = 0;        // loss of the last step of the last training frame
     uint32_t last_samples = 0;        // training samples gathered in the last training frame

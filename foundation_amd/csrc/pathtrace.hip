@@ -374,25 +374,34 @@ __global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
 // vanishing weight are skipped).
 __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint32_t n)
 {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t tslot = a.b.p_train[i];
-        if (tslot == kNoTrainSlot) continue;
-        const uint32_t rc = a.b.rec_count[tslot];
-        if (rc == 0) continue;
-        const float4 L = a.b.p_L[i];
-        for (uint32_t k = 0; k < rc; ++k) {
-            const float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + k) * 4;
-            const float4 w = r[2];
-            if (!(w.x > 1e-4f && w.y > 1e-4f && w.z > 1e-4f)) continue;
-            const float4 m = r[3].w != 0.0f ? r[3] : L;
-            const uint32_t slot = atomicAdd(&a.b.counters[kCntTrain], 1u);
-            if (slot >= a.b.t_cap) return;
-            CacheQuery q;
-            q.pos = r[0];
-            q.nrm = r[1];
-            a.b.t_query[slot] = q;
-            a.b.t_target[slot] = make_float4(fmaxf(L.x - m.x, 0.0f) / w.x, fmaxf(L.y - m.y, 0.0f) / w.y,
-                                             fmaxf(L.z - m.z, 0.0f) / w.z, 0.0f);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t tslot = i < n ? a.b.p_train[i] : kNoTrainSlot;
+        const uint32_t rc = tslot != kNoTrainSlot ? a.b.rec_count[tslot] : 0u;
+        const float4 L = rc ? a.b.p_L[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // one wave-aggregated append per record index (every lane takes part)
+        for (uint32_t k = 0; k < a.b.rec_max; ++k) {
+            bool take = false;
+            CacheQuery q{};
+            float4 target{};
+            if (k < rc) {
+                const float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + k) * 4;
+                const float4 w = r[2];
+                if (w.x > 1e-4f && w.y > 1e-4f && w.z > 1e-4f) {
+                    const float4 m = r[3].w != 0.0f ? r[3] : L;
+                    q.pos = r[0];
+                    q.nrm = r[1];
+                    target = make_float4(fmaxf(L.x - m.x, 0.0f) / w.x, fmaxf(L.y - m.y, 0.0f) / w.y,
+                                         fmaxf(L.z - m.z, 0.0f) / w.z, 0.0f);
+                    take = true;
+                }
+            }
+            const uint32_t slot = wave_append(take, &a.b.counters[kCntTrain], nullptr);
+            if (take && slot < a.b.t_cap) {
+                a.b.t_query[slot] = q;
+                a.b.t_target[slot] = target;
+            }
         }
     }
 }

@@ -437,34 +437,42 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
             launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery), ctx->stream);
             launch_generate(wa, g0, n, grid, ctx->stream);
+            // One traversal launch per bounce: the extension rays of bounce b + 1
+            // (closest hit) and the shadow rays of bounce b (any hit) share it,
+            // so their tails overlap.  The shadow results are added before the
+            // next shade, in the same order as with separate launches.
+            TraceArgs ta{};
+            ta.scenes = rs->d_scenes;
+            ta.hits = rs->b.ext_hits;
+            ta.counter = &rs->b.counters[kCntWorkExt];
+            ta.spill = ctx->d_spill;
+            ta.spill_stride = tl.grid * tl.block;
+            ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
+            ta.rays2 = rs->b.sh_rays;
+            ta.hits2 = rs->b.sh_hits;
+            ta.ray_scene2 = rs->b.sh_scene;
+            ta.nrays2_dev = &rs->b.counters[kCntShadow];
+            TraceLaunch l = tl;
+            l.mode = kModeMixedInternal;
+            auto trace = [&](uint32_t q, const char* what) -> fnd_status {
+                ta.rays = rs->b.ext_rays[q];
+                ta.ray_scene = rs->b.ext_scene[q];
+                ta.nrays_dev = &rs->b.counters[q];
+                return check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
+            };
             uint32_t cur = kCntExtA;
+            launch_prep(wa, cur ^ 1u, ctx->stream);  // empty shadow queue, zero work counter
+            fnd_status e = trace(cur, "primary trace");
+            if (e != FND_OK) return e;
             for (uint32_t bounce = 0; bounce <= depth; ++bounce) {
                 const uint32_t nxt = cur ^ 1u;
                 launch_prep(wa, nxt, ctx->stream);
-                TraceArgs ta{};
-                ta.scenes = rs->d_scenes;
-                ta.ray_scene = rs->b.ext_scene[cur];
-                ta.rays = rs->b.ext_rays[cur];
-                ta.hits = rs->b.ext_hits;
-                ta.nrays_dev = &rs->b.counters[cur];
-                ta.counter = &rs->b.counters[kCntWorkExt];
-                ta.spill = ctx->d_spill;
-                ta.spill_stride = tl.grid * tl.block;
-                ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
-                TraceLaunch l = tl;
-                l.mode = kModeClosestInternal;
-                fnd_status e = check_hip(ctx, launch_trace(l, ta, ctx->stream), "extension trace");
-                if (e != FND_OK) return e;
                 launch_shade(wa, cur, bounce, grid, ctx->stream);
-                ta.ray_scene = rs->b.sh_scene;
-                ta.rays = rs->b.sh_rays;
-                ta.hits = rs->b.sh_hits;
-                ta.nrays_dev = &rs->b.counters[kCntShadow];
-                ta.counter = &rs->b.counters[kCntWorkShadow];
-                l.mode = kModeAnyInternal;
-                e = check_hip(ctx, launch_trace(l, ta, ctx->stream), "shadow trace");
-                if (e != FND_OK) return e;
-                launch_shadow_resolve(wa, grid, ctx->stream);
+                if (bounce < depth) {  // the last shade appends no rays
+                    e = trace(nxt, "extension + shadow trace");
+                    if (e != FND_OK) return e;
+                    launch_shadow_resolve(wa, grid, ctx->stream);
+                }
                 cur = nxt;
             }
             if (apply_cache) {
@@ -475,7 +483,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             }
             if (collect) launch_cache_collect(wa, n, grid, ctx->stream);
             launch_accumulate(wa, g0, n, grid, ctx->stream);
-            fnd_status e = check_hip(ctx, hipGetLastError(), "render launch");
+            e = check_hip(ctx, hipGetLastError(), "render launch");
             if (e != FND_OK) return e;
         }
         return FND_OK;

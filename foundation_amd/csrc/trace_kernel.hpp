@@ -37,7 +37,8 @@ enum TraceMode : int {
     kModeClosest = 0,          // fnd_hit records (global primitive ids)
     kModeAny = 1,              // fnd_hit records, first accepted hit
     kModeClosestInternal = 2,  // (t, leaf triangle index, instance slot) for shading
-    kModeAnyInternal = 3       // visibility for shadow rays
+    kModeAnyInternal = 3,      // visibility for shadow rays
+    kModeMixedInternal = 4     // one launch: set 1 closest (as 2), then set 2 any-hit (as 3)
 };
 
 struct TraceArgs {
@@ -53,6 +54,11 @@ struct TraceArgs {
     uint32_t spill_stride;     // threads in the grid
     uint32_t refill;
     unsigned long long* stats; // node visits, triangle tests, instance entries (COUNT)
+    // kModeMixedInternal: the second ray set (any-hit), indices after the first
+    const Ray* rays2;
+    float4* hits2;
+    const uint32_t* ray_scene2;
+    const uint32_t* nrays2_dev;
 };
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -63,13 +69,16 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
 template <int MODE, bool TWO, bool MULTI, bool COUNT, int SLDS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
 {
-    constexpr bool ANY = (MODE & 1) != 0;
+    constexpr bool MIXED = MODE == kModeMixedInternal;
+    constexpr bool ANY_CONST = !MIXED && (MODE & 1) != 0;
     constexpr bool INTERNAL = MODE >= kModeClosestInternal;
     __shared__ uint2 stk[SLDS * BLOCK];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
-    const uint32_t nrays = args.nrays_dev ? *args.nrays_dev : args.nrays;
+    const uint32_t nrays1 = args.nrays_dev ? *args.nrays_dev : args.nrays;
+    const uint32_t nrays = MIXED ? nrays1 + *args.nrays2_dev : nrays1;
+    bool lane_any = ANY_CONST;  // per lane in the mixed mode, a constant otherwise
 
     SceneDev sc = args.scene;
     bool active = false, retired = false;
@@ -113,11 +122,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             if (!active && !retired) {
                 const uint32_t idx = base + lanes_below(idle);
                 if (idx < nrays) {
-                    ray_idx = idx;
-                    const float4* rp = reinterpret_cast<const float4*>(args.rays + idx);
+                    const bool second = MIXED && idx >= nrays1;
+                    ray_idx = second ? idx - nrays1 : idx;
+                    if (MIXED) lane_any = second;
+                    const float4* rp = reinterpret_cast<const float4*>((second ? args.rays2 : args.rays) + ray_idx);
                     const float4 ra = rp[0];
                     const float4 rb = rp[1];
-                    if (MULTI) sc = args.scenes[args.ray_scene[idx]];
+                    if (MULTI) sc = args.scenes[(second ? args.ray_scene2 : args.ray_scene)[ray_idx]];
                     wox = ra.x; woy = ra.y; woz = ra.z; tmin = ra.w;
                     wdx = rb.x; wdy = rb.y; wdz = rb.z; tmax = rb.w;
                     ray_setup(r, wox, woy, woz, wdx, wdy, wdz);
@@ -185,7 +196,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 tk[k] = hit ? tn : __builtin_inff();
                 ek[k] = e;
             }
-            if (!ANY) {
+            const bool any_l = MIXED ? lane_any : ANY_CONST;
+            if (!any_l) {
                 // sorting network (0,1)(2,3)(0,2)(1,3)(1,2): nearest child first
                 auto cs = [&](int a, int b) {
                     const bool s = tk[b] < tk[a];
@@ -221,7 +233,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             float t, u, v;
             if (isect_tri(r, a, b, c, t, u, v) && t > tmin && t < tmax) {
                 const uint32_t id = INTERNAL ? ti : id_base + __float_as_uint(a.w);
-                if (ANY) {
+                if (MIXED ? lane_any : ANY_CONST) {
                     tbest = t; best = id; bu = u; bv = v; best_aux = inst_slot;
                     finish = true;
                 } else if (t < tbest || (t == tbest && id < best)) {
@@ -263,7 +275,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 }
                 --sp;
                 const uint2 e = sp < SLDS ? stk[sp * BLOCK + tid] : args.spill[size_t(sp - SLDS) * args.spill_stride + gtid];
-                if (ANY || __uint_as_float(e.y) <= tbest * kSlack) {
+                if ((MIXED ? lane_any : ANY_CONST) || __uint_as_float(e.y) <= tbest * kSlack) {
                     cur = e.x;
                     break;
                 }
@@ -278,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             } else {
                 out = make_float4(miss ? tmax : tbest, __uint_as_float(best), miss ? 0.f : bu, miss ? 0.f : bv);
             }
-            args.hits[ray_idx] = out;
+            ((MIXED && lane_any) ? args.hits2 : args.hits)[ray_idx] = out;
             active = false;
         }
     }

@@ -30,6 +30,8 @@ static hipError_t launch_cfg(const TraceLaunch& l, const TraceArgs& args, hipStr
             hipLaunchKernelGGL((k_trace<kModeClosestInternal, true, true, false, S, B>), grid, block, 0, stream, args);
         else if (l.mode == kModeAnyInternal)
             hipLaunchKernelGGL((k_trace<kModeAnyInternal, true, true, false, S, B>), grid, block, 0, stream, args);
+        else if (l.mode == kModeMixedInternal)
+            hipLaunchKernelGGL((k_trace<kModeMixedInternal, true, true, false, S, B>), grid, block, 0, stream, args);
         else
             return hipErrorInvalidValue;
         return hipGetLastError();
