@@ -7,6 +7,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
+ABI_VERSION = 2  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
 LIB_PATH = os.path.join(_HERE, "libfnd_pt.so")
 
 fnd_handle = C.c_uint64
@@ -197,6 +198,7 @@ class CacheDesc(C.Structure):
         ("train_paths", C.c_uint32),
         ("train_steps", C.c_uint32),
         ("train_depth", C.c_uint32),
+        ("train_batch", C.c_uint32),
         ("seed", C.c_uint64),
         ("bounds_lo", C.c_float * 3),
         ("bounds_hi", C.c_float * 3),
@@ -292,5 +294,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.fnd_pt_abi_version() != ABI_VERSION:
+        raise RuntimeError(f"{LIB_PATH} has ABI {lib.fnd_pt_abi_version()}, these bindings need {ABI_VERSION}: rebuild")
     _lib = lib
     return lib

@@ -132,9 +132,9 @@ __device__ __forceinline__ int bwd_l0(int r, int s) { return 18432 + (r * 2 + s)
 constexpr int kP = 1;  // 16-sample tiles per wave
 
 template <bool TRAIN>
-__global__ __launch_bounds__(256) void k_cache_forward(CacheView v, const CacheQuery* q, uint32_t n_arg,
-                                                       const uint32_t* n_dev, float4* pred, uint16_t* act,
-                                                       uint32_t stride)
+__global__ __launch_bounds__(256) void k_cache_forward(CacheView v, const CacheQuery* q, uint32_t qstride,
+                                                       uint32_t n_arg, const uint32_t* n_dev, float4* pred,
+                                                       uint16_t* act, uint32_t stride)
 {
     __shared__ h8 wl[kMlpFwdElems / 8];
     for (int i = threadIdx.x; i < kMlpFwdElems / 8; i += 256) wl[i] = reinterpret_cast<const h8*>(v.wfwd)[i];
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(256) void k_cache_forward(CacheView v, const CacheQ
             float x = 0, y = 0, z = 0;
             float4 nr = make_float4(0, 0, 0, 0);
             if (s < n) {
-                const CacheQuery cq = q[s];
+                const CacheQuery cq = q[size_t(s) * qstride];
                 normalise(v, cq.pos, x, y, z);
                 nr = cq.nrm;
                 bh[t] = encode4(v, x, y, z, g);
@@ -232,15 +232,15 @@ __global__ __launch_bounds__(256) void k_cache_forward(CacheView v, const CacheQ
 
 // Relative L2 (NRC): sum over RGB of (p - y)^2 / (sg(p)^2 + 0.01); the
 // gradient row dZ5 = 2 (p - y) / (p^2 + 0.01) in bf16, row 3 = 0.
-__global__ __launch_bounds__(256) void k_cache_loss(const float4* pred, const float4* target, uint32_t n, uint16_t* dz,
-                                                    uint32_t stride, float* loss)
+__global__ __launch_bounds__(256) void k_cache_loss(const float4* pred, const float4* target, uint32_t tstride,
+                                                    uint32_t n, uint16_t* dz, uint32_t stride, float* loss)
 {
     __shared__ float red[256];
     float acc = 0.0f;
     for (uint32_t s = blockIdx.x * 256u + threadIdx.x; s < stride; s += gridDim.x * 256u) {
         float gr[3] = {0, 0, 0};
         if (s < n) {
-            const float4 p = pred[s], y = target[s];
+            const float4 p = pred[s], y = target[size_t(s) * tstride];
             const float pv[3] = {p.x, p.y, p.z}, yv[3] = {y.x, y.y, y.z};
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_cache_loss(const float4* pred, const fl
     if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
-__global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const CacheQuery* q, uint32_t n,
+__global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const CacheQuery* q, uint32_t qstride, uint32_t n,
                                                         const uint16_t* act, uint16_t* dz, uint32_t stride,
                                                         float* grad_grid)
 {
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const Cache
             const uint32_t s = base + 16u * t + col;
             if (s >= n) continue;
             float x, y, z;
-            normalise(v, q[s].pos, x, y, z);
+            normalise(v, q[size_t(s) * qstride].pos, x, y, z);
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
 #pragma unroll 1
@@ -494,27 +494,28 @@ __global__ __launch_bounds__(256) void k_zero(float* p, uint64_t n)
 
 }  // namespace
 
-void launch_cache_forward(const CacheView& v, const CacheQuery* q, uint32_t n, const uint32_t* n_dev, float4* pred,
-                          uint16_t* act, uint32_t act_stride, uint32_t grid, hipStream_t s)
+void launch_cache_forward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint32_t* n_dev,
+                          float4* pred, uint16_t* act, uint32_t act_stride, uint32_t grid, hipStream_t s)
 {
     if (act)
-        hipLaunchKernelGGL(k_cache_forward<true>, dim3(grid), dim3(256), 0, s, v, q, n, n_dev, pred, act, act_stride);
+        hipLaunchKernelGGL(k_cache_forward<true>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, n_dev, pred, act,
+                           act_stride);
     else
-        hipLaunchKernelGGL(k_cache_forward<false>, dim3(grid), dim3(256), 0, s, v, q, n, n_dev, pred, act, 0u);
+        hipLaunchKernelGGL(k_cache_forward<false>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, n_dev, pred, act, 0u);
 }
 
-void launch_cache_loss(const float4* pred, const float4* target, uint32_t n, uint16_t* dz, uint32_t stride, float* loss,
-                       hipStream_t s)
+void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstride, uint32_t n, uint16_t* dz,
+                       uint32_t stride, float* loss, hipStream_t s)
 {
     const uint32_t grid = std::min<uint32_t>(1024u, (stride + 255u) / 256u);
-    hipLaunchKernelGGL(k_cache_loss, dim3(grid), dim3(256), 0, s, pred, target, n, dz, stride, loss);
+    hipLaunchKernelGGL(k_cache_loss, dim3(grid), dim3(256), 0, s, pred, target, tstride, n, dz, stride, loss);
 }
 
-void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t n, const uint16_t* act, uint16_t* dz,
-                           uint32_t stride, float* grad_grid, hipStream_t s)
+void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint16_t* act,
+                           uint16_t* dz, uint32_t stride, float* grad_grid, hipStream_t s)
 {
     const uint32_t grid = std::max(1u, std::min<uint32_t>(2048u, (n + 127u) / 128u));
-    hipLaunchKernelGGL(k_cache_backward, dim3(grid), dim3(256), 0, s, v, q, n, act, dz, stride, grad_grid);
+    hipLaunchKernelGGL(k_cache_backward, dim3(grid), dim3(256), 0, s, v, q, qstride, n, act, dz, stride, grad_grid);
 }
 
 void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32_t stride, float* grad_mlp,

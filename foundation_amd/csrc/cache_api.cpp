@@ -44,9 +44,10 @@ fnd_status validate(fnd_pt_ctx* ctx, const fnd_cache_desc& d)
                          "cache kernels are built for 16 levels x 2 features, 2^19 entries, resolutions 16..2048, "
                          "5 x 64 hidden");
     if (!(d.learning_rate > 0.0f) || d.train_paths == 0 || d.train_paths > kCacheTrainMaxPaths || d.train_steps == 0 ||
-        d.train_steps > 64 || d.train_depth == 0 || d.train_depth > kCacheMaxRecords)
+        d.train_steps > 64 || d.train_depth == 0 || d.train_depth > kCacheMaxRecords || d.train_batch < 256 ||
+        d.train_batch > kCacheBatch)
         return set_error(ctx, FND_E_INVALID_ARG, "cache: learning_rate > 0, train_paths in 1..65536, "
-                                                 "train_steps in 1..64, train_depth in 1..16");
+                                                 "train_steps in 1..64, train_depth in 1..16, train_batch in 256..65536");
     return FND_OK;
 }
 
@@ -79,7 +80,7 @@ void cache_set_bounds(CacheState* cs, const float* lo, const float* hi, bool pad
 
 // One optimiser step on n device-resident samples.
 fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, const float4* target, uint32_t n,
-                      bool apply, float* dmlp_host, float* dgrid_host, float* loss_host)
+                      uint32_t sample_stride, bool apply, float* dmlp_host, float* dgrid_host, float* loss_host)
 {
     if (n == 0) return FND_OK;
     if (n > cs->tb.cap) return set_error(ctx, FND_E_INVALID_ARG, "cache batch of %u exceeds %u", n, cs->tb.cap);
@@ -91,9 +92,9 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
     cs->grid_grad_dirty = !apply;
     launch_zero(cs->tb.loss, 1, s);
     const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
-    launch_cache_forward(cs->view, q, n, nullptr, cs->tb.pred, cs->tb.act, cs->tb.cap, grid, s);
-    launch_cache_loss(cs->tb.pred, target, n, cs->tb.dz, cs->tb.cap, cs->tb.loss, s);
-    launch_cache_backward(cs->view, q, n, cs->tb.act, cs->tb.dz, cs->tb.cap, cs->tb.grad_grid, s);
+    launch_cache_forward(cs->view, q, sample_stride, n, nullptr, cs->tb.pred, cs->tb.act, cs->tb.cap, grid, s);
+    launch_cache_loss(cs->tb.pred, target, sample_stride, n, cs->tb.dz, cs->tb.cap, cs->tb.loss, s);
+    launch_cache_backward(cs->view, q, sample_stride, n, cs->tb.act, cs->tb.dz, cs->tb.cap, cs->tb.grad_grid, s);
     launch_cache_dw(cs->tb.act, cs->tb.dz, n, cs->tb.cap, cs->tb.grad_mlp, s);
     fnd_status st = check_hip(ctx, hipGetLastError(), "cache step");
     if (st != FND_OK) return st;
@@ -178,6 +179,7 @@ fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* d)
     d->train_paths = 1u << 16;
     d->train_steps = 4;
     d->train_depth = 5;
+    d->train_batch = 1u << 14;
     d->seed = 1;
     return FND_OK;
 }
@@ -329,7 +331,7 @@ fnd_status fnd_pt_cache_infer(fnd_pt_ctx* ctx, const float* queries, uint32_t n,
         st = upload_queries(ctx, cs, queries + 6ull * b0, nullptr, m);
         if (st != FND_OK) return st;
         const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (m + 127u) / 128u));
-        launch_cache_forward(cs->view, cs->io_query, m, nullptr, cs->tb.pred, nullptr, 0, grid, ctx->stream);
+        launch_cache_forward(cs->view, cs->io_query, 1, m, nullptr, cs->tb.pred, nullptr, 0, grid, ctx->stream);
         std::vector<float4> hp(m);
         st = check_hip(ctx, hipMemcpyAsync(hp.data(), cs->tb.pred, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream),
                        "read predictions");
@@ -354,7 +356,7 @@ fnd_status fnd_pt_cache_gradients(fnd_pt_ctx* ctx, const float* queries, const f
     st = upload_queries(ctx, cs, queries, targets, n);
     if (st != FND_OK) return st;
     float l = 0;
-    st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, false, dmlp, dgrid, &l);
+    st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, false, dmlp, dgrid, &l);
     if (loss) *loss = l;
     return st;
 }
@@ -370,7 +372,7 @@ fnd_status fnd_pt_cache_train(fnd_pt_ctx* ctx, const float* queries, const float
     if (st != FND_OK) return st;
     float l = 0;
     for (uint32_t k = 0; k < steps && st == FND_OK; ++k)
-        st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, true, nullptr, nullptr, k + 1 == steps ? &l : nullptr);
+        st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, true, nullptr, nullptr, k + 1 == steps ? &l : nullptr);
     if (loss) *loss = l;
     return st;
 }

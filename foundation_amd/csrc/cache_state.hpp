@@ -46,7 +46,8 @@ struct CacheState {
 CacheState* cache_get(fnd_pt_ctx* ctx);
 void cache_free(fnd_pt_ctx* ctx);
 void cache_set_bounds(CacheState* cs, const float* lo, const float* hi, bool pad);
+// One optimiser step on n samples q[s * sample_stride], target[s * sample_stride].
 fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, const float4* target, uint32_t n,
-                      bool apply, float* dmlp_host, float* dgrid_host, float* loss_host);
+                      uint32_t sample_stride, bool apply, float* dmlp_host, float* dgrid_host, float* loss_host);
 
 }  // namespace fnd

@@ -477,7 +477,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             }
             if (apply_cache) {
                 const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
-                launch_cache_forward(cs->view, rs->b.q_query, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
+                launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
                                      cg, ctx->stream);
                 launch_cache_apply(wa, grid, ctx->stream);
             }
@@ -510,13 +510,14 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         if (st != FND_OK) return st;
         nsamp = std::min(nsamp, cs->sample_cap);
         if (nsamp) {
-            const uint32_t steps = cs->desc.train_steps;
-            const uint32_t per = std::min(kCacheBatch, (nsamp + steps - 1) / steps);
+            // step k takes samples k, k + S, k + 2S, ... (per of them): every
+            // step spans the whole frame, and the steps are disjoint
+            const uint32_t steps = std::min(cs->desc.train_steps, nsamp);
+            const uint32_t per = std::max(1u, std::min(cs->desc.train_batch, nsamp / steps));
+            const uint32_t S = nsamp / per;
             for (uint32_t k = 0; k < steps; ++k) {
-                const uint32_t off = uint32_t((uint64_t(k) * per) % nsamp);
-                const uint32_t m = std::min(per, nsamp - off);
                 float* lp = (k + 1 == steps) ? &cs->train_loss_host : nullptr;
-                st = cache_step(ctx, cs, cs->s_query + off, cs->s_target + off, m, true, nullptr, nullptr, lp);
+                st = cache_step(ctx, cs, cs->s_query + k, cs->s_target + k, per, S, true, nullptr, nullptr, lp);
                 if (st != FND_OK) return st;
             }
             cs->last_samples = nsamp;

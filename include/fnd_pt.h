@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FND_PT_ABI_VERSION 1u
+#define FND_PT_ABI_VERSION 2u
 
 typedef uint64_t fnd_handle;
 #define FND_INVALID_HANDLE (~(uint64_t)0)
@@ -311,9 +311,12 @@ typedef struct fnd_cache_desc {
     uint32_t hidden;            /* 64 */
     uint32_t hidden_layers;     /* 5 */
     float learning_rate;        /* 1e-2 */
-    uint32_t train_paths;       /* training paths traced per frame (FND_JOB_TRAIN_CACHE), <= 2^16 */
-    uint32_t train_steps;       /* optimiser steps per frame */
-    uint32_t train_depth;       /* bounces of a training path */
+    uint32_t train_paths;       /* training paths per frame (FND_JOB_TRAIN_CACHE), <= 2^16: a subset of the
+                                   frame's own paths (at most one in four) that goes on to full depth */
+    uint32_t train_steps;       /* optimiser steps per frame, 1..64 */
+    uint32_t train_depth;       /* diffuse vertices recorded per training path, 1..16 */
+    uint32_t train_batch;       /* samples per optimiser step, 256..65536; each step takes an evenly
+                                   strided subset of the frame's samples */
     uint64_t seed;              /* initialisation */
     float bounds_lo[3];         /* region the grid covers; lo == hi takes the bounds of the */
     float bounds_hi[3];         /* first scene rendered with the cache */
