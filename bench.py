@@ -22,6 +22,10 @@ displaced-sphere scene (1.31M-triangle icosphere + 200k soup, 4 quad lights,
 GPU reference, plain and with the radiance cache trained every frame; device
 seconds of the frames are summed.  extra.path_tracing_10M is samples/s of
 full path tracing on the 10M-triangle scene.  --no-render skips both legs.
+extra.batch_64_scenes: 64 scenes of the batch generator (triangles
+log-uniform in [10k, 2M], images 256/512/1024 square) rendered at 16 spp,
+depth 6, by one fnd_pt_render call with launch knobs from the autotuner
+(--no-batch skips it).
 Progress goes to stderr.
 """
 import argparse
@@ -85,6 +89,54 @@ def render_throughput(ctx, scene, camera, width, height, spp, depth, rr):
     return {"workload": f"{width}x{height}, {spp} spp, depth {depth}, RR from {rr}, NEE+MIS on 4 quad lights",
             "msamples_s": round(width * height * spp / ms / 1e3, 2), "ms_per_spp": round(ms / spp, 3),
             "render_mrays_s": round(st.render_rays / ms / 1e3, 1), "wall_ms": round(wall * 1e3, 1)}
+
+
+def batch64(ctx, count, spp, depth, cache_path):
+    """The multi-scene workload: `count` scenes of the batch generator
+    (displaced sphere + soup, triangles log-uniform in [10k, 2M], square
+    images of side 256/512/1024), all rendered by one fnd_pt_render call at
+    `spp` spp and depth `depth`, with launch knobs from the autotuner."""
+    from foundation_amd import pt
+    t = time.perf_counter()
+    gens, scenes, fbs, jobs = [], [], [], []
+    tris = pixels = 0
+    try:
+        for i in range(count):
+            g, side = pt.batch_scene(i)
+            gens.append(g)
+            scenes.append(ctx.scene_create(g.desc))
+            fbs.append(ctx.framebuffer_create(side, side))
+            jobs.append(pt.make_job(scenes[-1], fbs[-1], g.camera, spp, depth, seed=i))
+            tris += g.num_prims()
+            pixels += side * side
+            g.close()
+        build_s = time.perf_counter() - t
+        log(f"batch: {count} scenes, {tris} tris, {pixels} pixels built in {build_s:.1f} s")
+        ctx.set_tuning(block_size=0, stack_lds=0, waves_per_cu=0, refill=0, queue_capacity=0, tile=0)
+        t = time.perf_counter()
+        res = ctx.autotune(jobs, cache_path=cache_path, force=cache_path is None)
+        tune_s = time.perf_counter() - t
+        tun = res.tuning
+        log(f"batch autotune: {res.candidates} configs, {res.renders} renders in {tune_s:.1f} s; best {res.best_ms:.1f} ms"
+            f" vs default {res.default_ms:.1f} ms (1 spp)")
+        ctx.reset_stats()
+        ctx.render(jobs)
+        st = ctx.stats()
+        ms = st.last_call_ms
+        return {"scenes": count, "triangles": int(tris), "pixels": int(pixels), "spp": spp, "max_depth": depth,
+                "paths": int(pixels * spp), "build_s": round(build_s, 1), "seconds": round(ms / 1e3, 3),
+                "msamples_s": round(pixels * spp / ms / 1e3, 1), "render_mrays_s": round(st.render_rays / ms / 1e3, 1),
+                "autotune": {"key": res.key.decode(), "configs": res.candidates, "renders": res.renders,
+                             "seconds": round(tune_s, 1), "from_cache": bool(res.from_cache),
+                             "best_ms_1spp": round(res.best_ms, 2), "default_ms_1spp": round(res.default_ms, 2),
+                             "median_ms_1spp": round(res.median_ms, 2),
+                             "tuning": {k: getattr(tun, k) for k, _ in tun._fields_}}}
+    finally:
+        for f in fbs:
+            ctx.framebuffer_destroy(f)
+        for s_ in scenes:
+            ctx.scene_destroy(s_)
+        ctx.set_tuning(block_size=0, stack_lds=0, waves_per_cu=0, refill=0, queue_capacity=0, tile=0)
 
 
 def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
@@ -155,6 +207,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the path-tracing legs (samples/s and time to relMSE 0.01, plain vs cache)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched launch leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--autotune-cache", default=None,
+                    help="autotuner cache file for the batch leg (default: search every run)")
     ap.add_argument("--relmse-width", type=int, default=1920)
     ap.add_argument("--relmse-height", type=int, default=1080)
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
@@ -262,13 +318,19 @@ def main():
                   "shadow_rays": int(shadow.shape[0]), "anyhit_mismatches": int((socc != oocc).sum())}
         log(f"cpu: {cpu['value']} Mrays/s on {threads} threads; parity {parity}")
 
-    render = relmse = None
+    render = relmse = batch = None
     if not args.no_render and not args.only_incoherent:
         render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 4, 8, 3)
         log(f"path tracing @10M tris: {render}")
         ctx.scene_destroy(scene)  # free the 10M-triangle scene before the next one
         scene = None
         relmse = time_to_rel_mse(ctx, args.relmse_width, args.relmse_height, 8, 3, 0.01, args.relmse_ref_spp, 1500)
+    if not args.no_batch and not args.only_incoherent:
+        if scene is not None:
+            ctx.scene_destroy(scene)
+            scene = None
+        batch = batch64(ctx, args.batch_scenes, 16, 6, args.autotune_cache)
+        log(f"batch: {batch}")
 
     line = {
         "metric": "Mrays/s incoherent @10M tris; time-to-relMSE 0.01 (cache vs plain)",
@@ -298,7 +360,7 @@ def main():
                   "scene_nodes": int(info.num_nodes), "scene_build_ms": round(info.build_ms, 1),
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
                   "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
-                  "path_tracing_10M": render, "time_to_rel_mse_0_01": relmse},
+                  "path_tracing_10M": render, "time_to_rel_mse_0_01": relmse, "batch_64_scenes": batch},
     }
     print(json.dumps(line), flush=True)
     ctx.close()

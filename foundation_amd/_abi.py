@@ -184,6 +184,24 @@ class GenParams(C.Structure):
     ]
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [
+        ("tuning", Tuning),
+        ("median", Tuning),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
+        ("median_ms", C.c_double),
+        ("candidates", C.c_uint32),
+        ("renders", C.c_uint32),
+        ("from_cache", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("key", C.c_char * 128),
+    ]
+
+
+AUTOTUNE_FORCE = 1
+
+
 class CacheDesc(C.Structure):
     _fields_ = [
         ("enable", C.c_uint32),
@@ -274,6 +292,9 @@ PROTOTYPES = {
     "fnd_gen_scene": (_S, [C.POINTER(GenParams), C.POINTER(_P)]),
     "fnd_gen_scene_get": (_S, [_P, C.POINTER(SceneDesc), C.POINTER(Camera)]),
     "fnd_gen_scene_free": (None, [_P]),
+    "fnd_gen_batch_params": (_S, [C.c_uint32, C.POINTER(GenParams), C.POINTER(C.c_uint32)]),
+    "fnd_pt_autotune": (_S, [_P, C.POINTER(RenderJob), C.c_uint32, C.c_char_p, C.c_uint32,
+                             C.POINTER(AutotuneResult)]),
 }
 
 _lib = None

@@ -483,6 +483,23 @@ fnd_status fnd_gen_default_params(uint32_t kind, fnd_gen_params* out)
     }
 }
 
+fnd_status fnd_gen_batch_params(uint32_t index, fnd_gen_params* out, uint32_t* image_side)
+{
+    if (!out || !image_side) return FND_E_INVALID_ARG;
+    fnd_status st = fnd_gen_default_params(FND_GEN_DISPLACED_SPHERE, out);
+    if (st != FND_OK) return st;
+    Pcg rng(0xBA7C4ull, index);
+    const double lo = std::log(1.0e4), hi = std::log(2.0e6);
+    const double n = std::exp(lo + (hi - lo) * rng.uniform());
+    const uint32_t total = uint32_t(std::lround(n));
+    out->seed = index;
+    out->soup_triangles = total / 8u;
+    out->subdivision = std::max(1u, uint32_t(std::lround(std::sqrt(double(total - out->soup_triangles) / 20.0))));
+    static const uint32_t sides[3] = {256, 512, 1024};
+    *image_side = sides[rng.next() % 3u];
+    return FND_OK;
+}
+
 fnd_status fnd_gen_scene(const fnd_gen_params* p, fnd_generated_scene** out)
 {
     if (!p || !out) return FND_E_INVALID_ARG;

@@ -306,6 +306,28 @@ class Context:
         arr = (A.RenderJob * len(jobs))(*jobs)
         self._ck(self.lib.fnd_pt_render(self.h, arr, len(jobs)))
 
+    def autotune(self, jobs, cache_path=None, force=False):
+        """Searches launch configurations for `jobs` (fnd_pt_autotune) and
+        applies the winner; returns the fnd_autotune_result."""
+        arr = (A.RenderJob * len(jobs))(*jobs)
+        res = A.AutotuneResult()
+        path = cache_path.encode() if cache_path else None
+        self._ck(self.lib.fnd_pt_autotune(self.h, arr, len(jobs), path, A.AUTOTUNE_FORCE if force else 0,
+                                          C.byref(res)))
+        return res
+
+
+def batch_scene(index):
+    """Scene `index` of the multi-scene batch workload (fnd_gen_batch_params):
+    returns (GeneratedScene, image side)."""
+    p = A.GenParams()
+    side = C.c_uint32()
+    st = A.load().fnd_gen_batch_params(index, C.byref(p), C.byref(side))
+    if st != A.OK:
+        raise FndError(st, "fnd_gen_batch_params")
+    kw = {name: getattr(p, name) for name, _ in A.GenParams._fields_ if name != "kind"}
+    return GeneratedScene(p.kind, **kw), side.value
+
 
 def make_job(scene, framebuffer, camera, spp, max_depth, rr_depth=None, seed=1, sample_offset=0, flags=0):
     j = A.RenderJob()

@@ -289,6 +289,36 @@ fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle fb, double* r
 /* Renders all jobs in one batched wavefront launch sequence. */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
 
+/* ---------------------------------------------------------------- autotuner */
+
+/* Launch-configuration search for a render workload (no counterpart in the
+ * reference).  Times renders of `jobs` with spp = 1 and cache training off,
+ * into scratch framebuffers (the callers' images are not touched), over
+ * fnd_tuning candidates: the library default plus a fixed pseudo-random
+ * sample of {block, LDS stack} x waves/CU x refill x queue capacity x tile,
+ * narrowed by successive halving (a third survives each round, timed with
+ * twice the renders).  The winner is applied to the context.  With a
+ * cache_path, results are kept as text lines keyed by (GPU arch, CUs,
+ * log2 triangles, log2 pixels, max depth, log2 jobs); a key found there is
+ * applied without a search unless FND_AUTOTUNE_FORCE is set. */
+typedef struct fnd_autotune_result {
+    fnd_tuning tuning;          /* the configuration now applied */
+    fnd_tuning median;          /* the first round's median configuration (for held-out checks) */
+    double best_ms;             /* its mean device ms per timing render, last round */
+    double default_ms;          /* the library default, first round */
+    double median_ms;           /* median over the first round */
+    uint32_t candidates;        /* configurations timed */
+    uint32_t renders;           /* timing renders run */
+    uint32_t from_cache;        /* 1: taken from the cache file */
+    uint32_t reserved;
+    char key[128];              /* cache key of the workload */
+} fnd_autotune_result;
+
+#define FND_AUTOTUNE_FORCE 1u
+
+fnd_status fnd_pt_autotune(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs, const char* cache_path,
+                           uint32_t flags, fnd_autotune_result* out);
+
 /* ---------------------------------------------------------------- radiance cache */
 
 /* Online-trained radiance cache (no counterpart in the reference).  A
@@ -392,6 +422,11 @@ typedef struct fnd_gen_params {
 typedef struct fnd_generated_scene fnd_generated_scene;
 
 fnd_status fnd_gen_default_params(uint32_t kind, fnd_gen_params* out);
+/* Scene `index` of the multi-scene batch workload: a displaced sphere plus
+ * soup whose triangle count is log-uniform in [10k, 2M] (one in eight soup
+ * triangles), seed = index, and a square image side drawn from {256, 512,
+ * 1024}; both drawn from PCG32 stream `index`. */
+fnd_status fnd_gen_batch_params(uint32_t index, fnd_gen_params* out, uint32_t* image_side);
 fnd_status fnd_gen_scene(const fnd_gen_params* params, fnd_generated_scene** out);
 fnd_status fnd_gen_scene_get(const fnd_generated_scene* scene, fnd_scene_desc* out_desc, fnd_camera* out_camera);
 void fnd_gen_scene_free(fnd_generated_scene* scene);
