@@ -73,10 +73,12 @@ __device__ __forceinline__ void xform_ray(const float* m, float ox, float oy, fl
 }
 
 // Woop et al. 2013, FP32 only: exact zeros of U, V, W count as inside.
-// Returns true with (t, u, v) when the ray's line crosses the triangle; the
-// caller range-checks t.
+// Returns true with t = T / det and the barycentric numerators V, W and
+// det when the ray's line crosses the triangle; the caller range-checks t and
+// divides u = V / det, v = W / det only for the hit it keeps (the same
+// operations, so the same bits, as dividing here).
 __device__ __forceinline__ bool isect_tri(const RayK& r, const float4 a, const float4 b, const float4 c, float& t,
-                                          float& u, float& v)
+                                          float& Vn, float& Wn, float& detn)
 {
     const uint32_t kx = r.perm & 3u, ky = (r.perm >> 2) & 3u, kz = (r.perm >> 4) & 3u;
     const float Akx = sel3(kx, a.x, a.y, a.z) - r.okx;
@@ -109,8 +111,9 @@ __device__ __forceinline__ bool isect_tri(const RayK& r, const float4 a, const f
     const float Cz = r.Sz * Ckz;
     const float T = fmaf(U, Az, fmaf(V, Bz, W * Cz));
     t = T / det;
-    u = V / det;
-    v = W / det;
+    Vn = V;
+    Wn = W;
+    detn = det;
     return true;
 }
 

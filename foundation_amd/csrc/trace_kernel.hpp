@@ -83,9 +83,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
     SceneDev sc = args.scene;
     bool active = false, retired = false;
     uint32_t ray_idx = 0;
-    RayK r;
-    float wox = 0.f, woy = 0.f, woz = 0.f, wdx = 0.f, wdy = 0.f, wdz = 0.f;
-    float tmin = 0.f, tmax = 0.f, tbest = 0.f, bu = 0.f, bv = 0.f;
+    RayK r;   // the ray in the current space
+    RayK rw;  // the world-space ray: leaving an instance restores it without divisions
+    float tmin = 0.f, tmax = 0.f, tbest = 0.f, bV = 0.f, bW = 0.f, bdet = 1.f;
     uint32_t best = 0xFFFFFFFFu, best_aux = 0u;
     uint32_t cur = 0, sp = 0, id_base = 0, inst_slot = 0xFFFFFFFFu;
     uint32_t c_nodes = 0, c_tris = 0, c_inst = 0;
@@ -129,13 +129,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                     const float4 ra = rp[0];
                     const float4 rb = rp[1];
                     if (MULTI) sc = args.scenes[(second ? args.ray_scene2 : args.ray_scene)[ray_idx]];
-                    wox = ra.x; woy = ra.y; woz = ra.z; tmin = ra.w;
-                    wdx = rb.x; wdy = rb.y; wdz = rb.z; tmax = rb.w;
-                    ray_setup(r, wox, woy, woz, wdx, wdy, wdz);
+                    tmin = ra.w;
+                    tmax = rb.w;
+                    ray_setup(rw, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z);
+                    r = rw;
                     tbest = tmax;
                     best = 0xFFFFFFFFu;
                     best_aux = 0xFFFFFFFFu;
-                    bu = bv = 0.f;
+                    bV = bW = 0.f;
+                    bdet = 1.f;
                     sp = 0;
                     cur = sc.root;
                     id_base = 0;
@@ -230,14 +232,14 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const float4 a = tp[0];
             const float4 b = tp[1];
             const float4 c = tp[2];
-            float t, u, v;
-            if (isect_tri(r, a, b, c, t, u, v) && t > tmin && t < tmax) {
+            float t, Vn, Wn, detn;
+            if (isect_tri(r, a, b, c, t, Vn, Wn, detn) && t > tmin && t < tmax) {
                 const uint32_t id = INTERNAL ? ti : id_base + __float_as_uint(a.w);
                 if (MIXED ? lane_any : ANY_CONST) {
-                    tbest = t; best = id; bu = u; bv = v; best_aux = inst_slot;
+                    tbest = t; best = id; bV = Vn; bW = Wn; bdet = detn; best_aux = inst_slot;
                     finish = true;
                 } else if (t < tbest || (t == tbest && id < best)) {
-                    tbest = t; best = id; bu = u; bv = v; best_aux = inst_slot;
+                    tbest = t; best = id; bV = Vn; bW = Wn; bdet = detn; best_aux = inst_slot;
                 }
             }
             if (!finish) {
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                     pop = true;
             }
         } else if (TWO && cur == kEntryReturn) {
-            ray_setup(r, wox, woy, woz, wdx, wdy, wdz);
+            r = rw;
             id_base = 0;
             inst_slot = 0xFFFFFFFFu;
             pop = true;
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const uint4 m3 = reinterpret_cast<const uint4*>(ip)[3];
             const float mm[12] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
             float rox, roy, roz, rdx, rdy, rdz;
-            xform_ray(mm, wox, woy, woz, wdx, wdy, wdz, rox, roy, roz, rdx, rdy, rdz);
+            xform_ray(mm, rw.ox, rw.oy, rw.oz, rw.dx, rw.dy, rw.dz, rox, roy, roz, rdx, rdy, rdz);
             ray_setup(r, rox, roy, roz, rdx, rdy, rdz);
             push(kEntryReturn, -__builtin_inff());
             cur = m3.x;
@@ -288,7 +290,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 out = make_float4(miss ? tmax : tbest, __uint_as_float(best), __uint_as_float(miss ? 0xFFFFFFFFu : best_aux),
                                   0.f);
             } else {
-                out = make_float4(miss ? tmax : tbest, __uint_as_float(best), miss ? 0.f : bu, miss ? 0.f : bv);
+                out = make_float4(miss ? tmax : tbest, __uint_as_float(best), miss ? 0.f : bV / bdet,
+                                  miss ? 0.f : bW / bdet);
             }
             ((MIXED && lane_any) ? args.hits2 : args.hits)[ray_idx] = out;
             active = false;
