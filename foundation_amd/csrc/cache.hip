@@ -104,18 +104,25 @@ __device__ __forceinline__ h8 encode4(const CacheView& v, float x, float y, floa
 }
 
 // Extra inputs 32..47 as a K=32 B operand (slots 16..31 are zero padding):
-// lanes of group 0 hold oriented normal, 1, normalised position, 0.
-__device__ __forceinline__ h8 extras8(float x, float y, float z, float4 n, int g)
+// lanes of group 0 hold oriented normal, 1, normalised position, 0; lanes of
+// group 1 the outgoing direction, roughness, glossy flag (roughness > 0), 0.
+__device__ __forceinline__ h8 extras8(float x, float y, float z, const CacheQuery& q, int g)
 {
     h8 e = {0, 0, 0, 0, 0, 0, 0, 0};
     if (g == 0) {
-        e[0] = _Float16(n.x);
-        e[1] = _Float16(n.y);
-        e[2] = _Float16(n.z);
+        e[0] = _Float16(q.nrm.x);
+        e[1] = _Float16(q.nrm.y);
+        e[2] = _Float16(q.nrm.z);
         e[3] = _Float16(1.0f);
         e[4] = _Float16(x);
         e[5] = _Float16(y);
         e[6] = _Float16(z);
+    } else if (g == 1) {
+        e[0] = _Float16(q.dir.x);
+        e[1] = _Float16(q.dir.y);
+        e[2] = _Float16(q.dir.z);
+        e[3] = _Float16(q.pos.w);
+        e[4] = _Float16(q.pos.w > 0.0f ? 1.0f : 0.0f);
     }
     return e;
 }
@@ -150,13 +157,11 @@ __global__ __launch_bounds__(256) void k_cache_forward(CacheView v, const CacheQ
         for (int t = 0; t < kP; ++t) {
             const uint32_t s = base + 16u * t + col;
             float x = 0, y = 0, z = 0;
-            float4 nr = make_float4(0, 0, 0, 0);
             if (s < n) {
                 const CacheQuery cq = q[size_t(s) * qstride];
                 normalise(v, cq.pos, x, y, z);
-                nr = cq.nrm;
                 bh[t] = encode4(v, x, y, z, g);
-                bx[t] = extras8(x, y, z, nr, g);
+                bx[t] = extras8(x, y, z, cq, g);
             } else {
                 bh[t] = h8{0, 0, 0, 0, 0, 0, 0, 0};
                 bx[t] = h8{0, 0, 0, 0, 0, 0, 0, 0};

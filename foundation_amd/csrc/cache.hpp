@@ -8,7 +8,8 @@
 //
 // Shapes are fixed at compile time so every layer maps onto whole MFMA tiles:
 //   input  48 = 16 levels x 2 features (hash) + 16 extras
-//               (oriented normal, 1, normalised position, zeros)
+//               (oriented normal, 1, normalised position, 0, outgoing
+//               direction, roughness, glossy flag, zeros)
 //   hidden 64 x 5 (ReLU), output 16 rows of which 3 are RGB.
 // Forward uses v_mfma_f32_16x16x32_f16 with FP16 weights and activations,
 // FP32 accumulation; backward uses v_mfma_f32_16x16x32_bf16.  Only these two
@@ -55,11 +56,19 @@ struct CacheView {
     uint32_t dense_mask;        // bit l: level l indexes densely
 };
 
-// A query: world position, oriented normal (both padded to float4).
+// A query: world position (w = GGX roughness, 0 for a Lambert vertex),
+// oriented normal, outgoing direction (0 for a Lambert vertex).
 struct CacheQuery {
     float4 pos;
     float4 nrm;
+    float4 dir;
 };
+
+// A training record: position (w = roughness), oriented normal, outgoing
+// direction, weight (throughput * albedo), radiance mark (w = 1 once set).
+constexpr uint32_t kCacheRecordF4 = 5;
+// From this bounce on every GGX vertex is a cache vertex, whatever its roughness.
+constexpr uint32_t kCacheAnyGlossyBounce = 3;
 
 struct CacheTrainBufs {
     uint16_t* act;              // [kActRows][cap] bf16

@@ -45,7 +45,7 @@ fnd_status validate(fnd_pt_ctx* ctx, const fnd_cache_desc& d)
                          "5 x 64 hidden");
     if (!(d.learning_rate > 0.0f) || d.train_paths == 0 || d.train_paths > kCacheTrainMaxPaths || d.train_steps == 0 ||
         d.train_steps > 64 || d.train_depth == 0 || d.train_depth > kCacheMaxRecords || d.train_batch < 256 ||
-        d.train_batch > kCacheBatch)
+        d.train_batch > kCacheBatch || !(d.glossy_min_roughness >= 0.0f))
         return set_error(ctx, FND_E_INVALID_ARG, "cache: learning_rate > 0, train_paths in 1..65536, "
                                                  "train_steps in 1..64, train_depth in 1..16, train_batch in 256..65536");
     return FND_OK;
@@ -132,9 +132,10 @@ fnd_status upload_queries(fnd_pt_ctx* ctx, CacheState* cs, const float* queries,
     if (n > cs->tb.cap) return set_error(ctx, FND_E_INVALID_ARG, "at most %u queries per call", cs->tb.cap);
     std::vector<CacheQuery> hq(n);
     for (uint32_t i = 0; i < n; ++i) {
-        const float* p = queries + 6ull * i;
-        hq[i].pos = make_float4(p[0], p[1], p[2], 0.0f);
+        const float* p = queries + 10ull * i;
+        hq[i].pos = make_float4(p[0], p[1], p[2], p[9]);
         hq[i].nrm = make_float4(p[3], p[4], p[5], 0.0f);
+        hq[i].dir = make_float4(p[6], p[7], p[8], 0.0f);
     }
     fnd_status st = check_hip(ctx, hipMemcpyAsync(cs->io_query, hq.data(), n * sizeof(CacheQuery), hipMemcpyHostToDevice,
                                                   ctx->stream), "upload queries");
@@ -177,9 +178,10 @@ fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* d)
     d->hidden_layers = kCacheHidLayers;
     d->learning_rate = 1e-2f;
     d->train_paths = 1u << 16;
-    d->train_steps = 4;
-    d->train_depth = 5;
+    d->train_steps = 2;
+    d->train_depth = 2;
     d->train_batch = 1u << 14;
+    d->glossy_min_roughness = 0.5f;
     d->seed = 1;
     return FND_OK;
 }
@@ -224,9 +226,10 @@ fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* d)
     cs->sample_cap = kCacheTrainMaxPaths * d->train_depth;
     if ((st = calloc_dev(ctx, cs, &cs->s_query, cs->sample_cap)) != FND_OK) return fail(st);
     if ((st = calloc_dev(ctx, cs, &cs->s_target, cs->sample_cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->rec, uint64_t(kCacheTrainMaxPaths) * kCacheMaxRecords * 4)) != FND_OK)
+    if ((st = calloc_dev(ctx, cs, &cs->rec, uint64_t(kCacheTrainMaxPaths) * kCacheMaxRecords * kCacheRecordF4)) != FND_OK)
         return fail(st);
     if ((st = calloc_dev(ctx, cs, &cs->rec_count, kCacheTrainMaxPaths)) != FND_OK) return fail(st);
+    if ((st = calloc_dev(ctx, cs, &cs->t_list, kCacheTrainMaxPaths)) != FND_OK) return fail(st);
     // level resolutions: N_l = floor(N_min * b^l), b = (N_max / N_min)^(1 / (L - 1))
     const double b = std::exp((std::log(2048.0) - std::log(16.0)) / double(kCacheLevels - 1));
     cs->view.dense_mask = 0;
@@ -328,7 +331,7 @@ fnd_status fnd_pt_cache_infer(fnd_pt_ctx* ctx, const float* queries, uint32_t n,
     if (n && (!queries || !out)) return set_error(ctx, FND_E_INVALID_ARG, "null array");
     for (uint32_t b0 = 0; b0 < n; b0 += cs->tb.cap) {
         const uint32_t m = std::min(cs->tb.cap, n - b0);
-        st = upload_queries(ctx, cs, queries + 6ull * b0, nullptr, m);
+        st = upload_queries(ctx, cs, queries + 10ull * b0, nullptr, m);
         if (st != FND_OK) return st;
         const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (m + 127u) / 128u));
         launch_cache_forward(cs->view, cs->io_query, 1, m, nullptr, cs->tb.pred, nullptr, 0, grid, ctx->stream);

@@ -32,8 +32,9 @@ struct CacheState {
     CacheQuery* s_query = nullptr;    // training samples gathered from traced paths
     float4* s_target = nullptr;
     uint32_t sample_cap = 0;
-    float4* rec = nullptr;            // per training path: kCacheMaxRecords x (pos, nrm, weight, L mark)
+    float4* rec = nullptr;            // per training path: kCacheMaxRecords records of kCacheRecordF4 float4
     uint32_t* rec_count = nullptr;
+    uint32_t* t_list = nullptr;       // training paths of the current batch (kCacheTrainMaxPaths)
     CacheView view{};
     bool bounds_set = false;
     bool grid_grad_dirty = false;     // grid gradient holds values no update has consumed

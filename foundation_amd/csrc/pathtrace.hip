@@ -80,7 +80,13 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
                 if (job.train_count) {
                     const uint64_t qi = uint64_t(s) * (uint64_t(job.width) * job.height) + pixel;
                     if (qi < job.train_mod) {
-                        const uint64_t ti = (qi * job.train_mul + job.train_add) % job.train_mod;
+                        // (qi * mul + add) mod train_mod, x < 2^64 (qi < train_mod < 2^32,
+                        // mul < 2^32).  Barrett with floor((2^64 - 1) / train_mod): the
+                        // quotient estimate is low by at most 2.
+                        const uint64_t x = qi * job.train_mul + job.train_add;
+                        uint64_t ti = x - __umul64hi(x, job.train_barrett) * job.train_mod;
+                        if (ti >= job.train_mod) ti -= job.train_mod;
+                        if (ti >= job.train_mod) ti -= job.train_mod;
                         if (ti < job.train_count) {
                             train = job.train_base + uint32_t(ti);
                             a.b.rec_count[train] = 0u;
@@ -108,6 +114,9 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             a.b.p_L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.b.p_train[i] = train;
         }
+        const bool trains = train != kNoTrainSlot;
+        const uint32_t tpos = wave_append(trains, &a.b.counters[kCntTrainList], nullptr);
+        if (trains) a.b.t_list[tpos] = i;
         const uint32_t slot = wave_append(live, &a.b.counters[kCntExtA], a.b.rays);
         if (live) {
             a.b.ext_rays[0][slot] = ray;
@@ -167,8 +176,8 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                 // the previous diffuse vertex's target starts after its own light sample
                 rc = a.b.rec_count[tslot];
                 if (rc > 0) {
-                    float4* last = a.b.rec + (size_t(tslot) * a.b.rec_max + rc - 1) * 4;
-                    if (last[3].w == 0.0f) last[3] = make_float4(L.x, L.y, L.z, 1.0f);
+                    float4* last = a.b.rec + (size_t(tslot) * a.b.rec_max + rc - 1) * kCacheRecordF4;
+                    if (last[4].w == 0.0f) last[4] = make_float4(L.x, L.y, L.z, 1.0f);
                 }
             }
             const uint32_t tri = __float_as_uint(hit.y);
@@ -244,21 +253,34 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                             }
                         }
                     }
+                    // cache vertices: Lambert, and GGX at least glossy_min rough (with the
+                    // outgoing direction and roughness as inputs); from bounce
+                    // kCacheAnyGlossyBounce on any GGX vertex, whose throughput is small by then
                     const bool diffuse = mat.type == 0u;
-                    if (record && diffuse && rc < a.b.rec_max) {
-                        float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + rc) * 4;
-                        r[0] = make_float4(p.x, p.y, p.z, 0.0f);
+                    const bool cachable =
+                        diffuse || (mat.type == 1u && (mat.roughness >= a.b.glossy_min || bounce >= kCacheAnyGlossyBounce));
+                    const float q_rough = diffuse ? 0.0f : mat.roughness;
+                    const V3 q_dir = diffuse ? v3(0, 0, 0) : -dir;
+                    // a training path records its first rec_max cache vertices and ends in
+                    // the cache at the next one: earlier targets then include the cache's
+                    // estimate of the remaining tail (self-training, as in NRC)
+                    const bool train_end = record && cachable && rc >= a.b.rec_max;
+                    if (record && cachable && rc < a.b.rec_max) {
+                        float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + rc) * kCacheRecordF4;
+                        r[0] = make_float4(p.x, p.y, p.z, q_rough);
                         r[1] = make_float4(nf.x, nf.y, nf.z, 0.0f);
-                        r[2] = make_float4(thr.x * albedo.x, thr.y * albedo.y, thr.z * albedo.z, 0.0f);
-                        r[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        r[2] = make_float4(q_dir.x, q_dir.y, q_dir.z, 0.0f);
+                        r[3] = make_float4(thr.x * albedo.x, thr.y * albedo.y, thr.z * albedo.z, 0.0f);
+                        r[4] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         a.b.rec_count[tslot] = rc + 1;
                     }
-                    if (diffuse && (job.flags & FND_JOB_USE_CACHE) && !record) {
+                    if (cachable && (job.flags & FND_JOB_USE_CACHE) && (!record || train_end)) {
                         // terminate into the cache: it stands for the indirect light leaving p.
-                        // Training paths go on (their pixel gets the plain estimate).
+                        // Training paths go on past their first cache vertices.
                         query = true;
-                        cq.pos = make_float4(p.x, p.y, p.z, 0.0f);
+                        cq.pos = make_float4(p.x, p.y, p.z, q_rough);
                         cq.nrm = make_float4(nf.x, nf.y, nf.z, 0.0f);
+                        cq.dir = make_float4(q_dir.x, q_dir.y, q_dir.z, 0.0f);
                         q_w = thr * albedo;
                     }
                     // BSDF sampling
@@ -372,12 +394,14 @@ __global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
 // (L_path - L_mark) / weight, the albedo-demodulated radiance the path
 // gathered after that vertex's own light sample (per channel; vertices with a
 // vanishing weight are skipped).
-__global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint32_t n)
+__global__ __launch_bounds__(256) void k_cache_collect(WfArgs a)
 {
+    const uint32_t n = a.b.counters[kCntTrainList];  // the batch's training paths
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t tslot = i < n ? a.b.p_train[i] : kNoTrainSlot;
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t i = j < n ? a.b.t_list[j] : 0u;
+        const uint32_t tslot = j < n ? a.b.p_train[i] : kNoTrainSlot;
         const uint32_t rc = tslot != kNoTrainSlot ? a.b.rec_count[tslot] : 0u;
         const float4 L = rc ? a.b.p_L[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         // one wave-aggregated append per record index (every lane takes part)
@@ -386,12 +410,13 @@ __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint32_t n)
             CacheQuery q{};
             float4 target{};
             if (k < rc) {
-                const float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + k) * 4;
-                const float4 w = r[2];
+                const float4* r = a.b.rec + (size_t(tslot) * a.b.rec_max + k) * kCacheRecordF4;
+                const float4 w = r[3];
                 if (w.x > 1e-4f && w.y > 1e-4f && w.z > 1e-4f) {
-                    const float4 m = r[3].w != 0.0f ? r[3] : L;
+                    const float4 m = r[4].w != 0.0f ? r[4] : L;
                     q.pos = r[0];
                     q.nrm = r[1];
+                    q.dir = r[2];
                     target = make_float4(fmaxf(L.x - m.x, 0.0f) / w.x, fmaxf(L.y - m.y, 0.0f) / w.y,
                                          fmaxf(L.z - m.z, 0.0f) / w.z, 0.0f);
                     take = true;
@@ -418,9 +443,9 @@ void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s)
     hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(256), 0, s, a);
 }
 
-void launch_cache_collect(const WfArgs& a, uint32_t n, uint32_t grid, hipStream_t s)
+void launch_cache_collect(const WfArgs& a, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cache_collect, dim3(grid), dim3(256), 0, s, a, n);
+    hipLaunchKernelGGL(k_cache_collect, dim3(grid), dim3(256), 0, s, a);
 }
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)

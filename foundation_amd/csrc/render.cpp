@@ -162,6 +162,7 @@ void select_training(JobDev& j, uint32_t spp, uint32_t base, uint32_t count, uin
     j.train_mod = P * ts;
     j.train_mul = (j.train_mod % 2654435789ull) != 0 ? 2654435789ull : 2654435803ull;
     j.train_add = mix64(j.seed ^ mix64(uint64_t(j.sample_offset) ^ (frame << 32))) % j.train_mod;
+    j.train_barrett = ~0ull / j.train_mod;
     j.train_base = base;
     // at most one path in kTrainShare trains, so most pixels keep the cached estimate
     j.train_count = uint32_t(std::min<uint64_t>(count, std::max<uint64_t>(1, j.train_mod / kTrainShare)));
@@ -421,7 +422,9 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     if (cs) {
         wa.b.rec = cs->rec;
         wa.b.rec_count = cs->rec_count;
+        wa.b.t_list = cs->t_list;
         wa.b.rec_max = std::min(cs->desc.train_depth, kCacheMaxRecords);
+        wa.b.glossy_min = cs->desc.glossy_min_roughness;
         wa.b.t_query = cs->s_query;
         wa.b.t_target = cs->s_target;
         wa.b.t_cap = cs->sample_cap;
@@ -435,7 +438,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         for (uint64_t g0 = 0; g0 < paths; g0 += batch) {
             const uint32_t n = uint32_t(std::min<uint64_t>(batch, paths - g0));
             const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
-            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery), ctx->stream);
+            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntTrainList), ctx->stream);
             launch_generate(wa, g0, n, grid, ctx->stream);
             // One traversal launch per bounce: the extension rays of bounce b + 1
             // (closest hit) and the shadow rays of bounce b (any hit) share it,
@@ -481,7 +484,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                                      cg, ctx->stream);
                 launch_cache_apply(wa, grid, ctx->stream);
             }
-            if (collect) launch_cache_collect(wa, n, grid, ctx->stream);
+            if (collect) launch_cache_collect(wa, std::max(1u, std::min(grid, kCacheTrainMaxPaths / 256u)), ctx->stream);
             launch_accumulate(wa, g0, n, grid, ctx->stream);
             e = check_hip(ctx, hipGetLastError(), "render launch");
             if (e != FND_OK) return e;

@@ -35,6 +35,7 @@ struct JobDev {
     uint64_t train_mod;       // W * H * spp
     uint64_t train_mul;       // coprime with train_mod
     uint64_t train_add;       // < train_mod, changes every call
+    uint64_t train_barrett;   // floor((2^64 - 1) / train_mod), for the reduction mod train_mod
     uint64_t path_base;       // first global path index of this job in the render call
     uint64_t path_end;
     float eye[3];
@@ -69,6 +70,7 @@ enum WfCounter : uint32_t {
     kCntWorkShadow = 4,
     kCntQuery = 5,
     kCntTrain = 6,
+    kCntTrainList = 7,
     kCntCount = 8
 };
 
@@ -98,10 +100,12 @@ struct WfBuffers {
     uint32_t* q_path;
     float4* q_weight;         // throughput * albedo at the query vertex
     float4* q_pred;
-    // training records: per training slot, rec_max x (pos, normal, weight, L mark (w = 1 once set))
+    // training records: per training slot, rec_max records of kCacheRecordF4 float4
     float4* rec;
     uint32_t* rec_count;
+    uint32_t* t_list;         // batch-local path indices of the batch's training paths
     uint32_t rec_max;
+    float glossy_min;         // GGX at least this rough queries and trains the cache like Lambert
     CacheQuery* t_query;      // gathered training samples
     float4* t_target;
     uint32_t t_cap;
@@ -121,6 +125,6 @@ void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s);
-void launch_cache_collect(const WfArgs& a, uint32_t n, uint32_t grid, hipStream_t s);
+void launch_cache_collect(const WfArgs& a, uint32_t grid, hipStream_t s);
 
 }  // namespace fnd

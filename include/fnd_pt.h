@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FND_PT_ABI_VERSION 2u
+#define FND_PT_ABI_VERSION 3u
 
 typedef uint64_t fnd_handle;
 #define FND_INVALID_HANDLE (~(uint64_t)0)
@@ -324,10 +324,11 @@ fnd_status fnd_pt_autotune(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t
 /* Online-trained radiance cache (no counterpart in the reference).  A
  * multiresolution hash grid (Mueller et al. 2022) feeds a ReLU MLP without
  * biases (Mueller et al. 2021) that predicts the albedo-demodulated indirect
- * radiance leaving a diffuse point.  The kernels are compiled for exactly:
- * 16 levels x 2 features, 2^19 entries per level, resolutions 16..2048,
- * 5 hidden layers of 64; any other shape returns FND_E_UNSUPPORTED.
- * Inputs per query: world position and oriented unit normal.  Loss: relative
+ * radiance leaving a Lambert point, or a GGX point toward the outgoing
+ * direction.  The kernels are compiled for exactly: 16 levels x 2 features,
+ * 2^19 entries per level, resolutions 16..2048, 5 hidden layers of 64; any
+ * other shape returns FND_E_UNSUPPORTED.  Inputs per query: world position,
+ * oriented unit normal, outgoing direction and roughness.  Loss: relative
  * L2, sum over samples and RGB of (p - y)^2 / (p^2 + 0.01) with p held
  * constant in the denominator.  Optimiser: Adam (beta1 0.9, beta2 0.99,
  * eps 1e-15) on FP32 master weights; the kernels read FP16 copies. */
@@ -344,9 +345,12 @@ typedef struct fnd_cache_desc {
     uint32_t train_paths;       /* training paths per frame (FND_JOB_TRAIN_CACHE), <= 2^16: a subset of the
                                    frame's own paths (at most one in four) that goes on to full depth */
     uint32_t train_steps;       /* optimiser steps per frame, 1..64 */
-    uint32_t train_depth;       /* diffuse vertices recorded per training path, 1..16 */
+    uint32_t train_depth;       /* cache vertices a training path records, 1..16; it then ends in the
+                                   cache at its next cache vertex (self-training) */
     uint32_t train_batch;       /* samples per optimiser step, 256..65536; each step takes an evenly
                                    strided subset of the frame's samples */
+    float glossy_min_roughness; /* GGX vertices at least this rough also terminate into (and train) the
+                                   cache, with their outgoing direction as input; > 1 disables */
     uint64_t seed;              /* initialisation */
     float bounds_lo[3];         /* region the grid covers; lo == hi takes the bounds of the */
     float bounds_hi[3];         /* first scene rendered with the cache */
@@ -359,7 +363,8 @@ fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* desc);
 fnd_status fnd_pt_cache_param_count(fnd_pt_ctx* ctx, uint64_t* mlp, uint64_t* grid);
 fnd_status fnd_pt_cache_set_params(fnd_pt_ctx* ctx, const float* mlp, const float* grid);
 fnd_status fnd_pt_cache_get_params(fnd_pt_ctx* ctx, float* mlp, float* grid);
-/* queries: n * 6 floats (position, normal); out: n * 3 */
+/* queries: n * 10 floats (position, oriented normal, outgoing direction,
+ * GGX roughness; direction and roughness 0 for a Lambert point); out: n * 3 */
 fnd_status fnd_pt_cache_infer(fnd_pt_ctx* ctx, const float* queries, uint32_t n, float* out);
 /* Loss and gradients of the loss for one batch, without an optimiser step.
  * dmlp: fnd_pt_cache_param_count(mlp) floats, dgrid: (grid) floats or NULL. */

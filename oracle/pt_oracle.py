@@ -257,7 +257,10 @@ def cache_loss_grad(mlp, grid, queries, targets, lo, inv_extent, round_fp16=True
     """FP64 loss, prediction and gradients of the radiance cache."""
     mlp = np.ascontiguousarray(mlp, np.float32)
     grid = np.ascontiguousarray(grid, np.float32)
-    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+    q = np.asarray(queries, np.float32)
+    if q.shape[-1] == 6:  # diffuse queries: direction 0, roughness 0
+        q = np.concatenate([q.reshape(-1, 6), np.zeros((q.size // 6, 4), np.float32)], 1)
+    q = np.ascontiguousarray(q, np.float32).reshape(-1, 10)
     t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
     lo = np.ascontiguousarray(lo, np.float32)
     n = q.shape[0]

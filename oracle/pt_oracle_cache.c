@@ -12,8 +12,11 @@
  *             (x * 1 xor y * 2654435761 xor z * 805459861) mod T.
  *   network   ReLU MLP without biases, Mueller, Rousselle, Novak, Keller,
  *             "Real-time Neural Radiance Caching for Path Tracing", ACM TOG
- *             40(4), 2021: input 48 (32 grid features, oriented normal, 1,
- *             normalised position, zeros), 5 x 64 hidden, 16 outputs (RGB).
+ *             40(4), 2021: input 48 (32 grid features; oriented normal, 1,
+ *             normalised position, 0; outgoing direction, roughness, glossy
+ *             flag = roughness > 0, zeros), 5 x 64 hidden, 16 outputs (RGB).
+ *             The paper also feeds the direction and roughness to the
+ *             network; a diffuse query passes direction 0 and roughness 0.
  *   loss      relative L2 of the same paper: (p - y)^2 / (sg(p)^2 + 0.01),
  *             summed over samples and RGB (sg: held constant when
  *             differentiating); plain_l2 = 1 drops the denominator, which
@@ -86,7 +89,8 @@ static void corners(double x, double y, double z, int res, int dense, uint32_t* 
 
 /* Returns the loss summed over samples; pred (n*3), dmlp (20480) and dgrid
  * (16 * 2^19 * 2, or NULL) receive the prediction and the gradients.
- * queries: n * 6 (position, normal); lo3, inv_extent: the normalisation. */
+ * queries: n * 10 (position, normal, outgoing direction, roughness);
+ * lo3, inv_extent: the normalisation. */
 double orc_cache_loss_grad(const float* mlp, const float* grid, const float* queries, const float* targets, uint32_t n,
                            const float* lo3, float inv_extent, int round_fp16, int plain_l2, float* pred, double* dmlp,
                            double* dgrid)
@@ -101,7 +105,7 @@ double orc_cache_loss_grad(const float* mlp, const float* grid, const float* que
     double act[7][CH];   /* act[0] = input (48), act[1..5] = hidden outputs, act[6] = output */
     double z[6][CH];
     for (uint32_t s = 0; s < n; ++s) {
-        const float* q = queries + 6 * (size_t)s;
+        const float* q = queries + 10 * (size_t)s;
         double x[3];
         for (int a = 0; a < 3; ++a) {  /* normalised in float, as the device does */
             float vf = (q[a] - lo3[a]) * inv_extent;
@@ -125,6 +129,7 @@ double orc_cache_loss_grad(const float* mlp, const float* grid, const float* que
         double ex[16] = {0};
         ex[0] = q[3]; ex[1] = q[4]; ex[2] = q[5]; ex[3] = 1.0;
         ex[4] = x[0]; ex[5] = x[1]; ex[6] = x[2];
+        ex[8] = q[6]; ex[9] = q[7]; ex[10] = q[8]; ex[11] = q[9]; ex[12] = q[9] > 0.0f ? 1.0 : 0.0;
         for (int k = 0; k < 16; ++k) act[0][32 + k] = round_fp16 ? to_half(ex[k]) : ex[k];
         for (int L = 0; L < 6; ++L) {
             for (int o = 0; o < W_OUT[L]; ++o) {

@@ -57,7 +57,12 @@ def params(oracle):
     pos = rng.uniform(0.02, 0.98, (n, 3))
     nrm = rng.normal(size=(n, 3))
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
-    q = np.concatenate([pos, nrm], 1).astype(np.float32)
+    # half the queries are glossy: outgoing direction and roughness in [0.5, 1]
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rough = rng.uniform(0.5, 1.0, (n, 1))
+    glossy = (np.arange(n) % 2 == 1)[:, None]
+    q = np.concatenate([pos, nrm, np.where(glossy, d, 0.0), np.where(glossy, rough, 0.0)], 1).astype(np.float32)
     t = rng.uniform(0, 2, (n, 3)).astype(np.float32)
     return mlp, grid, q, t
 
@@ -115,7 +120,7 @@ def test_training_fits_a_smooth_field(ctx):
     n = 1 << 16
     pos = rng.uniform(0, 1, (n, 3))
     nrm = np.tile([0.0, 0.0, 1.0], (n, 1))
-    q = np.concatenate([pos, nrm], 1).astype(np.float32)
+    q = np.concatenate([pos, nrm, np.zeros((n, 4))], 1).astype(np.float32)
     # targets in [0.5, 2.5] so the relative error below is well defined
     t = np.stack([1.5 + np.sin(6 * pos[:, 0]), 1.5 + np.cos(5 * pos[:, 1]), 1 + pos[:, 2] ** 2], 1).astype(np.float32)
     loss = C.c_float()

@@ -50,6 +50,43 @@ def test_backward_matches_finite_differences(oracle, problem):
         assert abs(fd - dg[idx]) <= 1e-4 * max(1.0, abs(fd)), (idx, fd, dg[idx])
 
 
+def test_glossy_inputs_reach_the_network(oracle):
+    """Outgoing direction, roughness and the glossy flag feed input rows
+    40..44: finite differences agree there, and Lambert queries (direction 0,
+    roughness 0) leave those rows without gradient."""
+    rng = np.random.default_rng(5)
+    mlp = rng.uniform(-0.3, 0.3, oracle.CACHE_MLP_PARAMS).astype(np.float32)
+    grid = rng.uniform(-0.1, 0.1, oracle.CACHE_GRID_PARAMS).astype(np.float32)
+    n = 6
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    q = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.normal(size=(n, 3)), d, rng.uniform(0.5, 1, (n, 1))],
+                       1).astype(np.float32)
+    t = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+
+    def loss(m, qq):
+        return oracle.cache_loss_grad(m, grid, qq, t, [0, 0, 0], 1.0, round_fp16=False, plain_l2=True,
+                                      want_grid=False)[0]
+
+    _, _, dm, _ = oracle.cache_loss_grad(mlp, grid, q, t, [0, 0, 0], 1.0, round_fp16=False, plain_l2=True,
+                                         want_grid=False)
+    rows = dm[40 * 64:45 * 64]
+    assert np.all(np.abs(rows) > 0) or np.count_nonzero(rows) > 200
+    h = 1e-3
+    for idx in [40 * 64 + 3, 43 * 64 + 17, 44 * 64 + 60]:
+        a, b = mlp.copy(), mlp.copy()
+        a[idx] += h
+        b[idx] -= h
+        fd = (loss(a, q) - loss(b, q)) / (2 * h)
+        assert abs(fd - dm[idx]) <= 1e-4 * max(1.0, abs(fd)), (idx, fd, dm[idx])
+    lam = q.copy()
+    lam[:, 6:] = 0
+    _, _, dm0, _ = oracle.cache_loss_grad(mlp, grid, lam, t, [0, 0, 0], 1.0, round_fp16=False, plain_l2=True,
+                                          want_grid=False)
+    assert not dm0[40 * 64:48 * 64].any()
+    assert dm0[32 * 64:40 * 64].any()
+
+
 def test_relative_l2_gradient_holds_denominator(oracle, problem):
     """Relative L2 differentiates (p - y)^2 / (p0^2 + 0.01) with p0 frozen."""
     mlp, grid, q, t = problem
