@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ABI_VERSION = 3  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
+ABI_VERSION = 4  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
 LIB_PATH = os.path.join(_HERE, "libfnd_pt.so")
 
 fnd_handle = C.c_uint64
@@ -218,6 +218,7 @@ class CacheDesc(C.Structure):
         ("train_depth", C.c_uint32),
         ("train_batch", C.c_uint32),
         ("glossy_min_roughness", C.c_float),
+        ("glossy_any_bounce", C.c_uint32),
         ("seed", C.c_uint64),
         ("bounds_lo", C.c_float * 3),
         ("bounds_hi", C.c_float * 3),

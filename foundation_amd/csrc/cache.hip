@@ -22,6 +22,7 @@ typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -270,7 +271,7 @@ __global__ __launch_bounds__(256) void k_cache_loss(const float4* pred, const fl
 
 __global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const CacheQuery* q, uint32_t qstride, uint32_t n,
                                                         const uint16_t* act, uint16_t* dz, uint32_t stride,
-                                                        float* grad_grid)
+                                                        uint32_t* grad_grid)
 {
     __shared__ s4 wl[kMlpBwdElems / 4];
     for (int i = threadIdx.x; i < kMlpBwdElems / 4; i += 256) wl[i] = reinterpret_cast<const s4*>(v.wbwd)[i];
@@ -368,11 +369,16 @@ __global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const Cache
                     uint32_t idx[8];
                     float wc[8];
                     level_corners(x, y, z, v.res[l], (v.dense_mask >> l) & 1u, idx, wc);
-                    float* gl = grad_grid + size_t(l) * kCacheT * 2u;
+                    // one packed bf16 atomic per corner carries both features
+                    // (global_atomic_pk_add_bf16): half the atomics and half the
+                    // table bytes of two FP32 adds
+                    auto* gl = (__attribute__((address_space(1))) s2*)(grad_grid + size_t(l) * kCacheT);
 #pragma unroll
                     for (int c = 0; c < 8; ++c) {
-                        atomicAdd(gl + 2u * idx[c], wc[c] * g0);
-                        atomicAdd(gl + 2u * idx[c] + 1u, wc[c] * g1);
+                        s2 val;
+                        val[0] = short(bf16_bits(wc[c] * g0));
+                        val[1] = short(bf16_bits(wc[c] * g1));
+                        (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(gl + idx[c], val);
                     }
                 }
             }
@@ -416,21 +422,28 @@ __global__ __launch_bounds__(256) void k_cache_dw(const uint16_t* act, const uin
 }
 
 // Adam (Kingma & Ba 2015) with bias-correction factors bc1 = 1 - b1^t,
-// bc2 = 1 - b2^t; optionally writes the FP16 copy the forward pass reads.
-// LAZY (the hash grid): a pair of entries whose gradients are both exactly 0
-// is left alone, moments included (Mueller et al. 2022, section 4: updates of
-// zero-gradient table entries are skipped), and every gradient read is
-// cleared, so the next step starts from zero without a separate pass.
-template <bool LAZY>
-__global__ __launch_bounds__(256) void k_adam(float* p, float* gr, float* m, float* v, uint64_t n, float lr,
+// bc2 = 1 - b2^t.
+// GRID (the hash grid): gradients arrive as one packed bf16 pair per table
+// entry; an entry whose two gradients are both 0 is left alone, moments
+// included (Mueller et al. 2022, section 4: updates of zero-gradient table
+// entries are skipped); every gradient read is cleared, so the next step
+// starts from zero without a separate pass; the FP16 copy the forward pass
+// reads is refreshed.
+template <bool GRID>
+__global__ __launch_bounds__(256) void k_adam(float* p, void* gr, float* m, float* v, uint64_t n, float lr,
                                               float b1, float b2, float eps, float bc1, float bc2, uint32_t* out_half2)
 {
     const uint64_t pairs = n / 2;
     for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < pairs; i += uint64_t(gridDim.x) * 256u) {
-        const float2 gg = reinterpret_cast<const float2*>(gr)[i];
-        if (LAZY) {
-            if (gg.x == 0.0f && gg.y == 0.0f) continue;
-            reinterpret_cast<float2*>(gr)[i] = make_float2(0.0f, 0.0f);
+        float2 gg;
+        if (GRID) {
+            uint32_t* gb = static_cast<uint32_t*>(gr);
+            const uint32_t w = gb[i];
+            if ((w & 0x7FFF7FFFu) == 0u) continue;  // both +-0: untouched this step
+            gb[i] = 0u;
+            gg = make_float2(__builtin_bit_cast(float, w << 16), __builtin_bit_cast(float, w & 0xFFFF0000u));
+        } else {
+            gg = static_cast<const float2*>(gr)[i];
         }
         float2 pp = reinterpret_cast<float2*>(p)[i];
         float2 mm = reinterpret_cast<float2*>(m)[i];
@@ -444,7 +457,7 @@ __global__ __launch_bounds__(256) void k_adam(float* p, float* gr, float* m, flo
         reinterpret_cast<float2*>(p)[i] = pp;
         reinterpret_cast<float2*>(m)[i] = mm;
         reinterpret_cast<float2*>(v)[i] = vv;
-        if (out_half2) {
+        if (GRID) {
             const uint32_t lo = __builtin_bit_cast(uint16_t, _Float16(pp.x));
             const uint32_t hi = __builtin_bit_cast(uint16_t, _Float16(pp.y));
             out_half2[i] = lo | (hi << 16);
@@ -517,9 +530,10 @@ void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstrid
 }
 
 void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint16_t* act,
-                           uint16_t* dz, uint32_t stride, float* grad_grid, hipStream_t s)
+                           uint16_t* dz, uint32_t stride, uint32_t* grad_grid, hipStream_t s)
 {
-    const uint32_t grid = std::max(1u, std::min<uint32_t>(2048u, (n + 127u) / 128u));
+    // one 16-sample tile per wave: the scatter's atomics want every CU
+    const uint32_t grid = std::max(1u, std::min<uint32_t>(4096u, (n + 63u) / 64u));
     hipLaunchKernelGGL(k_cache_backward, dim3(grid), dim3(256), 0, s, v, q, qstride, n, act, dz, stride, grad_grid);
 }
 
@@ -531,15 +545,24 @@ void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32
     hipLaunchKernelGGL(k_cache_dw, dim3(chunks, 6), dim3(256), 0, s, act, dz, n, stride, chunk, grad_mlp);
 }
 
-void launch_adam(float* p, float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
-                 float bc1, float bc2, uint32_t* out_half2, bool lazy, hipStream_t s)
+static dim3 adam_grid(uint64_t n)
 {
     const uint64_t blocks = std::min<uint64_t>(8192u, (n / 2 + 255u) / 256u);
-    const dim3 grid(uint32_t(std::max<uint64_t>(1, blocks)));
-    if (lazy)
-        hipLaunchKernelGGL(k_adam<true>, grid, dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2, out_half2);
-    else
-        hipLaunchKernelGGL(k_adam<false>, grid, dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2, out_half2);
+    return dim3(uint32_t(std::max<uint64_t>(1, blocks)));
+}
+
+void launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+                 float bc1, float bc2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam<false>, adam_grid(n), dim3(256), 0, s, p, const_cast<float*>(g), m, v, n, lr, b1, b2,
+                       eps, bc1, bc2, nullptr);
+}
+
+void launch_adam_grid(float* p, uint32_t* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+                      float bc1, float bc2, uint32_t* out_half2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adam<true>, adam_grid(n), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
+                       out_half2);
 }
 
 void launch_arrange(const float* w, _Float16* wfwd, uint16_t* wbwd, hipStream_t s)

@@ -67,15 +67,13 @@ struct CacheQuery {
 // A training record: position (w = roughness), oriented normal, outgoing
 // direction, weight (throughput * albedo), radiance mark (w = 1 once set).
 constexpr uint32_t kCacheRecordF4 = 5;
-// From this bounce on every GGX vertex is a cache vertex, whatever its roughness.
-constexpr uint32_t kCacheAnyGlossyBounce = 3;
 
 struct CacheTrainBufs {
     uint16_t* act;              // [kActRows][cap] bf16
     uint16_t* dz;               // [kDzRows][cap] bf16
     float4* pred;               // [cap]
     float* grad_mlp;            // kMlpParams
-    float* grad_grid;           // kGridParams
+    uint32_t* grad_grid;        // kGridParams / 2: the two features of an entry as packed bf16
     float* loss;                // 1
     uint32_t cap;               // samples the buffers hold (multiple of 32)
 };
@@ -86,12 +84,18 @@ void launch_cache_forward(const CacheView& v, const CacheQuery* q, uint32_t qstr
 void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstride, uint32_t n, uint16_t* dz,
                        uint32_t stride, float* loss, hipStream_t s);
 void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint16_t* act,
-                           uint16_t* dz, uint32_t stride, float* grad_grid, hipStream_t s);
+                           uint16_t* dz, uint32_t stride, uint32_t* grad_grid, hipStream_t s);
 void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32_t stride, float* grad_mlp,
                      hipStream_t s);
 // lazy: skip pairs whose gradients are both 0 and clear the gradients read
-void launch_adam(float* p, float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
-                 float bc1, float bc2, uint32_t* out_half2, bool lazy, hipStream_t s);
+// Adam over n FP32 parameters with FP32 gradients (the MLP).
+void launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+                 float bc1, float bc2, hipStream_t s);
+// Lazy Adam over the hash grid: gradients are packed bf16 pairs; pairs with a
+// zero gradient keep their moments, the others are updated, their gradient
+// cleared and their FP16 copy refreshed into out_half2.
+void launch_adam_grid(float* p, uint32_t* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
+                      float bc1, float bc2, uint32_t* out_half2, hipStream_t s);
 void launch_arrange(const float* w, _Float16* wfwd, uint16_t* wbwd, hipStream_t s);
 void launch_zero(float* p, uint64_t n, hipStream_t s);
 

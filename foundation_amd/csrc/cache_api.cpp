@@ -88,7 +88,7 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
     launch_zero(cs->tb.grad_mlp, kMlpParams, s);
     // the lazy grid update clears what it reads; only a step without update
     // (gradients for the caller) leaves the grid gradient to be cleared here
-    if (cs->grid_grad_dirty) launch_zero(cs->tb.grad_grid, kGridParams, s);
+    if (cs->grid_grad_dirty) launch_zero(reinterpret_cast<float*>(cs->tb.grad_grid), kGridParams / 2, s);
     cs->grid_grad_dirty = !apply;
     launch_zero(cs->tb.loss, 1, s);
     const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
@@ -103,9 +103,9 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
         const float b1 = 0.9f, b2 = 0.99f, eps = 1e-15f;
         const float bc1 = 1.0f - std::pow(b1, float(cs->step)), bc2 = 1.0f - std::pow(b2, float(cs->step));
         launch_adam(cs->w_master, cs->tb.grad_mlp, cs->w_m, cs->w_v, kMlpParams, cs->desc.learning_rate, b1, b2, eps,
-                    bc1, bc2, nullptr, false, s);
-        launch_adam(cs->g_master, cs->tb.grad_grid, cs->g_m, cs->g_v, kGridParams, cs->desc.learning_rate, b1, b2,
-                    eps, bc1, bc2, cs->grid16, true, s);
+                    bc1, bc2, s);
+        launch_adam_grid(cs->g_master, cs->tb.grad_grid, cs->g_m, cs->g_v, kGridParams, cs->desc.learning_rate, b1,
+                         b2, eps, bc1, bc2, cs->grid16, s);
         launch_arrange(cs->w_master, cs->wfwd, cs->wbwd, s);
         st = check_hip(ctx, hipGetLastError(), "cache update");
         if (st != FND_OK) return st;
@@ -115,10 +115,22 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
         if (st == FND_OK && dmlp_host)
             st = check_hip(ctx, hipMemcpyAsync(dmlp_host, cs->tb.grad_mlp, kMlpParams * 4, hipMemcpyDeviceToHost, s),
                            "read gradients");
+        // the grid gradient is bf16: copied into the front half of the caller's
+        // array, then widened in place from the back (element i's source bytes
+        // [2i, 2i+2) lie at or after its destination's start and are read first)
         if (st == FND_OK && dgrid_host)
-            st = check_hip(ctx, hipMemcpyAsync(dgrid_host, cs->tb.grad_grid, kGridParams * 4, hipMemcpyDeviceToHost, s),
+            st = check_hip(ctx, hipMemcpyAsync(dgrid_host, cs->tb.grad_grid, kGridParams * 2, hipMemcpyDeviceToHost, s),
                            "read gradients");
         if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(s), "cache step");
+        if (st == FND_OK && dgrid_host) {
+            auto* bytes = reinterpret_cast<unsigned char*>(dgrid_host);
+            for (uint64_t i = kGridParams; i-- > 0;) {
+                uint16_t h;
+                std::memcpy(&h, bytes + 2 * i, 2);
+                const uint32_t w = uint32_t(h) << 16;
+                std::memcpy(bytes + 4 * i, &w, 4);
+            }
+        }
     }
     return st;
 }
@@ -182,6 +194,7 @@ fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* d)
     d->train_depth = 2;
     d->train_batch = 1u << 14;
     d->glossy_min_roughness = 0.5f;
+    d->glossy_any_bounce = 1;
     d->seed = 1;
     return FND_OK;
 }
@@ -219,7 +232,7 @@ fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* d)
     if ((st = calloc_dev(ctx, cs, &tb.dz, uint64_t(kDzRows) * tb.cap)) != FND_OK) return fail(st);
     if ((st = calloc_dev(ctx, cs, &tb.pred, tb.cap)) != FND_OK) return fail(st);
     if ((st = calloc_dev(ctx, cs, &tb.grad_mlp, kMlpParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &tb.grad_grid, kGridParams)) != FND_OK) return fail(st);
+    if ((st = calloc_dev(ctx, cs, &tb.grad_grid, kGridParams / 2)) != FND_OK) return fail(st);
     if ((st = calloc_dev(ctx, cs, &tb.loss, 1)) != FND_OK) return fail(st);
     if ((st = calloc_dev(ctx, cs, &cs->io_query, tb.cap)) != FND_OK) return fail(st);
     if ((st = calloc_dev(ctx, cs, &cs->io_target, tb.cap)) != FND_OK) return fail(st);

@@ -34,7 +34,7 @@ struct CacheState {
     uint32_t sample_cap = 0;
     float4* rec = nullptr;            // per training path: kCacheMaxRecords records of kCacheRecordF4 float4
     uint32_t* rec_count = nullptr;
-    uint32_t* t_list = nullptr;       // training paths of the current batch (kCacheTrainMaxPaths)
+    unsigned long long* t_list = nullptr;  // per training slot: global path index (kCacheTrainMaxPaths)
     CacheView view{};
     bool bounds_set = false;
     bool grid_grad_dirty = false;     // grid gradient holds values no update has consumed

@@ -12,6 +12,8 @@
 // Appends to the next extension queue and to the shadow queue are compacted
 // per wave: a ballot of the lanes that append, one atomicAdd per wave, and
 // each lane's slot from the popcount of the ballot below it.
+#include <algorithm>
+
 #include "pt_device.hpp"
 #include "pt_shading.hpp"
 #include "wavefront.hpp"
@@ -90,6 +92,7 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
                         if (ti < job.train_count) {
                             train = job.train_base + uint32_t(ti);
                             a.b.rec_count[train] = 0u;
+                            a.b.t_list[train] = g;  // slots are unique: no atomic
                         }
                     }
                 }
@@ -114,9 +117,6 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             a.b.p_L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.b.p_train[i] = train;
         }
-        const bool trains = train != kNoTrainSlot;
-        const uint32_t tpos = wave_append(trains, &a.b.counters[kCntTrainList], nullptr);
-        if (trains) a.b.t_list[tpos] = i;
         const uint32_t slot = wave_append(live, &a.b.counters[kCntExtA], a.b.rays);
         if (live) {
             a.b.ext_rays[0][slot] = ray;
@@ -124,6 +124,11 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             a.b.ext_scene[0][slot] = a.jobs[j].scene_slot;
         }
     }
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
 // Zeroes the counter slots whose bit is set in mask.
@@ -255,10 +260,10 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                     }
                     // cache vertices: Lambert, and GGX at least glossy_min rough (with the
                     // outgoing direction and roughness as inputs); from bounce
-                    // kCacheAnyGlossyBounce on any GGX vertex, whose throughput is small by then
+                    // glossy_any_bounce on any GGX vertex, whose throughput is small by then
                     const bool diffuse = mat.type == 0u;
                     const bool cachable =
-                        diffuse || (mat.type == 1u && (mat.roughness >= a.b.glossy_min || bounce >= kCacheAnyGlossyBounce));
+                        diffuse || (mat.type == 1u && (mat.roughness >= a.b.glossy_min || bounce >= a.b.glossy_any_bounce));
                     const float q_rough = diffuse ? 0.0f : mat.roughness;
                     const V3 q_dir = diffuse ? v3(0, 0, 0) : -dir;
                     // a training path records its first rec_max cache vertices and ends in
@@ -394,14 +399,17 @@ __global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
 // (L_path - L_mark) / weight, the albedo-demodulated radiance the path
 // gathered after that vertex's own light sample (per channel; vertices with a
 // vanishing weight are skipped).
-__global__ __launch_bounds__(256) void k_cache_collect(WfArgs a)
+__global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint64_t g0, uint32_t n)
 {
-    const uint32_t n = a.b.counters[kCntTrainList];  // the batch's training paths
+    // the training slots whose path lies in this batch [g0, g0 + n)
+    const uint32_t ns = a.b.t_slots;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    for (uint32_t base = blockIdx.x * blockDim.x; base < ns; base += stride) {
         const uint32_t j = base + threadIdx.x;
-        const uint32_t i = j < n ? a.b.t_list[j] : 0u;
-        const uint32_t tslot = j < n ? a.b.p_train[i] : kNoTrainSlot;
+        const unsigned long long g = j < ns ? a.b.t_list[j] : ~0ull;
+        const bool mine = g >= g0 && g - g0 < n;
+        const uint32_t i = mine ? uint32_t(g - g0) : 0u;
+        const uint32_t tslot = mine ? a.b.p_train[i] : kNoTrainSlot;
         const uint32_t rc = tslot != kNoTrainSlot ? a.b.rec_count[tslot] : 0u;
         const float4 L = rc ? a.b.p_L[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         // one wave-aggregated append per record index (every lane takes part)
@@ -443,9 +451,14 @@ void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s)
     hipLaunchKernelGGL(k_cache_apply, dim3(grid), dim3(256), 0, s, a);
 }
 
-void launch_cache_collect(const WfArgs& a, uint32_t grid, hipStream_t s)
+void launch_cache_collect(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_cache_collect, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_cache_collect, dim3(grid), dim3(256), 0, s, a, g0, n);
+}
+
+void launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u64, dim3(std::max(1u, std::min(1024u, (n + 255u) / 256u))), dim3(256), 0, s, p, n, v);
 }
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)

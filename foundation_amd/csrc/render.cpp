@@ -425,6 +425,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         wa.b.t_list = cs->t_list;
         wa.b.rec_max = std::min(cs->desc.train_depth, kCacheMaxRecords);
         wa.b.glossy_min = cs->desc.glossy_min_roughness;
+        wa.b.glossy_any_bounce = cs->desc.glossy_any_bounce;
         wa.b.t_query = cs->s_query;
         wa.b.t_target = cs->s_target;
         wa.b.t_cap = cs->sample_cap;
@@ -438,7 +439,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         for (uint64_t g0 = 0; g0 < paths; g0 += batch) {
             const uint32_t n = uint32_t(std::min<uint64_t>(batch, paths - g0));
             const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
-            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntTrainList), ctx->stream);
+            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery), ctx->stream);
             launch_generate(wa, g0, n, grid, ctx->stream);
             // One traversal launch per bounce: the extension rays of bounce b + 1
             // (closest hit) and the shadow rays of bounce b (any hit) share it,
@@ -484,7 +485,8 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                                      cg, ctx->stream);
                 launch_cache_apply(wa, grid, ctx->stream);
             }
-            if (collect) launch_cache_collect(wa, std::max(1u, std::min(grid, kCacheTrainMaxPaths / 256u)), ctx->stream);
+            if (collect)
+                launch_cache_collect(wa, g0, n, std::max(1u, std::min(grid, (wa.b.t_slots + 255u) / 256u)), ctx->stream);
             launch_accumulate(wa, g0, n, grid, ctx->stream);
             e = check_hip(ctx, hipGetLastError(), "render launch");
             if (e != FND_OK) return e;
@@ -496,8 +498,13 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     if (st != FND_OK) return st;
     (void)hipEventRecord(ctx->ev[2], ctx->stream);
 
-    // ---- the frame; training paths among its own paths record their diffuse vertices
-    if (any_train) launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
+    // ---- the frame; training paths among its own paths record their cache vertices
+    if (any_train) {
+        wa.b.t_slots = 0;
+        for (uint32_t i = 0; i < num_jobs; ++i) wa.b.t_slots += hj[i].train_count;
+        launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
+        launch_fill_u64(wa.b.t_list, wa.b.t_slots, ~0ull, ctx->stream);
+    }
     st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice,
                                        ctx->stream), "upload jobs");
     if (st != FND_OK) return st;

@@ -70,7 +70,6 @@ enum WfCounter : uint32_t {
     kCntWorkShadow = 4,
     kCntQuery = 5,
     kCntTrain = 6,
-    kCntTrainList = 7,
     kCntCount = 8
 };
 
@@ -103,9 +102,11 @@ struct WfBuffers {
     // training records: per training slot, rec_max records of kCacheRecordF4 float4
     float4* rec;
     uint32_t* rec_count;
-    uint32_t* t_list;         // batch-local path indices of the batch's training paths
+    unsigned long long* t_list; // per training slot: the global path index (~0 until generated)
+    uint32_t t_slots;         // training slots of this call
     uint32_t rec_max;
     float glossy_min;         // GGX at least this rough queries and trains the cache like Lambert
+    uint32_t glossy_any_bounce; // from this bounce on every GGX vertex does
     CacheQuery* t_query;      // gathered training samples
     float4* t_target;
     uint32_t t_cap;
@@ -125,6 +126,7 @@ void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s);
-void launch_cache_collect(const WfArgs& a, uint32_t grid, hipStream_t s);
+void launch_cache_collect(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
 
 }  // namespace fnd

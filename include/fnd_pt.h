@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FND_PT_ABI_VERSION 3u
+#define FND_PT_ABI_VERSION 4u
 
 typedef uint64_t fnd_handle;
 #define FND_INVALID_HANDLE (~(uint64_t)0)
@@ -351,6 +351,7 @@ typedef struct fnd_cache_desc {
                                    strided subset of the frame's samples */
     float glossy_min_roughness; /* GGX vertices at least this rough also terminate into (and train) the
                                    cache, with their outgoing direction as input; > 1 disables */
+    uint32_t glossy_any_bounce; /* from this bounce on (0 = camera hit) every GGX vertex does so */
     uint64_t seed;              /* initialisation */
     float bounds_lo[3];         /* region the grid covers; lo == hi takes the bounds of the */
     float bounds_hi[3];         /* first scene rendered with the cache */

@@ -2,8 +2,9 @@
 
 Tolerances: the GPU forward stores weights, grid values and activations in
 FP16 (the oracle rounds the same quantities with round_fp16=True); the
-backward runs in BF16 (8-bit mantissa) with FP32 accumulation, so gradients
-are compared as whole vectors per layer, |g_gpu - g_ref| / |g_ref| < 5e-2.
+backward runs in BF16 (8-bit mantissa) with FP32 accumulation, and the hash
+grid's gradient is accumulated by packed BF16 atomics, so gradients are
+compared as whole vectors per layer, |g_gpu - g_ref| / |g_ref| < 5e-2.
 """
 import ctypes as C
 
@@ -111,7 +112,16 @@ def test_adam_first_step(ctx, params):
     np.testing.assert_allclose((new_m - mlp)[sel], -1e-2 * np.sign(dm[sel]), rtol=0, atol=2e-6)
     assert np.all(new_m[dm == 0] == mlp[dm == 0])
     gs = np.abs(dg) > 1e-6
-    np.testing.assert_allclose((new_g - grid)[gs], -1e-2 * np.sign(dg[gs]), rtol=0, atol=2e-6)
+    step = (new_g - grid)[gs]
+    ok = np.abs(step + 1e-2 * np.sign(dg[gs])) <= 2e-6
+    # The bf16 atomic sums of the two calls add in different orders, so an
+    # entry whose gradient nearly cancels can change sign between them: allow
+    # that for at most 1e-4 of the entries, each within 1e-3 of the largest
+    # gradient.  Every other entry moves by exactly -lr * sign(gradient).
+    bad = ~ok
+    assert bad.sum() <= 1e-4 * gs.sum(), bad.sum()
+    assert np.all(np.abs(dg[gs][bad]) < 1e-3 * np.abs(dg).max())
+    assert np.all(np.abs(np.abs(step) - 1e-2) <= 2e-6)
 
 
 def test_training_fits_a_smooth_field(ctx):
