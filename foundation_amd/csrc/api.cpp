@@ -139,10 +139,16 @@ fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end)
     return FND_OK;
 }
 
-uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block)
+// Persistent grid: waves_per_cu waves per CU, never more blocks than fit on
+// the chip at once (a persistent wave that cannot start until others finish
+// only adds spill memory and a late start).
+uint32_t trace_grid(const fnd_pt_ctx* ctx, const TraceLaunch& l)
 {
     const uint32_t waves = std::max(1u, ctx->num_cus * ctx->tuning.waves_per_cu);
-    return std::max(1u, waves * 64u / block);
+    uint32_t grid = std::max(1u, waves * 64u / l.block);
+    const uint32_t per_cu = trace_blocks_per_cu(l);
+    if (per_cu) grid = std::min(grid, per_cu * ctx->num_cus);
+    return grid;
 }
 
 void* host_alloc(const fnd_host_allocator* a, size_t size, size_t align)
@@ -177,9 +183,9 @@ fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes
 void default_tuning(fnd_tuning& t)
 {
     t.block_size = 128;
-    t.stack_lds = 16;
-    t.waves_per_cu = 20;
-    t.refill = 16;
+    t.stack_lds = 8;
+    t.waves_per_cu = 24;
+    t.refill = 32;
     t.queue_capacity = 1u << 21;
     t.tile = 8;
 }
@@ -563,7 +569,7 @@ static fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float
     l.count = count;
     l.block = count ? 128u : ctx->tuning.block_size;
     l.stack_lds = count ? 16u : ctx->tuning.stack_lds;
-    l.grid = trace_grid(ctx, l.block);
+    l.grid = trace_grid(ctx, l);
     fnd_status st = ensure_spill(ctx, s->info.stack_entries, l.stack_lds, l.grid * l.block);
     if (st != FND_OK) return st;
     TraceArgs a{};

@@ -90,7 +90,7 @@ void dev_free(fnd_pt_ctx* ctx, void* ptr, uint64_t bytes);
 fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_lds, uint32_t threads);
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
 Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
-uint32_t trace_grid(const fnd_pt_ctx* ctx, uint32_t block);
+uint32_t trace_grid(const fnd_pt_ctx* ctx, const TraceLaunch& l);
 void resolve_timings(fnd_pt_ctx* ctx);
 fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);
 void render_state_free(fnd_pt_ctx* ctx);

@@ -52,9 +52,11 @@ __device__ __forceinline__ void ray_setup(RayK& r, float ox, float oy, float oz,
     const float dkz = sel3(kz, dx, dy, dz);
     if (dkz < 0.0f) { const uint32_t t = kx; kx = ky; ky = t; }
     r.perm = kx | (ky << 2) | (kz << 4);
-    r.Sx = sel3(kx, dx, dy, dz) / dkz;
-    r.Sy = sel3(ky, dx, dy, dz) / dkz;
-    r.Sz = 1.0f / dkz;
+    // Sz is the slab reciprocal of axis kz and Sx, Sy are products with it
+    // (Woop divides d[kx] / d[kz]; one per-ray constant is as watertight as the other)
+    r.Sz = sel3(kz, r.ix, r.iy, r.iz);
+    r.Sx = sel3(kx, dx, dy, dz) * r.Sz;
+    r.Sy = sel3(ky, dx, dy, dz) * r.Sz;
     r.okx = sel3(kx, ox, oy, oz);
     r.oky = sel3(ky, ox, oy, oz);
     r.okz = sel3(kz, ox, oy, oz);
@@ -117,9 +119,10 @@ __device__ __forceinline__ bool isect_tri(const RayK& r, const float4 a, const f
     return true;
 }
 
-__device__ __forceinline__ float exp2_int(int e)
+// x * 2^e, exact (v_ldexp_f32)
+__device__ __forceinline__ float scale_exp2(float x, int e)
 {
-    return __int_as_float((e + 127) << 23);
+    return __builtin_amdgcn_ldexpf(x, e);
 }
 
 __device__ __forceinline__ float ubyte_f(uint32_t w, int k)

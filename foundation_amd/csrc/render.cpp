@@ -411,7 +411,8 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     tl.multi = true;
     tl.block = ctx->tuning.block_size;
     tl.stack_lds = ctx->tuning.stack_lds;
-    tl.grid = trace_grid(ctx, tl.block);
+    tl.mode = kModeMixedInternal;  // the mode every launch below uses
+    tl.grid = trace_grid(ctx, tl);
     st = ensure_spill(ctx, max_stack, tl.stack_lds, tl.grid * tl.block);
     if (st != FND_OK) return st;
 

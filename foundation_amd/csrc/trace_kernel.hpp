@@ -11,11 +11,13 @@
 //    LDS part go to a per-thread spill area in HBM sized from the tree depth.
 //    An entry is (word, entry distance); on pop, entries farther than the best
 //    hit so far are dropped.
-//  * Each loop iteration does one unit of work per lane: one 64-byte node
-//    (4 quantised slab tests, sort, push), one triangle, or an instance
-//    transition.  Leaf children are pushed as (first, count) triangle ranges.
+//  * Each loop iteration does at most one unit of work per lane: one 64-byte
+//    node (4 quantised slab tests, sort, push), one triangle, or an instance
+//    entry; the kinds that run in an iteration are picked by lanes per cost.
+//    Leaf children are pushed as (first, count) triangle ranges.
 //  * Two-level scenes: an instance entry moves the ray into object space and
-//    pushes a return marker; popping the marker restores the world ray.
+//    pushes a return marker; popping the marker restores the world ray and
+//    the pop continues.
 #pragma once
 #include "pt_device.hpp"
 
@@ -66,6 +68,10 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u));
 }
 
+// Lanes-per-cost weights of the three kinds of work, about 1000 / (VALU
+// instructions of the kind's branch): node 190, triangle 85, instance 122.
+constexpr uint32_t kCostWeightNode = 5, kCostWeightTri = 12, kCostWeightInst = 8;
+
 template <int MODE, bool TWO, bool MULTI, bool COUNT, int SLDS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
 {
@@ -84,7 +90,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
     bool active = false, retired = false;
     uint32_t ray_idx = 0;
     RayK r;   // the ray in the current space
-    RayK rw;  // the world-space ray: leaving an instance restores it without divisions
+    // The world ray, kept for instance entries and returns.  Every node of a
+    // two-level scene's top level is flagged top and has only instance
+    // leaves, so world space needs origin and 1/d for slab tests and the
+    // direction for the transform; the shear constants of r are rebuilt by
+    // each instance entry.
+    float wox = 0.f, woy = 0.f, woz = 0.f, wdx = 0.f, wdy = 0.f, wdz = 0.f, wix = 0.f, wiy = 0.f, wiz = 0.f;
     float tmin = 0.f, tmax = 0.f, tbest = 0.f, bV = 0.f, bW = 0.f, bdet = 1.f;
     uint32_t best = 0xFFFFFFFFu, best_aux = 0u;
     uint32_t cur = 0, sp = 0, id_base = 0, inst_slot = 0xFFFFFFFFu;
@@ -99,10 +110,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
         ++sp;
     };
 
-    // The first fetch is static (lane gtid takes ray gtid), so a launch whose
-    // queue is short or empty retires its surplus waves without touching the
-    // shared counter; later fetches take ray blocks past the grid's first
-    // gtid range from that counter.
+    // Ray fetch.  The first fetch is static (lane gtid takes ray gtid), so a
+    // launch whose queue is short or empty retires its surplus waves without
+    // touching the shared counter; later fetches take exactly the idle lanes'
+    // worth of rays past the grid's first gtid range from that counter.  The
+    // one shared counter keeps the rays in flight a sliding window over the
+    // ray array, and neighbouring rays share top-level nodes in L2: measured
+    // faster than fixed per-wave shares of the array, and than reserving the
+    // next block one refill ahead (the reply is waited for by the wave's next
+    // in-order load wait anyway).
     const uint32_t grid_threads = gridDim.x * BLOCK;
     bool first_fetch = true;
     for (;;) {
@@ -119,8 +135,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 if (int(__lane_id()) == leader) base = grid_threads + atomicAdd(args.counter, nidle);
                 base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
             }
+            const uint32_t idx = base + lanes_below(idle);
             if (!active && !retired) {
-                const uint32_t idx = base + lanes_below(idle);
                 if (idx < nrays) {
                     const bool second = MIXED && idx >= nrays1;
                     ray_idx = second ? idx - nrays1 : idx;
@@ -131,8 +147,12 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                     if (MULTI) sc = args.scenes[(second ? args.ray_scene2 : args.ray_scene)[ray_idx]];
                     tmin = ra.w;
                     tmax = rb.w;
-                    ray_setup(rw, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z);
-                    r = rw;
+                    ray_setup(r, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z);
+                    if (TWO) {
+                        wox = ra.x; woy = ra.y; woz = ra.z;
+                        wdx = rb.x; wdy = rb.y; wdz = rb.z;
+                        wix = r.ix; wiy = r.iy; wiz = r.iz;
+                    }
                     tbest = tmax;
                     best = 0xFFFFFFFFu;
                     best_aux = 0xFFFFFFFFu;
@@ -151,8 +171,21 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
         if (__ballot(active) == 0) break;
         if (!active) continue;
 
+        // Work selection: a lane's next unit is a node, a triangle or an
+        // instance entry, and each kind is a branch the whole wave pays for
+        // once any lane takes it.  Every kind whose lanes-per-cost is at least
+        // half the best kind's runs this iteration; the others wait a turn.
+        // Per-lane order is unchanged, so results are too.
+        const bool is_node = cur < kEntryInstance;
+        const bool is_tri = (cur & kEntryTri) != 0u;
+        const uint32_t s_node = uint32_t(__popcll(__ballot(is_node))) * kCostWeightNode;
+        const uint32_t s_tri = uint32_t(__popcll(__ballot(is_tri))) * kCostWeightTri;
+        const uint32_t s_inst = TWO ? uint32_t(__popcll(__ballot(!is_node && !is_tri))) * kCostWeightInst : 0u;
+        const uint32_t s_best = max(s_node, max(s_tri, s_inst));
+        const bool run_node = 2u * s_node >= s_best, run_tri = 2u * s_tri >= s_best, run_inst = 2u * s_inst >= s_best;
+
         bool pop = false, finish = false;
-        if (cur < kEntryInstance) {
+        if (is_node && run_node) {
             // ---- one 64-byte wide node: four quantised slab tests
             if (COUNT) ++c_nodes;
             const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + cur);
@@ -165,9 +198,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const int ez = int(int8_t((n0.w >> 16) & 0xffu));
             const uint32_t imask = n0.w >> 24;
             const uint32_t child_base = n1.x, prim_base = n1.y, meta = n1.z, flags = n1.w;
-            const float sx = exp2_int(ex) * r.ix;
-            const float sy = exp2_int(ey) * r.iy;
-            const float sz = exp2_int(ez) * r.iz;
+            const float sx = scale_exp2(r.ix, ex);
+            const float sy = scale_exp2(r.iy, ey);
+            const float sz = scale_exp2(r.iz, ez);
             const float cx = (__uint_as_float(n0.x) - r.ox) * r.ix;
             const float cy = (__uint_as_float(n0.y) - r.oy) * r.iy;
             const float cz = (__uint_as_float(n0.z) - r.oz) * r.iz;
@@ -188,15 +221,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 const uint32_t m = (meta >> (8 * k)) & 0xffu;
                 const bool internal = ((imask >> k) & 1u) != 0u;
                 const bool hit = (internal || m != 0u) && tn <= tf * kSlack;
-                uint32_t e;
-                if (internal)
-                    e = child_base + uint32_t(__popc(imask & ((1u << k) - 1u)));
-                else if (top)
-                    e = kEntryInstance | (prim_base + (m >> 3));
-                else
-                    e = kEntryTri | ((prim_base + (m >> 3)) << 3) | (m & 7u);
+                // the child's stack entry, selected without branches
+                const uint32_t leaf = prim_base + (m >> 3);
+                const uint32_t e_leaf = top ? (kEntryInstance | leaf) : (kEntryTri | (leaf << 3) | (m & 7u));
                 tk[k] = hit ? tn : __builtin_inff();
-                ek[k] = e;
+                ek[k] = internal ? child_base + uint32_t(__popc(imask & ((1u << k) - 1u))) : e_leaf;
             }
             const bool any_l = MIXED ? lane_any : ANY_CONST;
             if (!any_l) {
@@ -224,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 }
                 if (!have) pop = true;
             }
-        } else if (cur & kEntryTri) {
+        } else if (is_tri && run_tri) {
             // ---- one triangle of a leaf range
             if (COUNT) ++c_tris;
             const uint32_t ti = (cur >> 3) & 0x0FFFFFFFu;
@@ -248,12 +277,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                 else
                     pop = true;
             }
-        } else if (TWO && cur == kEntryReturn) {
-            r = rw;
-            id_base = 0;
-            inst_slot = 0xFFFFFFFFu;
-            pop = true;
-        } else if (TWO) {
+        } else if (TWO && !is_node && !is_tri && run_inst) {
             // ---- enter an instance: object-space ray, return marker
             if (COUNT) ++c_inst;
             const uint32_t ii = cur & 0x3FFFFFFFu;
@@ -262,7 +286,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const uint4 m3 = reinterpret_cast<const uint4*>(ip)[3];
             const float mm[12] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w, m2.x, m2.y, m2.z, m2.w};
             float rox, roy, roz, rdx, rdy, rdz;
-            xform_ray(mm, rw.ox, rw.oy, rw.oz, rw.dx, rw.dy, rw.dz, rox, roy, roz, rdx, rdy, rdz);
+            xform_ray(mm, wox, woy, woz, wdx, wdy, wdz, rox, roy, roz, rdx, rdy, rdz);
             ray_setup(r, rox, roy, roz, rdx, rdy, rdz);
             push(kEntryReturn, -__builtin_inff());
             cur = m3.x;
@@ -270,17 +294,39 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             inst_slot = ii;
         }
         if (pop) {
+            bool left_instance = false;
             for (;;) {
                 if (sp == 0) {
                     finish = true;
                     break;
                 }
                 --sp;
-                const uint2 e = sp < SLDS ? stk[sp * BLOCK + tid] : args.spill[size_t(sp - SLDS) * args.spill_stride + gtid];
+                // one 8-byte LDS read; the spill read is volatile so the two are
+                // not merged into one generic (flat) load
+                uint2 e;
+                if (sp < SLDS) {
+                    e = stk[sp * BLOCK + tid];
+                } else {
+                    const volatile unsigned long long* q = reinterpret_cast<const volatile unsigned long long*>(
+                        args.spill + size_t(sp - SLDS) * args.spill_stride + gtid);
+                    const unsigned long long w = *q;
+                    e = make_uint2(uint32_t(w), uint32_t(w >> 32));
+                }
+                if (TWO && e.x == kEntryReturn) {
+                    left_instance = true;  // back to the world ray; keep popping
+                    continue;
+                }
                 if ((MIXED ? lane_any : ANY_CONST) || __uint_as_float(e.y) <= tbest * kSlack) {
                     cur = e.x;
                     break;
                 }
+            }
+            if (TWO && left_instance) {
+                // back in the top level: slab-test fields only (no triangles there)
+                r.ox = wox; r.oy = woy; r.oz = woz;
+                r.ix = wix; r.iy = wiy; r.iz = wiz;
+                id_base = 0;
+                inst_slot = 0xFFFFFFFFu;
             }
         }
         if (finish) {
@@ -316,6 +362,8 @@ struct TraceLaunch {
 };
 
 hipError_t launch_trace(const TraceLaunch& l, const TraceArgs& args, hipStream_t stream);
+// Blocks of the launch's kernel that fit on one CU at once (0 if unknown).
+uint32_t trace_blocks_per_cu(const TraceLaunch& l);
 bool trace_config_supported(uint32_t block, uint32_t stack_lds);
 
 }  // namespace fnd

@@ -22,70 +22,60 @@ bool trace_config_supported(uint32_t block, uint32_t stack_lds)
 }
 
 template <int B, int S>
-static hipError_t launch_cfg(const TraceLaunch& l, const TraceArgs& args, hipStream_t stream)
+static const void* kernel_cfg(const TraceLaunch& l)
 {
-    const dim3 grid(l.grid), block(B);
     if (l.multi) {
-        if (l.mode == kModeClosestInternal)
-            hipLaunchKernelGGL((k_trace<kModeClosestInternal, true, true, false, S, B>), grid, block, 0, stream, args);
-        else if (l.mode == kModeAnyInternal)
-            hipLaunchKernelGGL((k_trace<kModeAnyInternal, true, true, false, S, B>), grid, block, 0, stream, args);
-        else if (l.mode == kModeMixedInternal)
-            hipLaunchKernelGGL((k_trace<kModeMixedInternal, true, true, false, S, B>), grid, block, 0, stream, args);
-        else
-            return hipErrorInvalidValue;
-        return hipGetLastError();
+        if (l.mode == kModeClosestInternal) return reinterpret_cast<const void*>(&k_trace<kModeClosestInternal, true, true, false, S, B>);
+        if (l.mode == kModeAnyInternal) return reinterpret_cast<const void*>(&k_trace<kModeAnyInternal, true, true, false, S, B>);
+        if (l.mode == kModeMixedInternal) return reinterpret_cast<const void*>(&k_trace<kModeMixedInternal, true, true, false, S, B>);
+        return nullptr;
     }
-    if (l.count) {
-        // counting variants exist for the default configuration only
-        return hipErrorInvalidValue;
-    }
+    if (l.count) return nullptr;  // counting variants exist for the default configuration only
     if (l.two_level) {
-        if (l.mode == kModeClosest)
-            hipLaunchKernelGGL((k_trace<kModeClosest, true, false, false, S, B>), grid, block, 0, stream, args);
-        else if (l.mode == kModeAny)
-            hipLaunchKernelGGL((k_trace<kModeAny, true, false, false, S, B>), grid, block, 0, stream, args);
-        else
-            return hipErrorInvalidValue;
-    } else {
-        if (l.mode == kModeClosest)
-            hipLaunchKernelGGL((k_trace<kModeClosest, false, false, false, S, B>), grid, block, 0, stream, args);
-        else if (l.mode == kModeAny)
-            hipLaunchKernelGGL((k_trace<kModeAny, false, false, false, S, B>), grid, block, 0, stream, args);
-        else
-            return hipErrorInvalidValue;
+        if (l.mode == kModeClosest) return reinterpret_cast<const void*>(&k_trace<kModeClosest, true, false, false, S, B>);
+        if (l.mode == kModeAny) return reinterpret_cast<const void*>(&k_trace<kModeAny, true, false, false, S, B>);
+        return nullptr;
     }
-    return hipGetLastError();
+    if (l.mode == kModeClosest) return reinterpret_cast<const void*>(&k_trace<kModeClosest, false, false, false, S, B>);
+    if (l.mode == kModeAny) return reinterpret_cast<const void*>(&k_trace<kModeAny, false, false, false, S, B>);
+    return nullptr;
 }
 
-static hipError_t launch_count(const TraceLaunch& l, const TraceArgs& args, hipStream_t stream)
+static const void* kernel_count(const TraceLaunch& l)
 {
-    const dim3 grid(l.grid), block(128);
-    if (l.two_level) {
-        if (l.mode == kModeClosest)
-            hipLaunchKernelGGL((k_trace<kModeClosest, true, false, true, 16, 128>), grid, block, 0, stream, args);
-        else
-            hipLaunchKernelGGL((k_trace<kModeAny, true, false, true, 16, 128>), grid, block, 0, stream, args);
-    } else {
-        if (l.mode == kModeClosest)
-            hipLaunchKernelGGL((k_trace<kModeClosest, false, false, true, 16, 128>), grid, block, 0, stream, args);
-        else
-            hipLaunchKernelGGL((k_trace<kModeAny, false, false, true, 16, 128>), grid, block, 0, stream, args);
-    }
-    return hipGetLastError();
+    if (l.block != 128 || l.stack_lds != 16) return nullptr;
+    if (l.two_level)
+        return l.mode == kModeClosest ? reinterpret_cast<const void*>(&k_trace<kModeClosest, true, false, true, 16, 128>)
+                                      : reinterpret_cast<const void*>(&k_trace<kModeAny, true, false, true, 16, 128>);
+    return l.mode == kModeClosest ? reinterpret_cast<const void*>(&k_trace<kModeClosest, false, false, true, 16, 128>)
+                                  : reinterpret_cast<const void*>(&k_trace<kModeAny, false, false, true, 16, 128>);
+}
+
+static const void* kernel_for(const TraceLaunch& l)
+{
+    if (l.count && !l.multi) return kernel_count(l);
+#define FND_PICK(B, S) \
+    if (l.block == B && l.stack_lds == S) return kernel_cfg<B, S>(l);
+    FND_TRACE_CONFIGS(FND_PICK)
+#undef FND_PICK
+    return nullptr;
+}
+
+uint32_t trace_blocks_per_cu(const TraceLaunch& l)
+{
+    const void* k = kernel_for(l);
+    int nb = 0;
+    if (!k || hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, int(l.block), 0) != hipSuccess) return 0;
+    return uint32_t(nb);
 }
 
 hipError_t launch_trace(const TraceLaunch& l, const TraceArgs& args, hipStream_t stream)
 {
-    if (l.count && !l.multi) {
-        if (l.block != 128 || l.stack_lds != 16) return hipErrorInvalidValue;
-        return launch_count(l, args, stream);
-    }
-#define FND_DISPATCH(B, S) \
-    if (l.block == B && l.stack_lds == S) return launch_cfg<B, S>(l, args, stream);
-    FND_TRACE_CONFIGS(FND_DISPATCH)
-#undef FND_DISPATCH
-    return hipErrorInvalidValue;
+    const void* k = kernel_for(l);
+    if (!k) return hipErrorInvalidValue;
+    TraceArgs a = args;
+    void* params[] = {&a};
+    return hipLaunchKernel(k, dim3(l.grid), dim3(l.block), params, 0, stream);
 }
 
 }  // namespace fnd

@@ -91,9 +91,12 @@ static void oray_setup(oray* r, const float* o, const float* d, float tmin, floa
     int ky = kx + 1; if (ky == 3) ky = 0;
     if (d[kz] < 0.0f) { int tmp = kx; kx = ky; ky = tmp; }
     r->kx = kx; r->ky = ky; r->kz = kz;
-    r->Sx = d[kx] / d[kz];
-    r->Sy = d[ky] / d[kz];
-    r->Sz = 1.0f / d[kz];
+    /* one division fewer than Woop's d[kx] / d[kz]: Sz is the reciprocal
+     * already formed for the slab tests and Sx, Sy are products with it; a
+     * per-ray constant either way, so shared edges stay watertight */
+    r->Sz = r->invd[kz];
+    r->Sx = d[kx] * r->Sz;
+    r->Sy = d[ky] * r->Sz;
 }
 
 /* Woop et al. 2013, section 3, FP32 only (see pt_oracle.h). */

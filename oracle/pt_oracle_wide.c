@@ -79,9 +79,12 @@ static void wray_setup(wray* r, const float* o, const float* d)
     int ky = kx == 2 ? 0 : kx + 1;
     if (d[kz] < 0.0f) { int t = kx; kx = ky; ky = t; }
     r->kx = kx; r->ky = ky; r->kz = kz;
-    r->Sx = d[kx] / d[kz];
-    r->Sy = d[ky] / d[kz];
-    r->Sz = 1.0f / d[kz];
+    /* one division fewer than Woop's d[kx] / d[kz]: Sz is the reciprocal
+     * already formed for the slab tests and Sx, Sy are products with it; a
+     * per-ray constant either way, so shared edges stay watertight */
+    r->Sz = r->invd[kz];
+    r->Sx = d[kx] * r->Sz;
+    r->Sy = d[ky] * r->Sz;
 }
 
 static int wray_isect(const wray* r, const float* a, const float* b, const float* c, float* t, float* u, float* v)

@@ -115,13 +115,14 @@ def test_adam_first_step(ctx, params):
     step = (new_g - grid)[gs]
     ok = np.abs(step + 1e-2 * np.sign(dg[gs])) <= 2e-6
     # The bf16 atomic sums of the two calls add in different orders, so an
-    # entry whose gradient nearly cancels can change sign between them: allow
-    # that for at most 1e-4 of the entries, each within 1e-3 of the largest
-    # gradient.  Every other entry moves by exactly -lr * sign(gradient).
+    # entry whose gradient nearly cancels can change sign, or cancel to exactly
+    # zero (the lazy update then skips it), between them: allow that for at
+    # most 1e-4 of the entries, each within 1e-3 of the largest gradient.
+    # Every other entry moves by exactly -lr * sign(gradient).
     bad = ~ok
     assert bad.sum() <= 1e-4 * gs.sum(), bad.sum()
     assert np.all(np.abs(dg[gs][bad]) < 1e-3 * np.abs(dg).max())
-    assert np.all(np.abs(np.abs(step) - 1e-2) <= 2e-6)
+    assert np.all((np.abs(step[bad]) == 0) | (np.abs(np.abs(step[bad]) - 1e-2) <= 2e-6))
 
 
 def test_training_fits_a_smooth_field(ctx):
