@@ -185,7 +185,7 @@ void default_tuning(fnd_tuning& t)
     t.block_size = 128;
     t.stack_lds = 8;
     t.waves_per_cu = 24;
-    t.refill = 32;
+    t.refill = 40;
     t.queue_capacity = 1u << 21;
     t.tile = 8;
 }

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <set>
 #include <sstream>
 #include <string>
@@ -33,7 +34,7 @@ struct Candidate {
 // Instantiated (block, LDS stack) pairs of the traversal kernel.
 const uint32_t kPairs[][2] = {{64, 16}, {64, 32}, {128, 8}, {128, 16}, {128, 32}, {256, 8}, {256, 16}};
 const uint32_t kWaves[] = {12, 16, 20, 24, 32};
-const uint32_t kRefill[] = {16, 32, 48};
+const uint32_t kRefill[] = {16, 32, 40, 48};
 const uint32_t kQueue[] = {1u << 20, 1u << 21, 1u << 22};
 const uint32_t kTile[] = {4, 8, 16};
 
@@ -58,13 +59,13 @@ std::vector<Candidate> candidates(const fnd_tuning& def, uint32_t count)
     uint64_t state = 0x7E57AB1Eull;
     for (int guard = 0; out.size() < count && guard < 10000; ++guard) {
         fnd_tuning t{};
-        const uint32_t p = pcg_next(state) % 7u;
+        const uint32_t p = pcg_next(state) % uint32_t(std::size(kPairs));
         t.block_size = kPairs[p][0];
         t.stack_lds = kPairs[p][1];
-        t.waves_per_cu = kWaves[pcg_next(state) % 5u];
-        t.refill = kRefill[pcg_next(state) % 3u];
-        t.queue_capacity = kQueue[pcg_next(state) % 3u];
-        t.tile = kTile[pcg_next(state) % 3u];
+        t.waves_per_cu = kWaves[pcg_next(state) % uint32_t(std::size(kWaves))];
+        t.refill = kRefill[pcg_next(state) % uint32_t(std::size(kRefill))];
+        t.queue_capacity = kQueue[pcg_next(state) % uint32_t(std::size(kQueue))];
+        t.tile = kTile[pcg_next(state) % uint32_t(std::size(kTile))];
         add(t);
     }
     return out;

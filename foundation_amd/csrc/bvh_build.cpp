@@ -265,6 +265,7 @@ void encode_node(WideNode& w, const Box* child, const bool* internal, const uint
     uint8_t imask = 0;
     uint32_t meta = 0;
     for (int k = 0; k < nc; ++k) {
+        imask |= uint8_t(0x10u << k);  // slot k holds a child
         if (internal[k])
             imask |= uint8_t(1u << k);
         else

@@ -32,14 +32,15 @@ constexpr uint32_t kMaxNodes = 1u << 30;
 
 // Traversal stack entry tags (32-bit words).
 constexpr uint32_t kEntryTri = 0x80000000u;      // | first << 3 | count
-constexpr uint32_t kEntryInstance = 0x40000000u; // | instance index
-constexpr uint32_t kEntryReturn = 0x7FFFFFFFu;   // back to the top level
+constexpr uint32_t kEntryInstance = 0x40000000u; // | index << 3 | 1 (index < 2^27 - 1)
+constexpr uint32_t kEntryReturn = 0x7FFFFFFFu;   // back to the top level (low bits 7: no instance entry)
+constexpr uint32_t kMaxInstances = (1u << 27) - 1u;
 // Anything below kEntryInstance is a node index.
 
 struct alignas(64) WideNode {
     float px, py, pz;        // quantisation origin
     int8_t ex, ey, ez;       // child box scale is 2^e per axis
-    uint8_t imask;           // bit k: slot k is an internal node
+    uint8_t imask;           // bit k: slot k is an internal node; bit 4 + k: slot k holds a child
     uint32_t child_base;     // node index of the first internal child
     uint32_t prim_base;      // first triangle (or instance) of the leaf children
     uint32_t meta;           // byte k: leaf slot (offset << 3 | count), 0 = empty or internal

@@ -46,6 +46,8 @@ fnd_status encode_scene(const fnd_scene_desc* d, HostScene& out, std::string& er
     if (d->num_instances == 0 && d->num_meshes != 1)
         return fail(err, FND_E_INVALID_ARG, "a scene without instances takes exactly one mesh");
     if (d->num_instances && !d->instances) return fail(err, FND_E_INVALID_ARG, "instances is null");
+    if (d->num_instances > kMaxInstances)
+        return fail(err, FND_E_INVALID_ARG, "at most %u instances", kMaxInstances);
     if (d->num_lights && !d->lights) return fail(err, FND_E_INVALID_ARG, "lights is null");
     if (d->num_materials && !d->materials) return fail(err, FND_E_INVALID_ARG, "materials is null");
     uint64_t total_tris = 0;

@@ -145,8 +145,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                     const float4 ra = rp[0];
                     const float4 rb = rp[1];
                     if (MULTI) sc = args.scenes[(second ? args.ray_scene2 : args.ray_scene)[ray_idx]];
-                    tmin = ra.w;
-                    tmax = rb.w;
+                    // canonical once here, so the slab tests' fmaxf/fminf need not
+                    // quiet them on every node
+                    tmin = __builtin_canonicalizef(ra.w);
+                    tmax = __builtin_canonicalizef(rb.w);
                     ray_setup(r, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z);
                     if (TWO) {
                         wox = ra.x; woy = ra.y; woz = ra.z;
@@ -210,6 +212,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const uint32_t ny = py_ ? n2.y : n3.x, fy = py_ ? n3.x : n2.y;
             const uint32_t nz = pz_ ? n2.z : n3.y, fz = pz_ ? n3.y : n2.z;
             const bool top = TWO && (flags & kNodeFlagTop) != 0u;
+            // a leaf child's entry is tag | ((prim_base + offset) << 3 | count) =
+            // tag | (prim_base << 3) + meta byte, for triangle ranges and instances alike
+            const uint32_t leaf_base = (top ? kEntryInstance : kEntryTri) | (prim_base << 3);
             float tk[4];
             uint32_t ek[4];
 #pragma unroll
@@ -220,12 +225,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                                        fminf(fmaf(ubyte_f(fz, k), sz, cz), tbest));
                 const uint32_t m = (meta >> (8 * k)) & 0xffu;
                 const bool internal = ((imask >> k) & 1u) != 0u;
-                const bool hit = (internal || m != 0u) && tn <= tf * kSlack;
-                // the child's stack entry, selected without branches
-                const uint32_t leaf = prim_base + (m >> 3);
-                const uint32_t e_leaf = top ? (kEntryInstance | leaf) : (kEntryTri | (leaf << 3) | (m & 7u));
+                const bool hit = ((imask >> (4 + k)) & 1u) != 0u && tn <= tf * kSlack;
                 tk[k] = hit ? tn : __builtin_inff();
-                ek[k] = internal ? child_base + uint32_t(__popc(imask & ((1u << k) - 1u))) : e_leaf;
+                ek[k] = internal ? child_base + uint32_t(__popc(imask & ((1u << k) - 1u))) : leaf_base + m;
             }
             const bool any_l = MIXED ? lane_any : ANY_CONST;
             if (!any_l) {
@@ -237,9 +239,24 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                     tk[a] = ta; tk[b] = tb; ek[a] = ea; ek[b] = eb;
                 };
                 cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
-                if (tk[3] != __builtin_inff()) push(ek[3], tk[3]);
-                if (tk[2] != __builtin_inff()) push(ek[2], tk[2]);
-                if (tk[1] != __builtin_inff()) push(ek[1], tk[1]);
+                // the hit children are a prefix of the sorted four: push 3, 2, 1
+                // (farthest first) where they are hits
+                const uint32_t v1 = tk[1] != __builtin_inff(), v2 = tk[2] != __builtin_inff(),
+                               v3 = tk[3] != __builtin_inff();
+                if (sp + 3u <= uint32_t(SLDS)) {
+                    // all three slots are in LDS: write them unconditionally at
+                    // prefix positions (a miss lands on a slot a later write or
+                    // the next push reuses), no branches
+                    const uint32_t p3 = sp, p2 = sp + v3, p1 = p2 + v2;
+                    stk[p3 * BLOCK + tid] = make_uint2(ek[3], __float_as_uint(tk[3]));
+                    stk[p2 * BLOCK + tid] = make_uint2(ek[2], __float_as_uint(tk[2]));
+                    stk[p1 * BLOCK + tid] = make_uint2(ek[1], __float_as_uint(tk[1]));
+                    sp = p1 + v1;
+                } else {
+                    if (v3) push(ek[3], tk[3]);
+                    if (v2) push(ek[2], tk[2]);
+                    if (v1) push(ek[1], tk[1]);
+                }
                 if (tk[0] != __builtin_inff()) cur = ek[0]; else pop = true;
             } else {
                 bool have = false;
@@ -280,7 +297,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
         } else if (TWO && !is_node && !is_tri && run_inst) {
             // ---- enter an instance: object-space ray, return marker
             if (COUNT) ++c_inst;
-            const uint32_t ii = cur & 0x3FFFFFFFu;
+            const uint32_t ii = (cur >> 3) & 0x07FFFFFFu;
             const float4* ip = reinterpret_cast<const float4*>(sc.insts + ii);
             const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
             const uint4 m3 = reinterpret_cast<const uint4*>(ip)[3];

@@ -175,8 +175,8 @@ static void walk(const wscene* s, const float* ray8, int anyhit, uint32_t* stk_e
                 uint32_t e;
                 if (internal)
                     e = n->child_base + (uint32_t)__builtin_popcount(n->imask & ((1u << k) - 1u));
-                else if (top)
-                    e = W_ENTRY_INST | (n->prim_base + (m >> 3));
+                else if (top) /* one instance per leaf: (index << 3) | 1 */
+                    e = W_ENTRY_INST | ((n->prim_base << 3) + m);
                 else
                     e = W_ENTRY_TRI | ((n->prim_base + (m >> 3)) << 3) | (m & 7u);
                 tk[k] = hit ? tn : INFINITY;
@@ -228,7 +228,7 @@ static void walk(const wscene* s, const float* ray8, int anyhit, uint32_t* stk_e
             pop = 1;
         } else {
             c_inst++;
-            const winst* in = &s->insts[cur & 0x3FFFFFFFu];
+            const winst* in = &s->insts[(cur >> 3) & 0x07FFFFFFu];
             const float* m = in->w2o;
             float oo[3], od[3];
             for (int i = 0; i < 3; ++i) {

@@ -134,5 +134,5 @@ def test_encode_single_triangle():
     enc = pt.encode_scene(d)
     assert enc["nodes"].shape == (1,) and enc["tris"].shape == (1,)
     n = enc["nodes"][0]
-    assert n["imask"] == 0 and (n["meta"] & 0xff) == 1  # one leaf slot with one triangle
+    assert n["imask"] == 0x10 and (n["meta"] & 0xff) == 1  # one leaf slot (no internal) with one triangle
     assert enc["stack_entries"] == 3 * enc["max_depth"] + 2

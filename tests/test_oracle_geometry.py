@@ -244,6 +244,8 @@ def test_encoder_boxes_are_conservative_and_leaves_partition(kind, kw):
         blo = np.full(3, np.inf)
         bhi = np.full(3, -np.inf)
         for k in range(4):
+            # bit 4 + k marks the slots the traversal may enter: internal or a non-empty leaf
+            assert ((imask >> (4 + k)) & 1) == int(bool((imask >> k) & 1) or ((meta >> (8 * k)) & 0xFF) != 0), (ni, k)
             if (imask >> k) & 1:
                 clo, chi = subtree(child + rank)
                 rank += 1
