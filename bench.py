@@ -282,10 +282,13 @@ def main():
         log(f"coherent: {coherent:.1f} Mrays/s, shadow any-hit: {shadow_rate:.1f} Mrays/s")
 
     traffic = None
+    valu = None
     if os.path.exists(args.pmc_summary):
         try:
             pmc = json.load(open(args.pmc_summary))
             traffic = pmc.get("hbm_bytes_per_launch")
+            v = pmc.get("valu") or {}
+            valu = {k: v[k] for k in ("valu_busy", "lane_utilization") if k in v} or None
         except (OSError, ValueError):
             traffic = None
 
@@ -359,6 +362,7 @@ def main():
                   "coherent_nodes_per_ray": round(ccounts["nodes"] / primary.shape[0], 3),
                   "scene_nodes": int(info.num_nodes), "scene_build_ms": round(info.build_ms, 1),
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
+                  "trace_valu_issue": valu,
                   "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
                   "path_tracing_10M": render, "time_to_rel_mse_0_01": relmse, "batch_64_scenes": batch},
     }

@@ -183,7 +183,7 @@ fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes
 void default_tuning(fnd_tuning& t)
 {
     t.block_size = 128;
-    t.stack_lds = 8;
+    t.stack_lds = 12;
     t.waves_per_cu = 24;
     t.refill = 40;
     t.queue_capacity = 1u << 21;

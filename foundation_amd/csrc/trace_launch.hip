@@ -7,9 +7,11 @@ namespace fnd {
     X(64, 16)                \
     X(64, 32)                \
     X(128, 8)                \
+    X(128, 12)               \
     X(128, 16)               \
     X(128, 32)               \
     X(256, 8)                \
+    X(256, 12)               \
     X(256, 16)
 
 bool trace_config_supported(uint32_t block, uint32_t stack_lds)

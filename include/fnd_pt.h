@@ -224,7 +224,7 @@ typedef struct fnd_pt_stats {
 /* Knobs the autotuner searches.  0 means "library default". */
 typedef struct fnd_tuning {
     uint32_t block_size;            /* threads; (block, stack_lds) pairs built: 64x16, 64x32, 128x8, */
-    uint32_t stack_lds;             /* 128x16, 128x32, 256x8, 256x16 (per-lane LDS stack entries) */
+    uint32_t stack_lds;             /* 128x12, 128x16, 128x32, 256x8, 256x12, 256x16 (LDS stack entries) */
     uint32_t waves_per_cu;          /* persistent grid = CUs * waves_per_cu waves, at most what fits */
     uint32_t refill;                /* refetch rays once this many lanes of a wave are idle (1..64) */
     uint32_t queue_capacity;        /* paths per wavefront batch (render) */
