@@ -28,18 +28,49 @@ __device__ __forceinline__ uint32_t lanes_below_mask(uint64_t mask)
 }
 
 // Wave-compacted append: returns this lane's slot (valid only where pred).
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter, unsigned long long* total)
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
 {
     const uint64_t mask = __ballot(pred);
     if (mask == 0) return 0;
     const int leader = __builtin_ctzll(mask);
     uint32_t base = 0;
-    if (int(__lane_id()) == leader) {
-        base = atomicAdd(counter, uint32_t(__popcll(mask)));
-        if (total) atomicAdd(total, (unsigned long long)__popcll(mask));
-    }
+    if (int(__lane_id()) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
     base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
     return base + lanes_below_mask(mask);
+}
+
+// Block-compacted append to Q queues at once, for 256-thread blocks: one
+// global atomic per queue per block instead of one per wave (the queue
+// counters are single hot addresses that every wave of a launch hits).
+// Every thread of the block must call it; returns each lane's slots (valid
+// only where its predicate holds).
+template <int Q>
+__device__ __forceinline__ void block_append(const bool (&pred)[Q], uint32_t* const (&counter)[Q], uint32_t (&slot)[Q])
+{
+    __shared__ uint32_t s_cnt[Q][4];
+    __shared__ uint32_t s_base[Q];
+    const uint32_t w = threadIdx.x >> 6;
+    uint64_t m[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) m[q] = __ballot(pred[q]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) s_cnt[q][w] = uint32_t(__popcll(m[q]));
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(Q)) {
+        const uint32_t q = threadIdx.x;
+        const uint32_t total = s_cnt[q][0] + s_cnt[q][1] + s_cnt[q][2] + s_cnt[q][3];
+        s_base[q] = total ? atomicAdd(counter[q], total) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        uint32_t before = 0;
+        for (uint32_t v = 0; v < w; ++v) before += s_cnt[q][v];
+        slot[q] = s_base[q] + before + lanes_below_mask(m[q]);
+    }
+    __syncthreads();  // s_cnt and s_base are reused by the next call
 }
 
 __device__ __forceinline__ uint32_t find_job(const JobDev* jobs, uint32_t num_jobs, uint64_t g)
@@ -117,11 +148,12 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
             a.b.p_L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
             a.b.p_train[i] = train;
         }
-        const uint32_t slot = wave_append(live, &a.b.counters[kCntExtA], a.b.rays);
+        uint32_t slot[1];
+        block_append<1>({live}, {&a.b.counters[kCntExtA]}, slot);
         if (live) {
-            a.b.ext_rays[0][slot] = ray;
-            a.b.ext_path[0][slot] = i;
-            a.b.ext_scene[0][slot] = a.jobs[j].scene_slot;
+            a.b.ext_rays[0][slot[0]] = ray;
+            a.b.ext_path[0][slot[0]] = i;
+            a.b.ext_scene[0][slot[0]] = a.jobs[j].scene_slot;
         }
     }
 }
@@ -314,19 +346,21 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             }
             a.b.p_L[path] = make_float4(L.x, L.y, L.z, 0.f);
         }
-        const uint32_t es = wave_append(cont, &a.b.counters[nxt], a.b.rays);
+        // one block-aggregated append for the three queues
+        uint32_t slots[3];
+        block_append<3>({cont, query, shadow}, {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow]},
+                        slots);
+        const uint32_t es = slots[0], qs = slots[1], ss = slots[2];
         if (cont) {
             a.b.ext_rays[nxt][es] = next_ray;
             a.b.ext_path[nxt][es] = path;
             a.b.ext_scene[nxt][es] = scene_slot;
         }
-        const uint32_t qs = wave_append(query, &a.b.counters[kCntQuery], nullptr);
         if (query) {
             a.b.q_query[qs] = cq;
             a.b.q_path[qs] = path;
             a.b.q_weight[qs] = make_float4(q_w.x, q_w.y, q_w.z, 0.f);
         }
-        const uint32_t ss = wave_append(shadow, &a.b.counters[kCntShadow], a.b.rays);
         if (shadow) {
             a.b.sh_rays[ss] = sh_ray;
             a.b.sh_path[ss] = path;
@@ -430,7 +464,7 @@ __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint64_t g0, ui
                     take = true;
                 }
             }
-            const uint32_t slot = wave_append(take, &a.b.counters[kCntTrain], nullptr);
+            const uint32_t slot = wave_append(take, &a.b.counters[kCntTrain]);
             if (take && slot < a.b.t_cap) {
                 a.b.t_query[slot] = q;
                 a.b.t_target[slot] = target;

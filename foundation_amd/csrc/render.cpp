@@ -457,6 +457,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             ta.hits2 = rs->b.sh_hits;
             ta.ray_scene2 = rs->b.sh_scene;
             ta.nrays2_dev = &rs->b.counters[kCntShadow];
+            ta.ray_total = rs->b.rays;
             TraceLaunch l = tl;
             l.mode = kModeMixedInternal;
             auto trace = [&](uint32_t q, const char* what) -> fnd_status {

@@ -61,6 +61,7 @@ struct TraceArgs {
     float4* hits2;
     const uint32_t* ray_scene2;
     const uint32_t* nrays2_dev;
+    unsigned long long* ray_total;  // if set, this launch's ray count is added here (statistics)
 };
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -84,6 +85,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t nrays1 = args.nrays_dev ? *args.nrays_dev : args.nrays;
     const uint32_t nrays = MIXED ? nrays1 + *args.nrays2_dev : nrays1;
+    if (args.ray_total && gtid == 0) atomicAdd(args.ray_total, (unsigned long long)nrays);
     bool lane_any = ANY_CONST;  // per lane in the mixed mode, a constant otherwise
 
     SceneDev sc = args.scene;

@@ -93,7 +93,7 @@ struct WfBuffers {
     float4* sh_contrib;
     float4* sh_hits;
     uint32_t* counters;       // kCntCount
-    unsigned long long* rays; // rays appended (extension + shadow), cumulative
+    unsigned long long* rays; // rays traced (extension + shadow), cumulative: added by the trace launches
     // radiance-cache queries (at most one per path per batch)
     CacheQuery* q_query;
     uint32_t* q_path;
