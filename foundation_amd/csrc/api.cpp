@@ -307,8 +307,8 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
     ctx->framebuffers.clear();
     render_state_free(ctx);
     cache_free(ctx);
-    dev_free(ctx, ctx->d_counters, 0);
-    dev_free(ctx, ctx->d_stats, 0);
+    dev_free(ctx, ctx->d_counters, 64 * sizeof(uint32_t));
+    dev_free(ctx, ctx->d_stats, 8 * sizeof(unsigned long long));
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);
     dev_free(ctx, ctx->d_tmp, ctx->tmp_bytes);
     for (auto& ev : ctx->ev)

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
     // ray array, and neighbouring rays share top-level nodes in L2: measured
     // faster than fixed per-wave shares of the array, and than reserving the
     // next block one refill ahead (the reply is waited for by the wave's next
-    // in-order load wait anyway).
+    // in-order load wait anyway); one counter per XCD measured no faster.
     const uint32_t grid_threads = gridDim.x * BLOCK;
     bool first_fetch = true;
     for (;;) {
