@@ -83,6 +83,124 @@ __device__ __forceinline__ uint32_t find_job(const JobDev* jobs, uint32_t num_jo
     return lo;
 }
 
+// ---- light selection adapted to the shading point
+//
+// With at most kAdaptiveLights lights, NEE picks light l with probability
+// 0.9 * w_l / sum(w) + 0.1 * (its power share), where w_l is the light's
+// power times a geometric estimate of how much of it reaches the point:
+// the largest cosine, over the light's four corners, at the receiver's
+// normal (0 when the whole quad lies behind the tangent plane: the cosine
+// numerator is linear over the quad), times the point's distance from the
+// light plane (0 behind a one-sided light) over d^3 to the light's centre
+// (d^2 at least a quarter of the area).  The power floor keeps every light
+// possible, so the estimator stays unbiased.  MIS needs the same probability
+// again when a BSDF-sampled ray hits the light: both evaluations take the
+// offset ray origin and the normal quantised to 16+16-bit octahedral
+// (Cigolle et al., JCGT 3(2), 2014), whose code the path keeps in p_L.w.
+constexpr uint32_t kAdaptiveLights = 8;
+
+__device__ __forceinline__ uint32_t oct_encode(V3 n)
+{
+    const float s = fabsf(n.x) + fabsf(n.y) + fabsf(n.z);
+    float u = n.x / s, v = n.y / s;
+    if (n.z < 0.0f) {
+        const float ou = u;
+        u = (1.0f - fabsf(v)) * (ou >= 0.0f ? 1.0f : -1.0f);
+        v = (1.0f - fabsf(ou)) * (v >= 0.0f ? 1.0f : -1.0f);
+    }
+    const uint32_t qu = uint32_t(fminf(fmaxf(u * 0.5f + 0.5f, 0.0f), 1.0f) * 65535.0f + 0.5f);
+    const uint32_t qv = uint32_t(fminf(fmaxf(v * 0.5f + 0.5f, 0.0f), 1.0f) * 65535.0f + 0.5f);
+    return qu | (qv << 16);
+}
+
+__device__ __forceinline__ V3 oct_decode(uint32_t c)
+{
+    float u = float(c & 0xFFFFu) * (2.0f / 65535.0f) - 1.0f;
+    float v = float(c >> 16) * (2.0f / 65535.0f) - 1.0f;
+    const float z = 1.0f - fabsf(u) - fabsf(v);
+    if (z < 0.0f) {
+        const float ou = u;
+        u = (1.0f - fabsf(v)) * (ou >= 0.0f ? 1.0f : -1.0f);
+        v = (1.0f - fabsf(ou)) * (v >= 0.0f ? 1.0f : -1.0f);
+    }
+    return normalize(v3(u, v, z));
+}
+
+__device__ __forceinline__ float light_weight(const DeviceLight& lt, V3 p, V3 n)
+{
+    const V3 c0 = v3(lt.corner[0], lt.corner[1], lt.corner[2]);
+    const V3 e0 = v3(lt.edge0[0], lt.edge0[1], lt.edge0[2]), e1 = v3(lt.edge1[0], lt.edge1[1], lt.edge1[2]);
+    const V3 nl = normalize(cross(e0, e1));
+    const float side = dot(nl, p - c0);
+    if (!(side > 0.0f)) return 0.0f;
+    float cx = 0.0f;
+    const V3 cs[4] = {c0, c0 + e0, c0 + e1, c0 + e0 + e1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const V3 v = cs[k] - p;
+        const float l2 = dot(v, v);
+        if (l2 > 0.0f) cx = fmaxf(cx, dot(n, v) * rsqrtf(l2));
+    }
+    if (!(cx > 0.0f)) return 0.0f;
+    const V3 v = c0 + (e0 + e1) * 0.5f - p;
+    const float d2 = fmaxf(dot(v, v), 0.25f * lt.area);
+    const float power = (lt.radiance[0] + lt.radiance[1] + lt.radiance[2]) * lt.area;
+    return power * cx * side / (d2 * sqrtf(d2));
+}
+
+// The light NEE takes for uniform u at (p, n), with its selection probability.
+__device__ __forceinline__ uint32_t pick_light(const ShadeScene& sc, V3 p, V3 n, float u, float& pdf)
+{
+    if (sc.num_lights > kAdaptiveLights) {
+        uint32_t li = 0;
+        while (li + 1 < sc.num_lights && u >= sc.lights[li].cdf) ++li;
+        pdf = sc.lights[li].pdf_select;
+        return li;
+    }
+    float w[kAdaptiveLights];
+    float total = 0.0f;
+#pragma unroll
+    for (uint32_t l = 0; l < kAdaptiveLights; ++l) {
+        w[l] = l < sc.num_lights ? light_weight(sc.lights[l], p, n) : 0.0f;
+        total += w[l];
+    }
+    const float inv = total > 0.0f ? 0.9f / total : 0.0f;
+    const float floor_share = total > 0.0f ? 0.1f : 1.0f;
+    float acc = 0.0f;
+    uint32_t pick = sc.num_lights - 1;
+    float pick_pdf = w[0] * inv + floor_share * sc.lights[0].pdf_select;
+    bool found = false;
+#pragma unroll
+    for (uint32_t l = 0; l < kAdaptiveLights; ++l) {
+        if (l < sc.num_lights) {
+            const float pl = w[l] * inv + floor_share * sc.lights[l].pdf_select;
+            acc += pl;
+            if (!found && u < acc) {
+                pick = l;
+                pick_pdf = pl;
+                found = true;
+            }
+            if (!found && l == sc.num_lights - 1) pick_pdf = pl;
+        }
+    }
+    pdf = pick_pdf;
+    return pick;
+}
+
+// Probability that NEE at (p, n) picks light li.
+__device__ __forceinline__ float light_pick_pdf(const ShadeScene& sc, uint32_t li, V3 p, V3 n)
+{
+    if (sc.num_lights > kAdaptiveLights) return sc.lights[li].pdf_select;
+    float total = 0.0f, wl = 0.0f;
+#pragma unroll
+    for (uint32_t l = 0; l < kAdaptiveLights; ++l) {
+        const float w = l < sc.num_lights ? light_weight(sc.lights[l], p, n) : 0.0f;
+        total += w;
+        if (l == li) wl = w;
+    }
+    return total > 0.0f ? 0.9f * wl / total + 0.1f * sc.lights[li].pdf_select : sc.lights[li].pdf_select;
+}
+
 __device__ __forceinline__ float ray_eps(V3 p)
 {
     return 1e-4f * fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
@@ -217,6 +335,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                     if (last[4].w == 0.0f) last[4] = make_float4(L.x, L.y, L.z, 1.0f);
                 }
             }
+            uint32_t ncode = 0;  // this vertex's quantised normal, for the next vertex's MIS
             const uint32_t tri = __float_as_uint(hit.y);
             if (tri == 0xFFFFFFFFu) {
                 L = L + thr * v3(sc.background[0], sc.background[1], sc.background[2]);
@@ -247,7 +366,10 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                         const V3 nl = normalize(cross(v3(lt.edge0[0], lt.edge0[1], lt.edge0[2]),
                                                       v3(lt.edge1[0], lt.edge1[1], lt.edge1[2])));
                         const float cl = fabsf(dot(nl, dir));
-                        const float pdf_l = cl > 0.0f ? lt.pdf_select * t * t / (lt.area * cl) : 0.0f;
+                        // the previous vertex's NEE would have picked this light with:
+                        const float psel = light_pick_pdf(sc, uint32_t(light_id), v3(ray.org[0], ray.org[1], ray.org[2]),
+                                                          oct_decode(__float_as_uint(L4.w)));
+                        const float pdf_l = cl > 0.0f ? psel * t * t / (lt.area * cl) : 0.0f;
                         w = power_heuristic(pdf_prev, pdf_l);
                     }
                     L = L + thr * Le * w;
@@ -259,11 +381,12 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                     const V3 albedo = v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
                     const float eps = ray_eps(p);
                     const V3 org = p + nf * eps;
+                    ncode = oct_encode(nf);
                     // next-event estimation
                     const float ul = rng_float(rng), ua = rng_float(rng), ub = rng_float(rng);
                     if (sc.num_lights > 0) {
-                        uint32_t li = 0;
-                        while (li + 1 < sc.num_lights && ul >= sc.lights[li].cdf) ++li;
+                        float psel;
+                        const uint32_t li = pick_light(sc, org, oct_decode(ncode), ul, psel);
                         const DeviceLight& lt = sc.lights[li];
                         const V3 e0 = v3(lt.edge0[0], lt.edge0[1], lt.edge0[2]);
                         const V3 e1 = v3(lt.edge1[0], lt.edge1[1], lt.edge1[2]);
@@ -274,7 +397,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                         wi = wi * (1.0f / sqrtf(d2));
                         const float cl = -dot(nl, wi);
                         if (cl > 0.0f && d2 > 0.0f) {
-                            const float pdf_l = lt.pdf_select * d2 / (lt.area * cl);
+                            const float pdf_l = psel * d2 / (lt.area * cl);
                             const BsdfEval ev = bsdf_eval(mat.type, albedo, mat.roughness, wo, to_local(frame, wi));
                             if (ev.pdf > 0.0f) {
                                 const float w = power_heuristic(pdf_l, ev.pdf);
@@ -344,7 +467,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                     }
                 }
             }
-            a.b.p_L[path] = make_float4(L.x, L.y, L.z, 0.f);
+            a.b.p_L[path] = make_float4(L.x, L.y, L.z, __uint_as_float(ncode));
         }
         // one block-aggregated append for the three queues
         uint32_t slots[3];
