@@ -191,7 +191,7 @@ fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* d)
     d->learning_rate = 1e-2f;
     d->train_paths = 1u << 16;
     d->train_steps = 2;
-    d->train_depth = 2;
+    d->train_depth = 1;
     d->train_batch = 1u << 14;
     d->glossy_min_roughness = 0.5f;
     d->glossy_any_bounce = 1;
