@@ -78,6 +78,8 @@ fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_
     const uint64_t per = stack_entries > stack_lds ? stack_entries - stack_lds : 0;
     const uint64_t need = std::max<uint64_t>(per * threads * sizeof(uint2), 256);
     if (need <= ctx->spill_bytes) return FND_OK;
+    // an FND_ASYNC trace still in flight may use the old area
+    (void)hipStreamSynchronize(ctx->stream);
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);
     ctx->d_spill = nullptr;
     ctx->spill_bytes = 0;
