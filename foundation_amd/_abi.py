@@ -71,6 +71,25 @@ class QuadLight(C.Structure):
     ]
 
 
+class VertexStream(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p),
+        ("bytes", C.c_uint64),
+        ("stride", C.c_uint32),
+        ("position_offset", C.c_uint32),
+        ("position_components", C.c_uint32),
+        ("num_vertices", C.c_uint32),
+    ]
+
+
+class IndexStream(C.Structure):
+    _fields_ = [
+        ("data", C.c_void_p),
+        ("index_bits", C.c_uint32),
+        ("num_indices", C.c_uint32),
+    ]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [
         ("positions", C.POINTER(C.c_float)),
@@ -278,6 +297,7 @@ PROTOTYPES = {
     "fnd_pt_framebuffer_clear": (_S, [_P, _H]),
     "fnd_pt_framebuffer_read": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_read_f64": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
+    "fnd_pt_framebuffer_read_rgba8": (_S, [_P, _H, C.c_float, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_render": (_S, [_P, C.POINTER(RenderJob), C.c_uint32]),
     "fnd_pt_cache_default_desc": (_S, [C.POINTER(CacheDesc)]),
     "fnd_pt_cache_configure": (_S, [_P, C.POINTER(CacheDesc)]),
@@ -288,6 +308,10 @@ PROTOTYPES = {
     "fnd_pt_cache_gradients": (_S, [_P, _P, _P, C.c_uint32, C.POINTER(C.c_float), _P, _P]),
     "fnd_pt_cache_train": (_S, [_P, _P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
     "fnd_scene_encode": (_S, [C.POINTER(SceneDesc), C.POINTER(_P), C.c_char_p, C.c_size_t]),
+    "fnd_mesh_from_streams": (_S, [C.POINTER(VertexStream), C.POINTER(IndexStream), _P, _P, C.c_char_p, C.c_size_t]),
+    "fnd_obj_load": (_S, [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_size_t]),
+    "fnd_obj_mesh_get": (_S, [_P, C.POINTER(MeshDesc)]),
+    "fnd_obj_free": (None, [_P]),
     "fnd_host_scene_get": (_S, [_P, C.POINTER(HostSceneView)]),
     "fnd_host_scene_free": (None, [_P]),
     "fnd_gen_default_params": (_S, [C.c_uint32, C.POINTER(GenParams)]),

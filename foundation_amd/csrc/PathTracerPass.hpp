@@ -180,22 +180,13 @@ public:
         check(fnd_pt_framebuffer_read_f64(ctx_, fb_, rgb.data(), &n), "fnd_pt_framebuffer_read_f64");
     }
     // R8G8B8A8_UNORM, the reference's presentation format
-    // (src/Renderer/Renderer.cpp:39-44): clamp(x, 0, 1)^(1/2.2) * 255,
-    // rounded to nearest, alpha 255.
-    void ReadRGBA8(std::vector<uint8_t>& rgba)
+    // (src/Renderer/Renderer.cpp:39-44), converted on the device:
+    // round(255 * clamp(x, 0, 1)^(1/2.2)), alpha 255 (fnd_pt_framebuffer_read_rgba8).
+    void ReadRGBA8(std::vector<uint8_t>& rgba, float exposure = 1.0f)
     {
-        std::vector<float> rgb;
-        ReadRGB(rgb);
-        const size_t n = size_t(desc_.width) * desc_.height;
-        rgba.resize(n * 4);
-        for (size_t i = 0; i < n; ++i) {
-            for (int c = 0; c < 3; ++c) {
-                float x = rgb[3 * i + c];
-                x = x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f;
-                rgba[4 * i + c] = uint8_t(std::pow(x, 1.0f / 2.2f) * 255.0f + 0.5f);
-            }
-            rgba[4 * i + 3] = 255;
-        }
+        rgba.resize(size_t(desc_.width) * desc_.height * 4);
+        uint32_t n = 0;
+        check(fnd_pt_framebuffer_read_rgba8(ctx_, fb_, exposure, rgba.data(), &n), "fnd_pt_framebuffer_read_rgba8");
     }
 
     uint32_t Samples() const { return samples_; }

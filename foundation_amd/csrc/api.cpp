@@ -168,10 +168,6 @@ void host_free(const fnd_host_allocator* a, void* p)
         std::free(p);
 }
 
-}  // namespace fnd
-
-namespace {
-
 // Copies on the context stream and waits: the stream is non-blocking, so a
 // plain hipMemcpy on the null stream would not order against its kernels.
 fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipMemcpyKind kind, const char* what)
@@ -181,6 +177,11 @@ fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes
     if (st != FND_OK) return st;
     return check_hip(ctx, hipStreamSynchronize(ctx->stream), what);
 }
+
+}  // namespace fnd
+
+namespace {
+
 
 void default_tuning(fnd_tuning& t)
 {
@@ -304,12 +305,14 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
         Framebuffer* f = kv.second;
         dev_free(ctx, f->accum, 0);
         dev_free(ctx, f->accum64, 0);
+        dev_free(ctx, f->rgba8, 0);
         ctx_delete(ctx, f);
     }
     ctx->framebuffers.clear();
     render_state_free(ctx);
     cache_free(ctx);
     dev_free(ctx, ctx->d_counters, 64 * sizeof(uint32_t));
+    dev_free(ctx, ctx->d_present, 255 * sizeof(float));
     dev_free(ctx, ctx->d_stats, 8 * sizeof(unsigned long long));
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);
     dev_free(ctx, ctx->d_tmp, ctx->tmp_bytes);

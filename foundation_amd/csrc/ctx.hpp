@@ -44,6 +44,7 @@ struct Framebuffer {
     uint32_t width = 0, height = 0, flags = 0;
     float* accum = nullptr;       // W*H*3 running sums (FP32)
     double* accum64 = nullptr;    // W*H*3 running sums (FP64, oracle images only)
+    uint32_t* rgba8 = nullptr;    // W*H packed R8G8B8A8 (made by the first RGBA8 read)
     uint32_t samples = 0;
 };
 
@@ -71,6 +72,7 @@ struct fnd_pt_ctx {
     std::map<uint64_t, fnd::Buffer*> buffers;
     std::map<uint64_t, fnd::Framebuffer*> framebuffers;
     // scratch owned by the context
+    float* d_present = nullptr;                // 255 byte thresholds of the RGBA8 transfer
     uint32_t* d_counters = nullptr;            // 64 work counters
     unsigned long long* d_stats = nullptr;     // 8 counters
     uint2* d_spill = nullptr;
@@ -86,6 +88,8 @@ namespace fnd {
 fnd_status set_error(fnd_pt_ctx* ctx, fnd_status st, const char* fmt, ...);
 fnd_status check_hip(fnd_pt_ctx* ctx, hipError_t e, const char* what);
 fnd_status dev_alloc(fnd_pt_ctx* ctx, void** ptr, uint64_t bytes);
+// hipMemcpyAsync on the context's stream, then wait for it.
+fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipMemcpyKind kind, const char* what);
 void dev_free(fnd_pt_ctx* ctx, void* ptr, uint64_t bytes);
 fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_lds, uint32_t threads);
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);

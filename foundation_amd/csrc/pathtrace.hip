@@ -623,6 +623,38 @@ void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hi
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, g0, n);
 }
 
+// One pixel per thread: mean = sum * inv (the same product as the FP32
+// readback), times exposure; each channel's byte is the number of the 255
+// ascending thresholds at or below it (binary search; NaN and x <= 0 give 0).
+__global__ __launch_bounds__(256) void k_present(const float* accum, uint64_t pixels, float inv, float exposure,
+                                                 const float* thresholds, uint32_t* out)
+{
+    __shared__ float t[256];
+    for (uint32_t k = threadIdx.x; k < 256; k += blockDim.x) t[k] = k < 255 ? thresholds[k] : __builtin_inff();
+    __syncthreads();
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < pixels; i += uint64_t(gridDim.x) * blockDim.x) {
+        uint32_t word = 0xFF000000u;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            float x = accum[3 * i + c] * inv;
+            if (exposure != 1.0f) x *= exposure;
+            uint32_t b = 0;  // thresholds t[0..254] ascending; count those <= x
+#pragma unroll
+            for (uint32_t step = 128; step > 0; step >>= 1)
+                if (t[b + step - 1] <= x) b += step;
+            word |= b << (8 * c);
+        }
+        out[i] = word;
+    }
+}
+
+void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
+                    uint32_t* out, hipStream_t s)
+{
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (pixels + 255) / 256)));
+    hipLaunchKernelGGL(k_present, dim3(grid), dim3(256), 0, s, accum, pixels, inv, exposure, thresholds, out);
+}
+
 void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s)
 {
     hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, next);

@@ -239,6 +239,7 @@ fnd_status fnd_pt_framebuffer_destroy(fnd_pt_ctx* ctx, fnd_handle h)
     const uint64_t n = uint64_t(f->width) * f->height * 3;
     dev_free(ctx, f->accum, n * sizeof(float));
     dev_free(ctx, f->accum64, n * sizeof(double));
+    dev_free(ctx, f->rgba8, n / 3 * sizeof(uint32_t));
     ctx_delete(ctx, f);
     ctx->framebuffers.erase(h);
     return FND_OK;
@@ -272,6 +273,45 @@ fnd_status fnd_pt_framebuffer_read(fnd_pt_ctx* ctx, fnd_handle h, float* rgb, ui
         const float inv = 1.0f / float(f->samples);
         for (uint64_t i = 0; i < n; ++i) rgb[i] *= inv;
     }
+    if (out_samples) *out_samples = f->samples;
+    return FND_OK;
+}
+
+// Present path: the reference presents R8G8B8A8_UNORM at 1920x1080
+// (src/Renderer/Renderer.cpp:39-44).  Byte b of a channel is the number of
+// thresholds ((k - 0.5) / 255)^2.2, k = 1..255, at or below x = clamp(mean *
+// exposure), i.e. round(255 * x^(1/2.2)) with the boundaries computed once
+// on the host in double precision, so any host can reproduce the bytes.
+fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle h, float exposure, uint8_t* rgba,
+                                         uint32_t* out_samples)
+{
+    if (!ctx || !rgba || !(exposure >= 0.0f)) return FND_E_INVALID_ARG;
+    Framebuffer* f = find_framebuffer(ctx, h);
+    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+    const uint64_t pixels = uint64_t(f->width) * f->height;
+    fnd_status st = FND_OK;
+    if (!ctx->d_present) {
+        float t[255];
+        for (int k = 1; k <= 255; ++k) t[k - 1] = float(std::pow((double(k) - 0.5) / 255.0, 2.2));
+        void* p = nullptr;
+        if ((st = dev_alloc(ctx, &p, sizeof t)) != FND_OK) return st;
+        ctx->d_present = static_cast<float*>(p);
+        st = copy_sync(ctx, ctx->d_present, t, sizeof t, hipMemcpyHostToDevice, "present table");
+        if (st != FND_OK) return st;
+    }
+    if (!f->rgba8) {
+        void* p = nullptr;
+        if ((st = dev_alloc(ctx, &p, pixels * sizeof(uint32_t))) != FND_OK) return st;
+        f->rgba8 = static_cast<uint32_t*>(p);
+    }
+    const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
+    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, f->rgba8, ctx->stream);
+    st = check_hip(ctx, hipGetLastError(), "present");
+    if (st == FND_OK)
+        st = check_hip(ctx, hipMemcpyAsync(rgba, f->rgba8, pixels * 4, hipMemcpyDeviceToHost, ctx->stream),
+                       "read framebuffer");
+    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
+    if (st != FND_OK) return st;
     if (out_samples) *out_samples = f->samples;
     return FND_OK;
 }

@@ -120,6 +120,8 @@ struct WfArgs {
 };
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
+void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
+                    uint32_t* out, hipStream_t s);
 void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s);
 void launch_shade(const WfArgs& a, uint32_t cur, uint32_t bounce, uint32_t grid, hipStream_t s);
 void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);

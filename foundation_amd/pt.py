@@ -149,6 +149,62 @@ INSTANCE_DTYPE = np.dtype([("w2o", "<f4", 12), ("blas_root", "<u4"), ("id_base",
 assert INSTANCE_DTYPE.itemsize == 64
 
 
+# ---------------------------------------------------------------- scene ingestion
+
+
+def mesh_from_streams(vertex_bytes, stride, position_offset, position_components, num_vertices, indices):
+    """fnd_mesh_from_streams over a reference-style interleaved vertex buffer
+    (bytes, stride, byte offset of the position, 2 or 3 floats) and a uint16
+    or uint32 index array: returns positions (V, 3) f32 and indices (T, 3) u32."""
+    lib = A.load()
+    vb = np.frombuffer(bytes(vertex_bytes), np.uint8)
+    ib = np.ascontiguousarray(indices).reshape(-1)
+    if ib.dtype not in (np.uint16, np.uint32):
+        raise TypeError("indices must be uint16 or uint32")
+    vs = A.VertexStream(vb.ctypes.data, vb.size, stride, position_offset, position_components, num_vertices)
+    ist = A.IndexStream(ib.ctypes.data, 8 * ib.dtype.itemsize, ib.size)
+    pos = np.zeros((num_vertices, 3), np.float32)
+    idx = np.zeros((ib.size // 3, 3), np.uint32)
+    err = C.create_string_buffer(256)
+    st = lib.fnd_mesh_from_streams(C.byref(vs), C.byref(ist), pos.ctypes.data, idx.ctypes.data, err, 256)
+    if st != A.OK:
+        raise FndError(st, err.value.decode())
+    return pos, idx
+
+
+class ObjMesh:
+    """A Wavefront OBJ mesh loaded by fnd_obj_load; ``desc`` is its
+    fnd_mesh_desc (valid while the object lives)."""
+
+    def __init__(self, path):
+        lib = A.load()
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        st = lib.fnd_obj_load(str(path).encode(), C.byref(h), err, 512)
+        if st != A.OK:
+            raise FndError(st, err.value.decode())
+        self._h = h
+        self.desc = A.MeshDesc()
+        lib.fnd_obj_mesh_get(h, C.byref(self.desc))
+
+    def arrays(self):
+        d = self.desc
+        pos = np.ctypeslib.as_array(d.positions, shape=(d.num_vertices * 3,)).reshape(-1, 3).copy()
+        idx = np.ctypeslib.as_array(d.indices, shape=(d.num_triangles * 3,)).reshape(-1, 3).copy()
+        return pos, idx
+
+    def close(self):
+        if self._h:
+            A.load().fnd_obj_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def encode_scene(desc):
     """Host encoding of a scene (fnd_scene_encode): dict of numpy arrays with
     the exact bytes fnd_pt_scene_create uploads."""
@@ -300,6 +356,13 @@ class Context:
         n = C.c_uint32()
         fn = self.lib.fnd_pt_framebuffer_read_f64 if fp64 else self.lib.fnd_pt_framebuffer_read
         self._ck(fn(self.h, fb, out.ctypes.data, C.byref(n)))
+        return out, n.value
+
+    def framebuffer_read_rgba8(self, fb, width, height, exposure=1.0):
+        """R8G8B8A8 bytes of the mean radiance, converted on the device."""
+        out = np.zeros((height, width, 4), np.uint8)
+        n = C.c_uint32()
+        self._ck(self.lib.fnd_pt_framebuffer_read_rgba8(self.h, fb, exposure, out.ctypes.data, C.byref(n)))
         return out, n.value
 
     def render(self, jobs):

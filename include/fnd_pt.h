@@ -286,6 +286,13 @@ fnd_status fnd_pt_framebuffer_clear(fnd_pt_ctx* ctx, fnd_handle fb);
 /* Mean radiance per pixel, RGB, row 0 at the top: width*height*3 values. */
 fnd_status fnd_pt_framebuffer_read(fnd_pt_ctx* ctx, fnd_handle fb, float* rgb, uint32_t* out_samples);
 fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle fb, double* rgb, uint32_t* out_samples);
+/* The reference's presentation format, R8G8B8A8_UNORM (src/Renderer/Renderer.cpp:39-44),
+ * converted on the device: width*height*4 bytes, row 0 at the top, alpha 255.
+ * x = mean * exposure; byte = number of thresholds ((k - 0.5) / 255)^2.2,
+ * k = 1..255, at or below x (that is round(255 * clamp(x, 0, 1)^(1/2.2)), with
+ * the thresholds rounded from double to float once); NaN and x <= 0 give 0. */
+fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle fb, float exposure, uint8_t* rgba,
+                                         uint32_t* out_samples);
 /* Renders all jobs in one batched wavefront launch sequence. */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
 
@@ -398,6 +405,45 @@ typedef struct fnd_host_scene_view {
 fnd_status fnd_scene_encode(const fnd_scene_desc* desc, fnd_host_scene** out, char* err, size_t err_len);
 fnd_status fnd_host_scene_get(const fnd_host_scene* scene, fnd_host_scene_view* out);
 void fnd_host_scene_free(fnd_host_scene* scene);
+
+/* ---------------------------------------------------------------- scene ingestion (host only) */
+
+/* Geometry as the reference hands it to the GPU: an interleaved vertex
+ * buffer whose position attribute is described like a Vulkan vertex input
+ * (stride, byte offset, 2 or 3 floats; z = 0 for 2, as in the reference's
+ * {vec2 pos, vec3 color, vec2 texCoord}, src/Renderer/Renderer.cpp:23-27),
+ * and an index buffer of R16_UINT or R32_UINT (src/Platform/RHI/Vulkan/
+ * Command.cpp:289-305). */
+typedef struct fnd_vertex_stream {
+    const void* data;
+    uint64_t bytes;                /* size of data */
+    uint32_t stride;               /* bytes from one vertex to the next */
+    uint32_t position_offset;      /* byte offset of the position in a vertex */
+    uint32_t position_components;  /* 2 or 3 floats */
+    uint32_t num_vertices;
+} fnd_vertex_stream;
+
+typedef struct fnd_index_stream {
+    const void* data;
+    uint32_t index_bits;           /* 16 or 32 */
+    uint32_t num_indices;          /* a multiple of 3 (triangle list) */
+} fnd_index_stream;
+
+/* Writes num_vertices * 3 floats and num_indices u32 indices, the arrays an
+ * fnd_mesh_desc points at.  Fails (no partial success) on an attribute past
+ * the stride, a buffer too short for num_vertices, or an index out of range. */
+fnd_status fnd_mesh_from_streams(const fnd_vertex_stream* vertices, const fnd_index_stream* indices,
+                                 float* positions, uint32_t* indices_out, char* err, size_t err_len);
+
+/* Wavefront OBJ: `v x y z` positions and `f` faces (1-based or negative
+ * references, `a/b/c` forms accepted, polygons split into fans around their
+ * first vertex); other statements are ignored.  The mesh owns its arrays
+ * until fnd_obj_free; fnd_obj_mesh_get fills positions, indices and counts
+ * of an fnd_mesh_desc (material and light ids NULL). */
+typedef struct fnd_obj_mesh fnd_obj_mesh;
+fnd_status fnd_obj_load(const char* path, fnd_obj_mesh** out, char* err, size_t err_len);
+fnd_status fnd_obj_mesh_get(const fnd_obj_mesh* mesh, fnd_mesh_desc* out);
+void fnd_obj_free(fnd_obj_mesh* mesh);
 
 /* ---------------------------------------------------------------- scene generators */
 

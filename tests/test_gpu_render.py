@@ -110,6 +110,33 @@ def test_four_lights_and_ggx_match_oracle(ctx, oracle):
     assert abs(gpu1.mean() - ref.mean()) < 0.01 * ref.mean()
 
 
+def test_rgba8_present_matches_the_declared_transfer(ctx, cornell):
+    """fnd_pt_framebuffer_read_rgba8 counts, per channel, the float
+    thresholds ((k - 0.5) / 255)^2.2 at or below mean * exposure: the same
+    count taken here from the FP32 readback must give the same bytes, and
+    those are round(255 * clamp(x, 0, 1)^(1/2.2)) within one step."""
+    g, s = cornell
+    w, h = 64, 48
+    fb = ctx.framebuffer_create(w, h)
+    try:
+        ctx.render([pt.make_job(s, fb, g.camera, 8, 4, seed=3)])
+        rgb, n = ctx.framebuffer_read(fb, w, h)
+        thresholds = (((np.arange(1, 256) - 0.5) / 255.0) ** 2.2).astype(np.float32)
+        for exposure in (1.0, 4.0):
+            rgba, n2 = ctx.framebuffer_read_rgba8(fb, w, h, exposure)
+            assert n2 == n == 8
+            x = rgb if exposure == 1.0 else rgb * np.float32(exposure)
+            expect = np.searchsorted(thresholds, x, side="right")
+            expect[np.isnan(x)] = 0
+            assert np.array_equal(rgba[..., :3], expect), np.argwhere(rgba[..., :3] != expect)[:5]
+            assert np.all(rgba[..., 3] == 255)
+            textbook = np.round(255.0 * np.clip(x.astype(np.float64), 0, 1) ** (1 / 2.2))
+            assert np.abs(rgba[..., :3].astype(np.int64) - textbook).max() <= 1
+            assert 0 < rgba[..., :3].mean() < 255
+    finally:
+        ctx.framebuffer_destroy(fb)
+
+
 def test_images_do_not_depend_on_queue_or_tiles(ctx, cornell):
     g, s = cornell
     base, _ = render(ctx, s, g.camera, 40, 24, 4, 4, seed=3)
