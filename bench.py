@@ -139,6 +139,35 @@ def batch64(ctx, count, spp, depth, cache_path):
         ctx.set_tuning(block_size=0, stack_lds=0, waves_per_cu=0, refill=0, queue_capacity=0, tile=0)
 
 
+def frames_only(frames, width, height, depth, rr):
+    """The frames of the time-to-relMSE leg without its reference: `frames`
+    plain frames, then `frames` frames with the radiance cache, 1 spp each;
+    prints the mean device ms per frame of each (for kernel-trace profiles)."""
+    import ctypes as C
+    from foundation_amd import _abi as A
+    from foundation_amd import pt
+    ctx = pt.Context(0)
+    gen = pt.GeneratedScene(A.GEN_DISPLACED_SPHERE)
+    scene = ctx.scene_create(gen.desc)
+    fb = ctx.framebuffer_create(width, height)
+    for mode in ("plain", "cache"):
+        flags = 0
+        if mode == "cache":
+            d = A.CacheDesc()
+            ctx.lib.fnd_pt_cache_default_desc(C.byref(d))
+            ctx._ck(ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d)))
+            flags = A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE
+        ctx.framebuffer_clear(fb)
+        ms = 0.0
+        for f in range(frames):
+            ctx.render([pt.make_job(scene, fb, gen.camera, 1, depth, rr_depth=rr, seed=5, sample_offset=f, flags=flags)])
+            ms += ctx.stats().last_call_ms
+        log(f"{mode}: {ms / frames:.3f} ms per frame over {frames} frames")
+    ctx.framebuffer_destroy(fb)
+    ctx.scene_destroy(scene)
+    ctx.close()
+
+
 def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
     """Progressive 1 spp/frame rendering of the displaced-sphere scene, plain and
     with the radiance cache (trained every frame on the frame's own paths),
@@ -217,6 +246,9 @@ def main():
     ap.add_argument("--pmc-summary", default=os.path.join(ROOT, "profiles", "trace_incoherent_10M_pmc.json"))
     ap.add_argument("--only-incoherent", action="store_true",
                     help="profiling runs: time only the incoherent set (no coherent/shadow sets, no CPU leg)")
+    ap.add_argument("--frames-only", type=int, default=0, metavar="N",
+                    help="profiling runs: render N plain then N cache frames of the relMSE scene (no reference, "
+                         "no traversal legs), print ms per frame, exit")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="profiling runs: first copy this many MiB device-to-device (a plain copy kernel of "
                          "known size that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -230,6 +262,10 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    if args.frames_only:
+        frames_only(args.frames_only, args.relmse_width, args.relmse_height, 8, 3)
+        return
 
     if args.calibrate_copy_mib:
         import torch
