@@ -21,7 +21,11 @@ displaced-sphere scene (1.31M-triangle icosphere + 200k soup, 4 quad lights,
 1920x1080, depth 8, RR from 3) until relMSE <= 0.01 against a 4096-spp FP64
 GPU reference, plain and with the radiance cache trained every frame; device
 seconds of the frames are summed.  extra.path_tracing_10M is samples/s of
-full path tracing on the 10M-triangle scene.  --no-render skips both legs.
+full path tracing on the 10M-triangle scene.  extra.frame_arena_10M: one
+1-spp 1080p frame of that scene with its per-frame ray queues carved from the
+library's own block and from caller device memory of 1024/256/64/16 MiB
+(smaller blocks run smaller batches): ms per frame, batch, and bitwise
+equality of the images.  --no-render skips these legs.
 extra.batch_64_scenes: 64 scenes of the batch generator (triangles
 log-uniform in [10k, 2M], images 256/512/1024 square) rendered at 16 spp,
 depth 6, by one fnd_pt_render call with launch knobs from the autotuner
@@ -89,6 +93,50 @@ def render_throughput(ctx, scene, camera, width, height, spp, depth, rr):
     return {"workload": f"{width}x{height}, {spp} spp, depth {depth}, RR from {rr}, NEE+MIS on 4 quad lights",
             "msamples_s": round(width * height * spp / ms / 1e3, 2), "ms_per_spp": round(ms / spp, 3),
             "render_mrays_s": round(st.render_rays / ms / 1e3, 1), "wall_ms": round(wall * 1e3, 1)}
+
+
+def frame_arena_sweep(ctx, scene, camera, width, height, depth, rr, frames=3):
+    """The same 1-spp frame with its per-frame ray queues carved from the
+    library's own block, then from caller device memory of decreasing size
+    (a smaller block runs smaller wavefront batches): device ms per frame
+    (median of `frames`), the batch, and whether the image equals the library
+    block's bit for bit."""
+    import numpy as np
+    from foundation_amd import pt
+    fb = ctx.framebuffer_create(width, height)
+    rows, ref = [], None
+    try:
+        for mib in (None, 1024, 256, 64, 16):
+            mem = None
+            if mib is None:
+                ctx.set_frame_arena(0, 0)
+            else:
+                mem = pt.DeviceMemory(mib << 20)
+                ctx.set_frame_arena(mem.ptr, mib << 20)
+            times = []
+            for k in range(frames + 1):  # the first call may grow the library's block
+                ctx.framebuffer_clear(fb)
+                ctx.render([pt.make_job(scene, fb, camera, 1, depth, rr_depth=rr, seed=1)])
+                if k:
+                    times.append(ctx.stats().last_call_ms)
+            img = ctx.framebuffer_read(fb, width, height)[0]
+            if ref is None:
+                ref = img
+            info = ctx.frame_arena_info()
+            ms = sorted(times)[len(times) // 2]
+            rows.append({"block": "library" if mib is None else f"caller {mib} MiB",
+                         "block_mib": round(info.size / 2**20, 1), "used_mib": round(info.used / 2**20, 1),
+                         "batch_paths": int(info.batch_paths), "ms_per_frame": round(ms, 3),
+                         "msamples_s": round(width * height / ms / 1e3, 1),
+                         "same_image": bool(np.array_equal(img.view(np.uint32), ref.view(np.uint32)))})
+            ctx.set_frame_arena(0, 0)
+            if mem is not None:
+                mem.free()
+    finally:
+        ctx.set_frame_arena(0, 0)
+        ctx.framebuffer_destroy(fb)
+    return {"workload": f"{width}x{height}, 1 spp, depth {depth}, RR from {rr}, 10.08M tris",
+            "queue_bytes_per_path": int(pt.queue_bytes(1 << 16) - pt.queue_bytes(1 << 15)) >> 15, "rows": rows}
 
 
 def batch64(ctx, count, spp, depth, cache_path):
@@ -357,10 +405,12 @@ def main():
                   "shadow_rays": int(shadow.shape[0]), "anyhit_mismatches": int((socc != oocc).sum())}
         log(f"cpu: {cpu['value']} Mrays/s on {threads} threads; parity {parity}")
 
-    render = relmse = batch = None
+    render = relmse = batch = arena = None
     if not args.no_render and not args.only_incoherent:
         render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 4, 8, 3)
         log(f"path tracing @10M tris: {render}")
+        arena = frame_arena_sweep(ctx, scene, gen.camera, args.width, args.height, 8, 3)
+        log(f"frame arena: {arena}")
         ctx.scene_destroy(scene)  # free the 10M-triangle scene before the next one
         scene = None
         relmse = time_to_rel_mse(ctx, args.relmse_width, args.relmse_height, 8, 3, 0.01, args.relmse_ref_spp, 1500)
@@ -400,7 +450,8 @@ def main():
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
                   "trace_valu_issue": valu,
                   "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
-                  "path_tracing_10M": render, "time_to_rel_mse_0_01": relmse, "batch_64_scenes": batch},
+                  "path_tracing_10M": render, "frame_arena_10M": arena, "time_to_rel_mse_0_01": relmse,
+                  "batch_64_scenes": batch},
     }
     print(json.dumps(line), flush=True)
     ctx.close()

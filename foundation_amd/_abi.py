@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ABI_VERSION = 4  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
+ABI_VERSION = 5  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
 LIB_PATH = os.path.join(_HERE, "libfnd_pt.so")
 
 fnd_handle = C.c_uint64
@@ -87,6 +87,23 @@ class IndexStream(C.Structure):
         ("data", C.c_void_p),
         ("index_bits", C.c_uint32),
         ("num_indices", C.c_uint32),
+    ]
+
+
+class Arena(C.Structure):
+    _fields_ = [
+        ("memory", C.c_uint64),
+        ("end", C.c_uint64),
+        ("current", C.c_uint64),
+    ]
+
+
+class FrameArenaInfo(C.Structure):
+    _fields_ = [
+        ("size", C.c_uint64),
+        ("used", C.c_uint64),
+        ("batch_paths", C.c_uint32),
+        ("caller_owned", C.c_uint32),
     ]
 
 
@@ -299,6 +316,13 @@ PROTOTYPES = {
     "fnd_pt_framebuffer_read_f64": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_read_rgba8": (_S, [_P, _H, C.c_float, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_render": (_S, [_P, C.POINTER(RenderJob), C.c_uint32]),
+    "fnd_arena_reset": (None, [C.POINTER(Arena), _P, C.c_uint64]),
+    "fnd_arena_alloc": (_P, [C.POINTER(Arena), C.c_uint64]),
+    "fnd_arena_alloc_aligned": (_P, [C.POINTER(Arena), C.c_uint64, C.c_uint64]),
+    "fnd_arena_used": (C.c_uint64, [C.POINTER(Arena)]),
+    "fnd_pt_queue_bytes": (_S, [C.c_uint32, C.POINTER(C.c_uint64)]),
+    "fnd_pt_set_frame_arena": (_S, [_P, _P, C.c_uint64]),
+    "fnd_pt_frame_arena_info": (_S, [_P, C.POINTER(FrameArenaInfo)]),
     "fnd_pt_cache_default_desc": (_S, [C.POINTER(CacheDesc)]),
     "fnd_pt_cache_configure": (_S, [_P, C.POINTER(CacheDesc)]),
     "fnd_pt_cache_param_count": (_S, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),

@@ -70,6 +70,29 @@ fnd_host_allocator make_host_allocator(Alloc* a)
     return h;
 }
 
+// Core::StackAllocator's interface (src/Core/Allocator/StackAllocator.hpp:9-63)
+// over fnd_arena: bump allocation from one block, no frees, Reset per frame.
+// The addresses are computed only, so the block may be device memory.
+class StackAllocator {
+public:
+    StackAllocator() { fnd_arena_reset(&arena_, nullptr, 0); }
+    StackAllocator(void* memory, uint64_t size) { Reset(memory, size); }
+
+    void Reset(void* memory, uint64_t size) { fnd_arena_reset(&arena_, memory, size); }
+    void Reset() { fnd_arena_reset(&arena_, nullptr, 0); }
+    // nullptr for size 0, without memory, or when the block would pass the end
+    void* Allocate(size_t size) { return fnd_arena_alloc(&arena_, size); }
+    void* Allocate(size_t size, size_t alignment) { return fnd_arena_alloc_aligned(&arena_, size, alignment); }
+    void Deallocate(void*) {}
+    void Deallocate(void*, size_t) {}
+    void* Reallocate(void*, size_t, size_t) { throw std::runtime_error("StackAllocator does not support reallocation"); }
+    uint64_t GetUsedMemory() const noexcept { return fnd_arena_used(&arena_); }
+    explicit operator bool() const noexcept { return arena_.memory != 0; }
+
+private:
+    fnd_arena arena_{};
+};
+
 class PathTracerPass {
 public:
     struct Desc {
@@ -187,6 +210,26 @@ public:
         rgba.resize(size_t(desc_.width) * desc_.height * 4);
         uint32_t n = 0;
         check(fnd_pt_framebuffer_read_rgba8(ctx_, fb_, exposure, rgba.data(), &n), "fnd_pt_framebuffer_read_rgba8");
+    }
+
+    // Per-frame ray queues from the caller's device memory (256-byte aligned,
+    // kept alive by the caller), or from the library's own block again with
+    // nullptr (fnd_pt_set_frame_arena).  QueueBytes(n) is the size a batch of
+    // n paths takes; a smaller block runs smaller batches.
+    void SetFrameArena(void* device_memory, uint64_t bytes)
+    {
+        check(fnd_pt_set_frame_arena(ctx_, device_memory, bytes), "fnd_pt_set_frame_arena");
+    }
+    fnd_frame_arena_info FrameArena()
+    {
+        fnd_frame_arena_info i{};
+        check(fnd_pt_frame_arena_info(ctx_, &i), "fnd_pt_frame_arena_info");
+        return i;
+    }
+    static uint64_t QueueBytes(uint32_t paths)
+    {
+        uint64_t b = 0;
+        return fnd_pt_queue_bytes(paths, &b) == FND_OK ? b : 0;
     }
 
     uint32_t Samples() const { return samples_; }

@@ -79,6 +79,8 @@ struct fnd_pt_ctx {
     uint64_t spill_bytes = 0;
     void* d_tmp = nullptr;                     // host-array trace staging
     uint64_t tmp_bytes = 0;
+    void* frame_memory = nullptr;              // caller's block for the per-frame queues, or null
+    uint64_t frame_bytes = 0;
     void* render_state = nullptr;              // wavefront buffers (render.cpp)
     void* cache = nullptr;                     // radiance cache (cache_api.cpp)
 };

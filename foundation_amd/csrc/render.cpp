@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <vector>
 
 #include "cache_state.hpp"
@@ -27,9 +28,11 @@ using namespace fnd;
 namespace fnd {
 
 struct RenderState {
-    uint32_t capacity = 0;
     WfBuffers b{};
-    std::vector<std::pair<void*, uint64_t>> allocs;
+    fnd_arena frame{};        // reset at every render call, the queues are carved from it
+    void* owned = nullptr;    // the library's block (when the caller gives none)
+    uint64_t owned_bytes = 0;
+    uint32_t batch = 0;       // paths per batch of the last call
     JobDev* d_jobs = nullptr;
     uint32_t jobs_cap = 0;
     ShadeScene* d_shade = nullptr;
@@ -41,7 +44,7 @@ void render_state_free(fnd_pt_ctx* ctx)
 {
     auto* rs = static_cast<RenderState*>(ctx->render_state);
     if (!rs) return;
-    for (auto& a : rs->allocs) dev_free(ctx, a.first, a.second);
+    dev_free(ctx, rs->owned, rs->owned_bytes);
     dev_free(ctx, rs->d_jobs, rs->jobs_cap * sizeof(JobDev));
     dev_free(ctx, rs->d_shade, rs->scenes_cap * sizeof(ShadeScene));
     dev_free(ctx, rs->d_scenes, rs->scenes_cap * sizeof(SceneDev));
@@ -49,44 +52,20 @@ void render_state_free(fnd_pt_ctx* ctx)
     ctx->render_state = nullptr;
 }
 
-namespace {
+// Alignment of every queue in the frame arena (hipMalloc's own is at least
+// this, so a carved buffer starts on a cache line like an allocated one).
+constexpr uint64_t kQueueAlign = 256;
 
-template <class T>
-fnd_status alloc_into(fnd_pt_ctx* ctx, RenderState* rs, T** out, uint64_t count)
+// Bump-allocates the buffers of one wavefront batch of `cap` paths from `ar`.
+// False when the arena runs out (the pointers are then partly null).
+static bool carve_queues(fnd_arena* ar, uint64_t cap, WfBuffers& b)
 {
-    void* p = nullptr;
-    const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
-    fnd_status st = dev_alloc(ctx, &p, bytes);
-    if (st != FND_OK) return st;
-    rs->allocs.emplace_back(p, bytes);
-    *out = static_cast<T*>(p);
-    return FND_OK;
-}
-
-fnd_status ensure_capacity(fnd_pt_ctx* ctx, uint32_t cap)
-{
-    auto* rs = static_cast<RenderState*>(ctx->render_state);
-    if (rs && rs->capacity >= cap) return FND_OK;
-    uint32_t jobs_cap = 0, scenes_cap = 0;
-    JobDev* dj = nullptr;
-    ShadeScene* ds = nullptr;
-    SceneDev* dsc = nullptr;
-    if (rs) {  // keep the small tables, drop the per-path buffers
-        for (auto& a : rs->allocs) dev_free(ctx, a.first, a.second);
-        rs->allocs.clear();
-        jobs_cap = rs->jobs_cap; scenes_cap = rs->scenes_cap;
-        dj = rs->d_jobs; ds = rs->d_shade; dsc = rs->d_scenes;
-        ctx_delete(ctx, rs);
-    }
-    rs = ctx_new<RenderState>(ctx);
-    if (!rs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
-    rs->jobs_cap = jobs_cap; rs->scenes_cap = scenes_cap;
-    rs->d_jobs = dj; rs->d_shade = ds; rs->d_scenes = dsc;
-    ctx->render_state = rs;
-    WfBuffers& b = rs->b;
-    fnd_status st = FND_OK;
+    bool ok = true;
     auto A = [&](auto** p, uint64_t n) {
-        if (st == FND_OK) st = alloc_into(ctx, rs, p, n);
+        using T = std::remove_reference_t<decltype(**p)>;
+        void* q = fnd_arena_alloc_aligned(ar, std::max<uint64_t>(n, 1) * sizeof(T), kQueueAlign);
+        ok = ok && q != nullptr;
+        *p = static_cast<T*>(q);
     };
     A(&b.p_job, cap); A(&b.p_pixel, cap); A(&b.p_sample, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
     A(&b.p_train, cap);
@@ -98,13 +77,73 @@ fnd_status ensure_capacity(fnd_pt_ctx* ctx, uint32_t cap)
     A(&b.counters, kCntCount);
     A(&b.rays, 1);
     A(&b.q_query, cap); A(&b.q_path, cap); A(&b.q_weight, cap); A(&b.q_pred, cap);
-    if (st != FND_OK) {
-        render_state_free(ctx);
-        return st;
+    return ok;
+}
+
+// Bytes carve_queues takes for `cap` paths from a kQueueAlign-aligned block.
+static uint64_t queue_bytes(uint64_t cap)
+{
+    fnd_arena m{};
+    fnd_arena_reset(&m, reinterpret_cast<void*>(uintptr_t(kQueueAlign)), UINT64_MAX - 2 * kQueueAlign);
+    WfBuffers b{};
+    carve_queues(&m, cap, b);
+    return fnd_arena_used(&m);
+}
+
+namespace {
+
+RenderState* render_state(fnd_pt_ctx* ctx)
+{
+    if (!ctx->render_state) ctx->render_state = ctx_new<RenderState>(ctx);
+    return static_cast<RenderState*>(ctx->render_state);
+}
+
+// The call's frame: reset the arena over the caller's block or the library's
+// own and carve the queues of a batch of `want` paths, or of the largest
+// batch of at least `least` paths that fits the caller's block.
+fnd_status begin_frame(fnd_pt_ctx* ctx, uint32_t want, uint32_t least, uint32_t* batch)
+{
+    RenderState* rs = render_state(ctx);
+    if (!rs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+    fnd_arena_reset(&rs->frame, nullptr, 0);  // nothing of the previous frame survives a failed call
+    rs->b = WfBuffers{};
+    rs->batch = 0;
+    uint32_t cap = want;
+    void* block = ctx->frame_memory;
+    uint64_t size = ctx->frame_bytes;
+    if (block) {
+        if (queue_bytes(cap) > size) {
+            uint32_t lo = 0, hi = cap;  // queue_bytes(lo) fits (or lo = 0), queue_bytes(hi) does not
+            while (hi - lo > 1) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                (queue_bytes(mid) <= size ? lo : hi) = mid;
+            }
+            cap = lo;
+        }
+        if (cap < least)
+            return set_error(ctx, FND_E_OOM_DEVICE,
+                             "frame arena of %llu bytes holds %u paths; the call needs at least %u (%llu bytes)",
+                             (unsigned long long)size, cap, least, (unsigned long long)queue_bytes(least));
+    } else {
+        const uint64_t need = queue_bytes(cap);
+        if (need > rs->owned_bytes) {  // grow: the previous call has finished (render is synchronous)
+            (void)hipStreamSynchronize(ctx->stream);
+            dev_free(ctx, rs->owned, rs->owned_bytes);
+            rs->owned = nullptr;
+            rs->owned_bytes = 0;
+            fnd_status st = dev_alloc(ctx, &rs->owned, need);
+            if (st != FND_OK) return st;
+            rs->owned_bytes = need;
+        }
+        block = rs->owned;
+        size = rs->owned_bytes;
     }
-    rs->capacity = cap;
-    st = check_hip(ctx, hipMemsetAsync(b.rays, 0, sizeof(unsigned long long), ctx->stream), "clear");
-    return st;
+    fnd_arena_reset(&rs->frame, block, size);
+    if (!carve_queues(&rs->frame, cap, rs->b))  // cannot happen: the sizes were checked
+        return set_error(ctx, FND_E_OOM_DEVICE, "frame arena exhausted");
+    rs->batch = cap;
+    *batch = cap;
+    return FND_OK;
 }
 
 fnd_status ensure_tables(fnd_pt_ctx* ctx, uint32_t njobs, uint32_t nscenes)
@@ -418,8 +457,9 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             base += hj[i].train_count;
         }
     }
-    const uint32_t cap = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
-    fnd_status st = ensure_capacity(ctx, cap);
+    const uint32_t want = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
+    uint32_t cap = 0;
+    fnd_status st = begin_frame(ctx, want, uint32_t(std::min<uint64_t>(1024u, total)), &cap);
     if (st != FND_OK) return st;
     st = ensure_tables(ctx, num_jobs, uint32_t(slots.size()));
     if (st != FND_OK) return st;
@@ -590,6 +630,53 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     ctx->stats.render_rays += rays;
     ctx->stats.paths += total;
     for (uint32_t i = 0; i < num_jobs; ++i) fbs[i]->samples += jobs[i].spp;
+    return FND_OK;
+}
+
+fnd_status fnd_pt_queue_bytes(uint32_t capacity, uint64_t* bytes)
+{
+    if (!bytes || capacity == 0) return FND_E_INVALID_ARG;
+    *bytes = queue_bytes(capacity);
+    return FND_OK;
+}
+
+fnd_status fnd_pt_set_frame_arena(fnd_pt_ctx* ctx, void* memory, uint64_t bytes)
+{
+    if (!ctx) return FND_E_INVALID_ARG;
+    if (memory && reinterpret_cast<uintptr_t>(memory) % kQueueAlign)
+        return set_error(ctx, FND_E_INVALID_ARG, "frame arena must be %llu-byte aligned",
+                         (unsigned long long)kQueueAlign);
+    if (memory && bytes == 0) return set_error(ctx, FND_E_INVALID_ARG, "frame arena of 0 bytes");
+    fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "set frame arena");
+    if (st != FND_OK) return st;
+    if (memory) {  // the library's block is no longer used: give it back
+        if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
+            dev_free(ctx, rs->owned, rs->owned_bytes);
+            rs->owned = nullptr;
+            rs->owned_bytes = 0;
+        }
+    }
+    if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
+        fnd_arena_reset(&rs->frame, nullptr, 0);
+        rs->b = WfBuffers{};
+        rs->batch = 0;
+    }
+    ctx->frame_memory = memory;
+    ctx->frame_bytes = memory ? bytes : 0;
+    return FND_OK;
+}
+
+fnd_status fnd_pt_frame_arena_info(fnd_pt_ctx* ctx, fnd_frame_arena_info* out)
+{
+    if (!ctx || !out) return FND_E_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    const auto* rs = static_cast<const RenderState*>(ctx->render_state);
+    out->caller_owned = ctx->frame_memory ? 1u : 0u;
+    out->size = ctx->frame_memory ? ctx->frame_bytes : (rs ? rs->owned_bytes : 0);
+    if (rs) {
+        out->used = fnd_arena_used(&rs->frame);
+        out->batch_paths = rs->batch;
+    }
     return FND_OK;
 }
 

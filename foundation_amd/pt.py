@@ -172,6 +172,101 @@ def mesh_from_streams(vertex_bytes, stride, position_offset, position_components
     return pos, idx
 
 
+# ---------------------------------------------------------------- frame arena
+
+
+class StackAllocator:
+    """Bump allocator over one block of memory (fnd_arena_*), named after
+    the reference's Core::StackAllocator (src/Core/Allocator/StackAllocator.hpp:9-63)
+    whose semantics it has: Allocate returns None for size 0, without memory,
+    or when the block would pass the end; Deallocate does nothing;
+    Reallocate raises.  Addresses are integers; the memory is never touched."""
+
+    def __init__(self, memory=0, size=0):
+        self.lib = A.load()
+        self.arena = A.Arena()
+        self.reset(memory, size)
+
+    def reset(self, memory=0, size=0):
+        self.lib.fnd_arena_reset(C.byref(self.arena), memory or None, size)
+
+    def allocate(self, size, alignment=None):
+        if alignment is None:
+            p = self.lib.fnd_arena_alloc(C.byref(self.arena), size)
+        else:
+            p = self.lib.fnd_arena_alloc_aligned(C.byref(self.arena), size, alignment)
+        return p  # ctypes gives None for a null pointer
+
+    def deallocate(self, ptr, size=0):
+        pass
+
+    def reallocate(self, ptr, new_size, alignment=None):
+        raise RuntimeError("StackAllocator does not support reallocation")
+
+    def used(self):
+        return self.lib.fnd_arena_used(C.byref(self.arena))
+
+
+class DeviceMemory:
+    """`nbytes` of device memory from hipMalloc of the HIP runtime that
+    libfnd_pt.so itself links: the caller-side allocation a renderer makes for
+    memory it lends to the library, such as a frame arena.  (A second copy of
+    the HIP runtime loaded into the same process, as some packages ship, is a
+    separate runtime; memory is taken from the library's own.)"""
+    _hip = None
+
+    def __init__(self, nbytes, device=0):
+        hip = DeviceMemory._runtime()
+        st = hip.hipSetDevice(device)
+        p = C.c_void_p()
+        if st == 0:
+            st = hip.hipMalloc(C.byref(p), nbytes)
+        if st != 0:
+            raise FndError(A.E_OOM_DEVICE if st == 2 else A.E_DEVICE, f"hipMalloc({nbytes}) failed with HIP error {st}")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @staticmethod
+    def _runtime():
+        if DeviceMemory._hip is None:
+            A.load()  # the library's DT_NEEDED runtime is then resolved by soname
+            hip = C.CDLL("libamdhip64.so.7")
+            hip.hipSetDevice.argtypes = [C.c_int]
+            hip.hipSetDevice.restype = C.c_int
+            hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            hip.hipMalloc.restype = C.c_int
+            hip.hipFree.argtypes = [C.c_void_p]
+            hip.hipFree.restype = C.c_int
+            DeviceMemory._hip = hip
+        return DeviceMemory._hip
+
+    def free(self):
+        if self.ptr:
+            DeviceMemory._hip.hipFree(self.ptr)
+            self.ptr = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def queue_bytes(capacity):
+    """Bytes of the per-frame ray queues of a batch of `capacity` paths."""
+    out = C.c_uint64()
+    st = A.load().fnd_pt_queue_bytes(capacity, C.byref(out))
+    if st != A.OK:
+        raise FndError(st, "fnd_pt_queue_bytes")
+    return out.value
+
+
 class ObjMesh:
     """A Wavefront OBJ mesh loaded by fnd_obj_load; ``desc`` is its
     fnd_mesh_desc (valid while the object lives)."""
@@ -368,6 +463,16 @@ class Context:
     def render(self, jobs):
         arr = (A.RenderJob * len(jobs))(*jobs)
         self._ck(self.lib.fnd_pt_render(self.h, arr, len(jobs)))
+
+    def set_frame_arena(self, device_ptr, nbytes):
+        """Per-frame ray queues from the caller's device memory (0: the
+        library's own block); the caller keeps the memory alive."""
+        self._ck(self.lib.fnd_pt_set_frame_arena(self.h, device_ptr or None, nbytes))
+
+    def frame_arena_info(self):
+        out = A.FrameArenaInfo()
+        self._ck(self.lib.fnd_pt_frame_arena_info(self.h, C.byref(out)))
+        return out
 
     def autotune(self, jobs, cache_path=None, force=False):
         """Searches launch configurations for `jobs` (fnd_pt_autotune) and

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define FND_PT_ABI_VERSION 4u
+#define FND_PT_ABI_VERSION 5u
 
 typedef uint64_t fnd_handle;
 #define FND_INVALID_HANDLE (~(uint64_t)0)
@@ -295,6 +295,69 @@ fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle fb, float e
                                          uint32_t* out_samples);
 /* Renders all jobs in one batched wavefront launch sequence. */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
+
+/* ---------------------------------------------------------------- frame arena */
+
+/* Bump allocation over one block of memory, with the semantics of the
+ * reference's Core::StackAllocator (src/Core/Allocator/StackAllocator.hpp:9-63,
+ * src/Core/Allocator/StackAllocator.cpp:5-29):
+ *   - fnd_arena_reset(a, memory, size) is Reset(Arena{memory, size}); memory
+ *     NULL is Reset(), an arena that gives out nothing.
+ *   - fnd_arena_alloc(a, size) takes the next `size` bytes, unaligned;
+ *     fnd_arena_alloc_aligned(a, size, alignment) rounds the start AND the
+ *     end of the block up to `alignment`.
+ *   - Both return NULL for size 0, for an arena without memory and when the
+ *     (padded) block would pass the end; nothing is taken then.
+ *   - fnd_arena_used(a) is GetUsedMemory(): bytes from the start of the
+ *     memory to the next free byte, padding included.
+ *   - Freeing is a no-op in the reference and there is no reallocation (it
+ *     throws, StackAllocator.hpp:51-53), so neither has an entry point here.
+ * Differences: `alignment` must be a power of two (NULL otherwise; the
+ * reference's mask arithmetic assumes one, Allocator.hpp:9-11), a block that
+ * would wrap the 64-bit address space returns NULL, and concurrent allocations
+ * from one arena are safe (one compare-and-swap on `current`, where the
+ * reference's multi-threaded variant loads, compares and stores separately,
+ * StackAllocator.cpp:8-15).  fnd_arena_reset must not overlap an allocation.
+ * The memory is never read or written, so it may be device memory. */
+typedef struct fnd_arena {
+    uint64_t memory;               /* address of the block; 0: none */
+    uint64_t end;                  /* memory + size */
+    uint64_t current;              /* next free address */
+} fnd_arena;
+
+void fnd_arena_reset(fnd_arena* arena, void* memory, uint64_t size);
+void* fnd_arena_alloc(fnd_arena* arena, uint64_t size);
+void* fnd_arena_alloc_aligned(fnd_arena* arena, uint64_t size, uint64_t alignment);
+uint64_t fnd_arena_used(const fnd_arena* arena);
+
+/* Per-frame ray queues.  Every fnd_pt_render resets the context's frame arena
+ * and bump-allocates from it, 256-byte aligned, the buffers of one wavefront
+ * batch (path state, both extension queues, the shadow queue, hits and cache
+ * queries): fnd_pt_queue_bytes(capacity) bytes for a batch of `capacity`
+ * paths.  This is the reference's command list, which resets a
+ * StackAllocator over its arena at Begin()
+ * (src/Platform/RHI/Vulkan/Command.cpp:38-50, Command.hpp:44-48).
+ * By default the block is the library's own: one device allocation, grown
+ * when a call needs a larger batch.  fnd_pt_set_frame_arena gives it the
+ * caller's memory instead (device memory of the context's device, 256-byte
+ * aligned, kept alive by the caller until it is replaced or the context is
+ * destroyed; the call waits for the context's stream first).  The library
+ * then allocates nothing for the queues, and a call whose batch
+ * (tuning.queue_capacity, at most the call's paths) does not fit runs the
+ * largest batch that does; images do not depend on the batch.  A block that
+ * holds fewer than min(1024, paths of the call) paths fails the call with
+ * FND_E_OOM_DEVICE.  memory NULL goes back to the library's own block. */
+fnd_status fnd_pt_queue_bytes(uint32_t capacity, uint64_t* bytes);
+fnd_status fnd_pt_set_frame_arena(fnd_pt_ctx* ctx, void* memory, uint64_t bytes);
+
+typedef struct fnd_frame_arena_info {
+    uint64_t size;                 /* bytes of the block (0 before the first render with the library's block) */
+    uint64_t used;                 /* bytes the last render call's queues took */
+    uint32_t batch_paths;          /* paths per batch of the last render call */
+    uint32_t caller_owned;         /* 1: the block came from fnd_pt_set_frame_arena */
+} fnd_frame_arena_info;
+
+fnd_status fnd_pt_frame_arena_info(fnd_pt_ctx* ctx, fnd_frame_arena_info* out);
 
 /* ---------------------------------------------------------------- autotuner */
 
