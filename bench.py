@@ -21,7 +21,8 @@ displaced-sphere scene (1.31M-triangle icosphere + 200k soup, 4 quad lights,
 1920x1080, depth 8, RR from 3) until relMSE <= 0.01 against a 4096-spp FP64
 GPU reference, plain and with the radiance cache trained every frame; device
 seconds of the frames are summed.  extra.path_tracing_10M is samples/s of
-full path tracing on the 10M-triangle scene.  extra.frame_arena_10M: one
+full path tracing on the 10M-triangle scene at 1920x1080, 64 spp in one call,
+depth 8, RR from 3 (NEE + MIS on the 4 quad lights).  extra.frame_arena_10M: one
 1-spp 1080p frame of that scene with its per-frame ray queues carved from the
 library's own block and from caller device memory of 1024/256/64/16 MiB
 (smaller blocks run smaller batches): ms per frame, batch, and bitwise
@@ -407,7 +408,7 @@ def main():
 
     render = relmse = batch = arena = None
     if not args.no_render and not args.only_incoherent:
-        render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 4, 8, 3)
+        render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 64, 8, 3)
         log(f"path tracing @10M tris: {render}")
         arena = frame_arena_sweep(ctx, scene, gen.camera, args.width, args.height, 8, 3)
         log(f"frame arena: {arena}")

@@ -206,73 +206,156 @@ __device__ __forceinline__ float ray_eps(V3 p)
     return 1e-4f * fmaxf(1.0f, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
 }
 
+// Job, pixel (0xFFFFFFFF for tile padding) and sample of global path g: the
+// path space is, per job, spp x padded pixels, sample-major, pixels in T x T
+// tile order.
+struct PathSite {
+    uint32_t job, pixel, sample, s, x, y;
+};
+
+__device__ __forceinline__ PathSite locate(const WfArgs& a, uint64_t g)
+{
+    PathSite q;
+    q.job = find_job(a.jobs, a.num_jobs, g);
+    const JobDev& job = a.jobs[q.job];
+    const uint64_t local = g - job.path_base;
+    q.s = uint32_t(local / job.padded);
+    const uint32_t k = uint32_t(local % job.padded);
+    const uint32_t T = job.tile;
+    const uint32_t t = k / (T * T), r = k % (T * T);
+    q.x = (t % job.tiles_x) * T + r % T;
+    q.y = (t / job.tiles_x) * T + r / T;
+    q.sample = job.sample_offset + q.s;
+    q.pixel = (q.x < job.width && q.y < job.height) ? q.y * job.width + q.x : 0xFFFFFFFFu;
+    return q;
+}
+
+// Starts window path `gid` (global path g0 + gid) in `slot`: the slot's
+// state, and the camera ray.  False for tile padding (no ray; radiance 0).
+__device__ __forceinline__ bool start_path(const WfArgs& a, uint64_t g0, uint32_t gid, uint32_t slot, Ray& ray)
+{
+    const uint64_t g = g0 + gid;
+    const PathSite q = locate(a, g);
+    const JobDev& job = a.jobs[q.job];
+    uint32_t train = kNoTrainSlot;
+    bool live = false;
+    if (q.pixel != 0xFFFFFFFFu) {
+        if (job.train_count) {
+            const uint64_t qi = uint64_t(q.s) * (uint64_t(job.width) * job.height) + q.pixel;
+            if (qi < job.train_mod) {
+                // (qi * mul + add) mod train_mod, x < 2^64 (qi < train_mod < 2^32,
+                // mul < 2^32).  Barrett with floor((2^64 - 1) / train_mod): the
+                // quotient estimate is low by at most 2.
+                const uint64_t x = qi * job.train_mul + job.train_add;
+                uint64_t ti = x - __umul64hi(x, job.train_barrett) * job.train_mod;
+                if (ti >= job.train_mod) ti -= job.train_mod;
+                if (ti >= job.train_mod) ti -= job.train_mod;
+                if (ti < job.train_count) {
+                    train = job.train_base + uint32_t(ti);
+                    a.b.rec_count[train] = 0u;
+                    a.b.t_list[train] = g;  // slots are unique: no atomic
+                }
+            }
+        }
+        Rng rng = rng_for(job.seed, q.sample, q.pixel, 0u);
+        const float jx = rng_float(rng), jy = rng_float(rng);
+        const float sx = (2.0f * (float(q.x) + jx) / float(job.width) - 1.0f) * job.tan_half * job.aspect;
+        const float sy = (1.0f - 2.0f * (float(q.y) + jy) / float(job.height)) * job.tan_half;
+        const V3 f = v3(job.fwd[0], job.fwd[1], job.fwd[2]);
+        const V3 r = v3(job.right[0], job.right[1], job.right[2]);
+        const V3 u = v3(job.up[0], job.up[1], job.up[2]);
+        const V3 d = normalize(f + r * sx + u * sy);
+        ray.org[0] = job.eye[0]; ray.org[1] = job.eye[1]; ray.org[2] = job.eye[2];
+        ray.tmin = 0.0f;
+        ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+        ray.tmax = 3.0e38f;
+        live = true;
+    }
+    a.b.p_job[slot] = q.job;
+    a.b.p_pixel[slot] = q.pixel;
+    a.b.p_sample[slot] = q.sample;
+    a.b.p_thr[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+    a.b.p_L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.b.p_train[slot] = train;
+    a.b.p_gid[slot] = gid;
+    a.b.p_bounce[slot] = 0u;
+    return live;
+}
+
+// The window's first n paths into slots 0..n-1; camera rays to extension
+// queue A, padding slots to the ended list.
 __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_t n)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.b.counters[kCntNext] = n;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        bool live = false;
-        uint32_t pixel = 0xFFFFFFFFu, sample = 0, j = 0, train = kNoTrainSlot;
         Ray ray{};
-        if (i < n) {
-            const uint64_t g = g0 + i;
-            j = find_job(a.jobs, a.num_jobs, g);
-            const JobDev& job = a.jobs[j];
-            const uint64_t local = g - job.path_base;
-            const uint32_t s = uint32_t(local / job.padded);
-            const uint32_t k = uint32_t(local % job.padded);
-            const uint32_t T = job.tile;
-            const uint32_t t = k / (T * T), q = k % (T * T);
-            const uint32_t x = (t % job.tiles_x) * T + q % T;
-            const uint32_t y = (t / job.tiles_x) * T + q / T;
-            sample = job.sample_offset + s;
-            if (x < job.width && y < job.height) {
-                pixel = y * job.width + x;
-                if (job.train_count) {
-                    const uint64_t qi = uint64_t(s) * (uint64_t(job.width) * job.height) + pixel;
-                    if (qi < job.train_mod) {
-                        // (qi * mul + add) mod train_mod, x < 2^64 (qi < train_mod < 2^32,
-                        // mul < 2^32).  Barrett with floor((2^64 - 1) / train_mod): the
-                        // quotient estimate is low by at most 2.
-                        const uint64_t x = qi * job.train_mul + job.train_add;
-                        uint64_t ti = x - __umul64hi(x, job.train_barrett) * job.train_mod;
-                        if (ti >= job.train_mod) ti -= job.train_mod;
-                        if (ti >= job.train_mod) ti -= job.train_mod;
-                        if (ti < job.train_count) {
-                            train = job.train_base + uint32_t(ti);
-                            a.b.rec_count[train] = 0u;
-                            a.b.t_list[train] = g;  // slots are unique: no atomic
-                        }
-                    }
-                }
-                Rng rng = rng_for(job.seed, sample, pixel, 0u);
-                const float jx = rng_float(rng), jy = rng_float(rng);
-                const float sx = (2.0f * (float(x) + jx) / float(job.width) - 1.0f) * job.tan_half * job.aspect;
-                const float sy = (1.0f - 2.0f * (float(y) + jy) / float(job.height)) * job.tan_half;
-                const V3 f = v3(job.fwd[0], job.fwd[1], job.fwd[2]);
-                const V3 r = v3(job.right[0], job.right[1], job.right[2]);
-                const V3 u = v3(job.up[0], job.up[1], job.up[2]);
-                const V3 d = normalize(f + r * sx + u * sy);
-                ray.org[0] = job.eye[0]; ray.org[1] = job.eye[1]; ray.org[2] = job.eye[2];
-                ray.tmin = 0.0f;
-                ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
-                ray.tmax = 3.0e38f;
-                live = true;
-            }
-            a.b.p_job[i] = j;
-            a.b.p_pixel[i] = pixel;
-            a.b.p_sample[i] = sample;
-            a.b.p_thr[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-            a.b.p_L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            a.b.p_train[i] = train;
-        }
-        uint32_t slot[1];
-        block_append<1>({live}, {&a.b.counters[kCntExtA]}, slot);
+        const bool live = i < n && start_path(a, g0, i, i, ray);
+        const bool dead = i < n && !live;
+        uint32_t slot[2];
+        block_append<2>({live, dead}, {&a.b.counters[kCntExtA], &a.b.counters[kCntEnded]}, slot);
         if (live) {
             a.b.ext_rays[0][slot[0]] = ray;
             a.b.ext_path[0][slot[0]] = i;
-            a.b.ext_scene[0][slot[0]] = a.jobs[j].scene_slot;
+            a.b.ext_scene[0][slot[0]] = a.jobs[a.b.p_job[i]].scene_slot;
         }
+        if (dead) a.b.ended[slot[1]] = i;
+    }
+}
+
+// Regeneration: each free slot of list fidx takes the window's next path
+// (while any is left); camera rays to extension queue `next`, padding slots
+// to the ended list.
+__global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t fidx, uint32_t next)
+{
+    const uint32_t nf = a.b.counters[kCntFree0 + fidx];
+    // No path left: return, so the counter stops growing.  One read per block,
+    // shared, so the whole block returns or none of it does (block_append
+    // synchronises the block).
+    __shared__ uint32_t s_next;
+    if (threadIdx.x == 0) s_next = a.b.counters[kCntNext];
+    __syncthreads();
+    if (s_next >= a.b.window) return;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nf; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool want = i < nf;
+        uint32_t gs[1];
+        block_append<1>({want}, {&a.b.counters[kCntNext]}, gs);
+        const bool has = want && gs[0] < a.b.window;
+        const uint32_t slot = want ? a.b.free_list[fidx][i] : 0u;
+        Ray ray{};
+        const bool live = has && start_path(a, g0, gs[0], slot, ray);
+        const bool dead = has && !live;
+        uint32_t q[2];
+        block_append<2>({live, dead}, {&a.b.counters[next], &a.b.counters[kCntEnded]}, q);
+        if (live) {
+            a.b.ext_rays[next][q[0]] = ray;
+            a.b.ext_path[next][q[0]] = slot;
+            a.b.ext_scene[next][q[0]] = a.jobs[a.b.p_job[slot]].scene_slot;
+        }
+        if (dead) a.b.ended[q[1]] = slot;
+    }
+}
+
+// Paths that ended this iteration (their shadow rays resolved): radiance to
+// the window record, slot to free list fidx.
+__global__ __launch_bounds__(256) void k_finalize(WfArgs a, uint32_t fidx)
+{
+    const uint32_t n = a.b.counters[kCntEnded];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool v = i < n;
+        if (v) {
+            const uint32_t slot = a.b.ended[i];
+            const float4 L = a.b.p_L[slot];
+            a.b.w_L[a.b.p_gid[slot]] = make_float4(L.x, L.y, L.z, 0.0f);
+        }
+        uint32_t q[2];
+        block_append<2>({v, v}, {&a.b.counters[kCntFree0 + fidx], &a.b.counters[kCntDone]}, q);
+        if (v) a.b.free_list[fidx][q[0]] = a.b.ended[i];
     }
 }
 
@@ -287,18 +370,22 @@ __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
     if (threadIdx.x < kCntCount && ((mask >> threadIdx.x) & 1u)) counters[threadIdx.x] = 0u;
 }
 
-// Zeroes the counters the next bounce appends to and the traversal work counters.
-__global__ void k_prep(uint32_t* counters, uint32_t next)
+// Zeroes the counters one iteration appends to: the next extension queue,
+// the shadow queue, the traversal work counters, the ended list and the free
+// list this iteration's finalize fills.
+__global__ void k_prep(uint32_t* counters, uint32_t next, uint32_t free_zero)
 {
     if (threadIdx.x == 0) {
         counters[next] = 0u;
         counters[kCntShadow] = 0u;
         counters[kCntWorkExt] = 0u;
         counters[kCntWorkShadow] = 0u;
+        counters[kCntEnded] = 0u;
+        counters[kCntFree0 + free_zero] = 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t bounce)
+__global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur)
 {
     const uint32_t n = a.b.counters[cur];
     const uint32_t nxt = cur ^ 1u;
@@ -313,6 +400,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
         uint32_t path = 0, scene_slot = 0;
         if (i < n) {
             path = a.b.ext_path[cur][i];
+            const uint32_t bounce = a.b.p_bounce[path];
             const JobDev& job = a.jobs[a.b.p_job[path]];
             scene_slot = job.scene_slot;
             const ShadeScene& sc = a.scenes[scene_slot];
@@ -463,17 +551,22 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                             thr = nthr;
                             cont = true;
                             a.b.p_thr[path] = make_float4(thr.x, thr.y, thr.z, bs.pdf);
+                            a.b.p_bounce[path] = bounce + 1u;
                         }
                     }
                 }
             }
             a.b.p_L[path] = make_float4(L.x, L.y, L.z, __uint_as_float(ncode));
         }
-        // one block-aggregated append for the three queues
-        uint32_t slots[3];
-        block_append<3>({cont, query, shadow}, {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow]},
+        // one block-aggregated append for the four lists; a path that does not
+        // continue has ended (its shadow ray, if any, resolves before finalize)
+        const bool ended = i < n && !cont;
+        uint32_t slots[4];
+        block_append<4>({cont, query, shadow, ended},
+                        {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow], &a.b.counters[kCntEnded]},
                         slots);
         const uint32_t es = slots[0], qs = slots[1], ss = slots[2];
+        if (ended) a.b.ended[slots[3]] = path;
         if (cont) {
             a.b.ext_rays[nxt][es] = next_ray;
             a.b.ext_path[nxt][es] = path;
@@ -481,7 +574,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
         }
         if (query) {
             a.b.q_query[qs] = cq;
-            a.b.q_path[qs] = path;
+            a.b.q_path[qs] = a.b.p_gid[path];
             a.b.q_weight[qs] = make_float4(q_w.x, q_w.y, q_w.z, 0.f);
         }
         if (shadow) {
@@ -508,16 +601,17 @@ __global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a)
 }
 
 // Per pixel, in increasing sample order: the thread of a pixel's first path in
-// the batch adds the batch's samples of that pixel one after another, so the
-// framebuffer sums are the same for any batch size or tile order.
+// the window adds the window's samples of that pixel one after another, so the
+// framebuffer sums are the same for any window size, slot count or tile order.
 __global__ __launch_bounds__(256) void k_accumulate(WfArgs a, uint64_t g0, uint32_t n)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t pixel = a.b.p_pixel[i];
-        if (pixel == 0xFFFFFFFFu) continue;
-        const JobDev& job = a.jobs[a.b.p_job[i]];
-        if (!job.accum) continue;
         const uint64_t g = g0 + i;
+        const PathSite site = locate(a, g);
+        const uint32_t pixel = site.pixel;
+        if (pixel == 0xFFFFFFFFu) continue;
+        const JobDev& job = a.jobs[site.job];
+        if (!job.accum) continue;
         const uint64_t lo = job.path_base > g0 ? job.path_base : g0;
         if (g >= lo + job.padded) continue;  // not the first sample of this pixel in the batch
         const uint64_t end = job.path_end < g0 + n ? job.path_end : g0 + n;
@@ -527,7 +621,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfArgs a, uint64_t g0, uint3
         double* acc64 = job.accum64 ? job.accum64 + 3ull * pixel : nullptr;
         if (acc64) { r64 = acc64[0]; g64 = acc64[1]; b64 = acc64[2]; }
         for (uint64_t h = g; h < end; h += job.padded) {
-            const float4 L = a.b.p_L[h - g0];
+            const float4 L = a.b.w_L[h - g0];
             r += L.x; gr += L.y; b += L.z;
             if (acc64) { r64 += double(L.x); g64 += double(L.y); b64 += double(L.z); }
         }
@@ -541,34 +635,33 @@ __global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
 {
     const uint32_t n = a.b.counters[kCntQuery];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t path = a.b.q_path[i];
+        const uint32_t gid = a.b.q_path[i];
         const float4 w = a.b.q_weight[i], p = a.b.q_pred[i];
-        float4 L = a.b.p_L[path];
+        float4 L = a.b.w_L[gid];
         L.x += w.x * fmaxf(p.x, 0.0f);
         L.y += w.y * fmaxf(p.y, 0.0f);
         L.z += w.z * fmaxf(p.z, 0.0f);
-        a.b.p_L[path] = L;
+        a.b.w_L[gid] = L;
     }
 }
 
-// Training targets from the finished training paths of a batch: at a
+// Training targets from the finished training paths of a window: at a
 // recorded diffuse vertex with weight = throughput * albedo, target =
 // (L_path - L_mark) / weight, the albedo-demodulated radiance the path
 // gathered after that vertex's own light sample (per channel; vertices with a
 // vanishing weight are skipped).
 __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint64_t g0, uint32_t n)
 {
-    // the training slots whose path lies in this batch [g0, g0 + n)
+    // the training slots whose path lies in this window [g0, g0 + n)
     const uint32_t ns = a.b.t_slots;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < ns; base += stride) {
         const uint32_t j = base + threadIdx.x;
         const unsigned long long g = j < ns ? a.b.t_list[j] : ~0ull;
         const bool mine = g >= g0 && g - g0 < n;
-        const uint32_t i = mine ? uint32_t(g - g0) : 0u;
-        const uint32_t tslot = mine ? a.b.p_train[i] : kNoTrainSlot;
+        const uint32_t tslot = mine ? j : kNoTrainSlot;  // t_list is indexed by training slot
         const uint32_t rc = tslot != kNoTrainSlot ? a.b.rec_count[tslot] : 0u;
-        const float4 L = rc ? a.b.p_L[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 L = rc ? a.b.w_L[uint32_t(g - g0)] : make_float4(0.f, 0.f, 0.f, 0.f);
         // one wave-aggregated append per record index (every lane takes part)
         for (uint32_t k = 0; k < a.b.rec_max; ++k) {
             bool take = false;
@@ -623,6 +716,16 @@ void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hi
     hipLaunchKernelGGL(k_generate, dim3(grid), dim3(256), 0, s, a, g0, n);
 }
 
+void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_regen, dim3(grid), dim3(256), 0, s, a, g0, fidx, next);
+}
+
+void launch_finalize(const WfArgs& a, uint32_t fidx, uint32_t grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(256), 0, s, a, fidx);
+}
+
 // One pixel per thread: mean = sum * inv (the same product as the FP32
 // readback), times exposure; each channel's byte is the number of the 255
 // ascending thresholds at or below it (binary search; NaN and x <= 0 give 0).
@@ -655,14 +758,14 @@ void launch_present(const float* accum, uint64_t pixels, float inv, float exposu
     hipLaunchKernelGGL(k_present, dim3(grid), dim3(256), 0, s, accum, pixels, inv, exposure, thresholds, out);
 }
 
-void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s)
+void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, next);
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, next, free_zero);
 }
 
-void launch_shade(const WfArgs& a, uint32_t cur, uint32_t bounce, uint32_t grid, hipStream_t s)
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a, cur, bounce);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a, cur);
 }
 
 void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s)

@@ -27,8 +27,17 @@ using namespace fnd;
 
 namespace fnd {
 
+// Drain polling: after iteration k the window's finished-path count is
+// copied to pinned host memory; the host reads iteration k - kPollLag's copy
+// before it queues iteration k + 1, so the device always has kPollLag
+// iterations queued and at most kPollLag empty ones run after the window drains.
+constexpr uint32_t kPollLag = 3;
+constexpr uint32_t kPollRing = kPollLag + 1;
+
 struct RenderState {
     WfBuffers b{};
+    uint32_t* poll_host = nullptr;  // kPollRing words of pinned host memory
+    hipEvent_t poll_ev[kPollRing] = {};
     fnd_arena frame{};        // reset at every render call, the queues are carved from it
     void* owned = nullptr;    // the library's block (when the caller gives none)
     uint64_t owned_bytes = 0;
@@ -44,6 +53,10 @@ void render_state_free(fnd_pt_ctx* ctx)
 {
     auto* rs = static_cast<RenderState*>(ctx->render_state);
     if (!rs) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (rs->poll_host) (void)hipHostFree(rs->poll_host);
+    for (auto& e : rs->poll_ev)
+        if (e) (void)hipEventDestroy(e);
     dev_free(ctx, rs->owned, rs->owned_bytes);
     dev_free(ctx, rs->d_jobs, rs->jobs_cap * sizeof(JobDev));
     dev_free(ctx, rs->d_shade, rs->scenes_cap * sizeof(ShadeScene));
@@ -68,7 +81,9 @@ static bool carve_queues(fnd_arena* ar, uint64_t cap, WfBuffers& b)
         *p = static_cast<T*>(q);
     };
     A(&b.p_job, cap); A(&b.p_pixel, cap); A(&b.p_sample, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
-    A(&b.p_train, cap);
+    A(&b.p_train, cap); A(&b.p_gid, cap); A(&b.p_bounce, cap);
+    A(&b.w_L, cap * kWindowSlots);
+    A(&b.ended, cap); A(&b.free_list[0], cap); A(&b.free_list[1], cap);
     for (int q = 0; q < 2; ++q) {
         A(&b.ext_rays[q], cap); A(&b.ext_path[q], cap); A(&b.ext_scene[q], cap);
     }
@@ -513,63 +528,127 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     }
     const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
 
-    // One pass of the wavefront over a job table already on the device.
-    auto run = [&](uint32_t njobs, uint64_t paths, uint32_t batch, uint32_t depth, bool apply_cache,
+    // Drain polling resources, made once per render state.
+    if (!rs->poll_host) {
+        void* h = nullptr;
+        st = check_hip(ctx, hipHostMalloc(&h, kPollRing * sizeof(uint32_t)), "hipHostMalloc");
+        if (st != FND_OK) return st;
+        rs->poll_host = static_cast<uint32_t*>(h);
+        for (auto& e : rs->poll_ev) {
+            st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            if (st != FND_OK) return st;
+        }
+    }
+
+    // One pass of the wavefront over a job table already on the device, window
+    // by window.  A window's first paths fill the slots; then every iteration
+    // shades the traced extension rays, gives the slots freed in the previous
+    // iteration the window's next paths (their camera rays join the same
+    // queue), traces extension and shadow rays in one launch, adds the shadow
+    // results and retires the paths that ended.  Paths ending into the
+    // radiance cache are queried once the window has drained, so a window
+    // that uses the cache holds at most one path per slot.
+    auto run = [&](uint32_t njobs, uint64_t paths, uint32_t slots, uint32_t depth, bool apply_cache,
                    bool collect) -> fnd_status {
         wa.num_jobs = njobs;
-        for (uint64_t g0 = 0; g0 < paths; g0 += batch) {
-            const uint32_t n = uint32_t(std::min<uint64_t>(batch, paths - g0));
-            const uint32_t grid = std::min(stage_grid, (n + 255u) / 256u);
-            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery), ctx->stream);
-            launch_generate(wa, g0, n, grid, ctx->stream);
-            // One traversal launch per bounce: the extension rays of bounce b + 1
-            // (closest hit) and the shadow rays of bounce b (any hit) share it,
-            // so their tails overlap.  The shadow results are added before the
-            // next shade, in the same order as with separate launches.
-            TraceArgs ta{};
-            ta.scenes = rs->d_scenes;
-            ta.hits = rs->b.ext_hits;
-            ta.counter = &rs->b.counters[kCntWorkExt];
-            ta.spill = ctx->d_spill;
-            ta.spill_stride = tl.grid * tl.block;
-            ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
-            ta.rays2 = rs->b.sh_rays;
-            ta.hits2 = rs->b.sh_hits;
-            ta.ray_scene2 = rs->b.sh_scene;
-            ta.nrays2_dev = &rs->b.counters[kCntShadow];
-            ta.ray_total = rs->b.rays;
-            TraceLaunch l = tl;
-            l.mode = kModeMixedInternal;
-            auto trace = [&](uint32_t q, const char* what) -> fnd_status {
-                ta.rays = rs->b.ext_rays[q];
-                ta.ray_scene = rs->b.ext_scene[q];
-                ta.nrays_dev = &rs->b.counters[q];
-                return check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
-            };
+        const uint64_t wmax = (apply_cache || collect) ? uint64_t(slots) : uint64_t(slots) * kWindowSlots;
+        TraceArgs ta{};
+        ta.scenes = rs->d_scenes;
+        ta.hits = rs->b.ext_hits;
+        ta.counter = &rs->b.counters[kCntWorkExt];
+        ta.spill = ctx->d_spill;
+        ta.spill_stride = tl.grid * tl.block;
+        ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
+        ta.rays2 = rs->b.sh_rays;
+        ta.hits2 = rs->b.sh_hits;
+        ta.ray_scene2 = rs->b.sh_scene;
+        ta.nrays2_dev = &rs->b.counters[kCntShadow];
+        ta.ray_total = rs->b.rays;
+        TraceLaunch l = tl;
+        l.mode = kModeMixedInternal;
+        // One traversal launch per iteration: the extension rays (closest hit)
+        // and the shadow rays of the vertices just shaded (any hit) share it,
+        // so their tails overlap.
+        auto trace = [&](uint32_t q, const char* what) -> fnd_status {
+            ta.rays = rs->b.ext_rays[q];
+            ta.ray_scene = rs->b.ext_scene[q];
+            ta.nrays_dev = &rs->b.counters[q];
+            return check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
+        };
+        for (uint64_t g0 = 0; g0 < paths; g0 += wmax) {
+            const uint32_t W = uint32_t(std::min<uint64_t>(wmax, paths - g0));
+            const uint32_t n0 = std::min(W, slots);
+            const uint32_t wgrid = std::min(stage_grid, (W + 255u) / 256u);
+            const uint32_t sgrid = std::min(stage_grid, (n0 + 255u) / 256u);  // a window uses n0 slots
+            // A window that fits in the slots has no regeneration: slot i holds
+            // window path i to its end, so its radiance stays in place and the
+            // bounce count bounds the iterations (no finalize, no polling).
+            const bool regen = W > slots;
+            wa.b.window = W;
+            wa.b.w_L = regen ? rs->b.w_L : rs->b.p_L;
             uint32_t cur = kCntExtA;
-            launch_prep(wa, cur ^ 1u, ctx->stream);  // empty shadow queue, zero work counter
+            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntDone) | (1u << kCntFree0),
+                              ctx->stream);
+            launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter
+            launch_generate(wa, g0, n0, sgrid, ctx->stream);
             fnd_status e = trace(cur, "primary trace");
             if (e != FND_OK) return e;
-            for (uint32_t bounce = 0; bounce <= depth; ++bounce) {
-                const uint32_t nxt = cur ^ 1u;
-                launch_prep(wa, nxt, ctx->stream);
-                launch_shade(wa, cur, bounce, grid, ctx->stream);
-                if (bounce < depth) {  // the last shade appends no rays
+            if (!regen) {
+                for (uint32_t k = 0; k <= depth; ++k) {
+                    const uint32_t nxt = cur ^ 1u;
+                    launch_prep(wa, nxt, (k & 1u) ^ 1u, ctx->stream);
+                    launch_shade(wa, cur, sgrid, ctx->stream);
+                    if (k < depth) {  // the last shade appends no rays
+                        e = trace(nxt, "extension + shadow trace");
+                        if (e != FND_OK) return e;
+                        launch_shadow_resolve(wa, sgrid, ctx->stream);
+                    }
+                    cur = nxt;
+                }
+            } else {
+                launch_finalize(wa, 0u, sgrid, ctx->stream);  // padding slots of the first paths
+                // every path is shaded at most depth + 1 times, and while paths are
+                // left no slot stays idle, so the window drains within this many
+                // iterations (plus the polling lag)
+                const uint64_t max_iters = (uint64_t(W) / slots + 2u) * (uint64_t(depth) + 3u) + kPollLag + 2u;
+                bool drained = false;
+                for (uint64_t k = 0; !drained; ++k) {
+                    if (k >= max_iters)
+                        return set_error(ctx, FND_E_DEVICE,
+                                         "wavefront window of %u paths did not drain in %llu iterations", W,
+                                         (unsigned long long)max_iters);
+                    const uint32_t nxt = cur ^ 1u;
+                    const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
+                    launch_prep(wa, nxt, fw, ctx->stream);
+                    launch_shade(wa, cur, sgrid, ctx->stream);
+                    launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
                     e = trace(nxt, "extension + shadow trace");
                     if (e != FND_OK) return e;
-                    launch_shadow_resolve(wa, grid, ctx->stream);
+                    launch_shadow_resolve(wa, sgrid, ctx->stream);
+                    launch_finalize(wa, fw, sgrid, ctx->stream);
+                    const uint32_t ring = uint32_t(k % kPollRing);
+                    e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring, &rs->b.counters[kCntDone], sizeof(uint32_t),
+                                                      hipMemcpyDeviceToHost, ctx->stream), "drain poll");
+                    if (e == FND_OK) e = check_hip(ctx, hipEventRecord(rs->poll_ev[ring], ctx->stream), "drain poll");
+                    if (e != FND_OK) return e;
+                    cur = nxt;
+                    if (k >= kPollLag) {
+                        const uint32_t old = uint32_t((k - kPollLag) % kPollRing);
+                        e = check_hip(ctx, hipEventSynchronize(rs->poll_ev[old]), "drain poll");
+                        if (e != FND_OK) return e;
+                        drained = rs->poll_host[old] >= W;
+                    }
                 }
-                cur = nxt;
             }
             if (apply_cache) {
-                const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
+                const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (W + 127u) / 128u));
                 launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
                                      cg, ctx->stream);
-                launch_cache_apply(wa, grid, ctx->stream);
+                launch_cache_apply(wa, wgrid, ctx->stream);
             }
             if (collect)
-                launch_cache_collect(wa, g0, n, std::max(1u, std::min(grid, (wa.b.t_slots + 255u) / 256u)), ctx->stream);
-            launch_accumulate(wa, g0, n, grid, ctx->stream);
+                launch_cache_collect(wa, g0, W, std::max(1u, std::min(wgrid, (wa.b.t_slots + 255u) / 256u)), ctx->stream);
+            launch_accumulate(wa, g0, W, wgrid, ctx->stream);
             e = check_hip(ctx, hipGetLastError(), "render launch");
             if (e != FND_OK) return e;
         }

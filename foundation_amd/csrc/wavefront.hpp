@@ -70,8 +70,19 @@ enum WfCounter : uint32_t {
     kCntWorkShadow = 4,
     kCntQuery = 5,
     kCntTrain = 6,
-    kCntCount = 8
+    kCntNext = 7,    // next window path to start (may pass the window's end)
+    kCntDone = 8,    // window paths finished
+    kCntEnded = 9,   // slots whose path ended this iteration
+    kCntFree0 = 10,  // free slots, two lists used alternately
+    kCntFree1 = 11,
+    kCntCount = 16
 };
+
+// Paths are rendered in windows of consecutive global path indices; a window
+// holds at most kWindowSlots times as many paths as there are slots, and a
+// slot whose path ends takes the window's next path (regeneration), so the
+// traversal launches stay full until the window's last paths drain.
+constexpr uint32_t kWindowSlots = 8;
 
 struct WfBuffers {
     // per path slot
@@ -81,6 +92,14 @@ struct WfBuffers {
     float4* p_thr;            // throughput, w = solid-angle pdf of the last BSDF sample (0: camera)
     float4* p_L;              // radiance
     uint32_t* p_train;        // training record slot, or kNoTrainSlot
+    uint32_t* p_gid;          // the slot's path: index in the window
+    uint32_t* p_bounce;       // vertices shaded so far
+    // per window path: final radiance (cache term added after the window drains)
+    float4* w_L;
+    uint32_t window;          // paths in the current window
+    // slot lists
+    uint32_t* ended;          // slots whose path ended this iteration (kCntEnded)
+    uint32_t* free_list[2];   // slots free for the next iteration (kCntFree0 + i)
     // extension queues (double-buffered) and hits
     Ray* ext_rays[2];
     uint32_t* ext_path[2];
@@ -96,7 +115,7 @@ struct WfBuffers {
     unsigned long long* rays; // rays traced (extension + shadow), cumulative: added by the trace launches
     // radiance-cache queries (at most one per path per batch)
     CacheQuery* q_query;
-    uint32_t* q_path;
+    uint32_t* q_path;         // window index of the querying path
     float4* q_weight;         // throughput * albedo at the query vertex
     float4* q_pred;
     // training records: per training slot, rec_max records of kCacheRecordF4 float4
@@ -120,10 +139,12 @@ struct WfArgs {
 };
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
+void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, uint32_t grid, hipStream_t s);
+void launch_finalize(const WfArgs& a, uint32_t fidx, uint32_t grid, hipStream_t s);
 void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
                     uint32_t* out, hipStream_t s);
-void launch_prep(const WfArgs& a, uint32_t next, hipStream_t s);
-void launch_shade(const WfArgs& a, uint32_t cur, uint32_t bounce, uint32_t grid, hipStream_t s);
+void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s);
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t grid, hipStream_t s);
 void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);

@@ -227,8 +227,9 @@ typedef struct fnd_tuning {
     uint32_t stack_lds;             /* 128x12, 128x16, 128x32, 256x8, 256x12, 256x16 (LDS stack entries) */
     uint32_t waves_per_cu;          /* persistent grid = CUs * waves_per_cu waves, at most what fits */
     uint32_t refill;                /* refetch rays once this many lanes of a wave are idle (1..64) */
-    uint32_t queue_capacity;        /* paths per wavefront batch (render) */
-    uint32_t tile;                  /* pixel tile edge for the batch pixel order (render) */
+    uint32_t queue_capacity;        /* path slots of the wavefront (render); a window of paths holds up to
+                                       8 per slot, and slots whose path ended take the window's next paths */
+    uint32_t tile;                  /* pixel tile edge for the pixel order of the path space (render) */
 } fnd_tuning;
 
 #define FND_TRACE_CLOSEST 0u
@@ -293,7 +294,8 @@ fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle fb, double* r
  * the thresholds rounded from double to float once); NaN and x <= 0 give 0. */
 fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle fb, float exposure, uint8_t* rgba,
                                          uint32_t* out_samples);
-/* Renders all jobs in one batched wavefront launch sequence. */
+/* Renders all jobs in one wavefront launch sequence: their paths form one path
+ * space, processed in windows (see fnd_tuning.queue_capacity). */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
 
 /* ---------------------------------------------------------------- frame arena */
@@ -331,29 +333,29 @@ void* fnd_arena_alloc_aligned(fnd_arena* arena, uint64_t size, uint64_t alignmen
 uint64_t fnd_arena_used(const fnd_arena* arena);
 
 /* Per-frame ray queues.  Every fnd_pt_render resets the context's frame arena
- * and bump-allocates from it, 256-byte aligned, the buffers of one wavefront
- * batch (path state, both extension queues, the shadow queue, hits and cache
- * queries): fnd_pt_queue_bytes(capacity) bytes for a batch of `capacity`
- * paths.  This is the reference's command list, which resets a
- * StackAllocator over its arena at Begin()
- * (src/Platform/RHI/Vulkan/Command.cpp:38-50, Command.hpp:44-48).
+ * and bump-allocates from it, 256-byte aligned, the buffers of the wavefront's
+ * path slots (path state, both extension queues, the shadow queue, hits, cache
+ * queries, the window's radiance and the slot lists):
+ * fnd_pt_queue_bytes(capacity) bytes for `capacity` slots.  This is the
+ * reference's command list, which resets a StackAllocator over its arena at
+ * Begin() (src/Platform/RHI/Vulkan/Command.cpp:38-50, Command.hpp:44-48).
  * By default the block is the library's own: one device allocation, grown
- * when a call needs a larger batch.  fnd_pt_set_frame_arena gives it the
- * caller's memory instead (device memory of the context's device, 256-byte
- * aligned, kept alive by the caller until it is replaced or the context is
- * destroyed; the call waits for the context's stream first).  The library
- * then allocates nothing for the queues, and a call whose batch
- * (tuning.queue_capacity, at most the call's paths) does not fit runs the
- * largest batch that does; images do not depend on the batch.  A block that
- * holds fewer than min(1024, paths of the call) paths fails the call with
- * FND_E_OOM_DEVICE.  memory NULL goes back to the library's own block. */
+ * when a call needs more slots.  fnd_pt_set_frame_arena gives it the caller's
+ * memory instead (device memory of the context's device, 256-byte aligned,
+ * kept alive by the caller until it is replaced or the context is destroyed;
+ * the call waits for the context's stream first).  The library then allocates
+ * nothing for the queues, and a call whose slots (tuning.queue_capacity, at
+ * most the call's paths) do not fit runs with the most slots that do; images
+ * do not depend on the slot count.  A block that holds fewer than
+ * min(1024, paths of the call) slots fails the call with FND_E_OOM_DEVICE.
+ * memory NULL goes back to the library's own block. */
 fnd_status fnd_pt_queue_bytes(uint32_t capacity, uint64_t* bytes);
 fnd_status fnd_pt_set_frame_arena(fnd_pt_ctx* ctx, void* memory, uint64_t bytes);
 
 typedef struct fnd_frame_arena_info {
     uint64_t size;                 /* bytes of the block (0 before the first render with the library's block) */
     uint64_t used;                 /* bytes the last render call's queues took */
-    uint32_t batch_paths;          /* paths per batch of the last render call */
+    uint32_t batch_paths;          /* path slots of the last render call */
     uint32_t caller_owned;         /* 1: the block came from fnd_pt_set_frame_arena */
 } fnd_frame_arena_info;
 

@@ -191,12 +191,14 @@ def test_concurrent_allocations_are_disjoint():
 
 # ---------------------------------------------------------------- per-frame ray queues
 
-# Bytes per path of the queues: path state (job, pixel, sample, train slot
-# 4 each; throughput and radiance 16 each) 48; two extension queues of a
-# 32-byte ray, a path id and a scene slot, 80; extension hits 16; the shadow
-# queue (ray, path, scene, contribution 16, hit 16) 72; cache queries
-# (48-byte query, path 4, weight 16, prediction 16) 84.
-QUEUE_BYTES_PER_PATH = 48 + 80 + 16 + 72 + 84
+# Bytes per slot of the queues: path state (job, pixel, sample, train slot,
+# window index, bounce 4 each; throughput and radiance 16 each) 56; two
+# extension queues of a 32-byte ray, a path id and a scene slot, 80;
+# extension hits 16; the shadow queue (ray, path, scene, contribution 16,
+# hit 16) 72; cache queries (48-byte query, path 4, weight 16, prediction 16)
+# 84; the window's radiance, 16 B for each of 8 window paths per slot, 128;
+# the ended list and two free lists, 12.
+QUEUE_BYTES_PER_PATH = 56 + 80 + 16 + 72 + 84 + 128 + 12
 QUEUE_BYTES_FIXED = 256 + 256  # counters and the ray total, one 256-byte slot each
 
 
@@ -205,9 +207,9 @@ def test_queue_bytes_per_path():
         cap = 64 * k  # every buffer a multiple of 256 bytes: no padding
         assert pt.queue_bytes(cap) == QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED, cap
     prev = 0
-    for cap in range(1, 600, 7):  # padding adds at most 255 bytes to each of the 25 buffers
+    for cap in range(1, 600, 7):  # padding adds at most 255 bytes to each of the 30 buffers
         q = pt.queue_bytes(cap)
-        assert QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED <= q <= QUEUE_BYTES_PER_PATH * cap + 25 * 256
+        assert QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED <= q <= QUEUE_BYTES_PER_PATH * cap + 30 * 256
         assert q % 256 == 0 and q >= prev
         prev = q
     with pytest.raises(pt.FndError):

@@ -9,8 +9,9 @@
   renders do; the test bounds relMSE(GPU, oracle) by a fraction of the
   seed-to-seed relMSE of the GPU itself.  The same holds for a displaced
   sphere with four lights and GGX materials (adaptive light selection).
-* Images do not depend on queue capacity, tile order, batching with other
-  jobs, or splitting the samples over several calls (bitwise).
+* Images do not depend on queue capacity, tile order, path regeneration,
+  batching with other jobs, or splitting the samples over several calls
+  (bitwise).
 """
 import numpy as np
 import pytest
@@ -148,6 +149,27 @@ def test_images_do_not_depend_on_queue_or_tiles(ctx, cornell):
             assert np.array_equal(img.view(np.uint32), base.view(np.uint32)), (cap, tile)
     finally:
         ctx.set_tuning(queue_capacity=old.queue_capacity, tile=old.tile)
+
+
+def test_regenerating_windows_equal_one_batch(ctx, cornell):
+    """15,360 paths: with the default capacity one window holds them all and
+    every slot keeps its path; with 1024 or 2000 slots, slots whose path ended
+    take the window's next paths (two windows of at most 8 x 1024).  Same
+    image bit for bit, and the same rays traced."""
+    g, s = cornell
+    old = ctx.tuning()
+    try:
+        ctx.reset_stats()
+        base, _ = render(ctx, s, g.camera, 40, 24, 16, 6, rr=2, seed=9)
+        rays = ctx.stats().render_rays
+        for cap in (1024, 2000):
+            ctx.set_tuning(queue_capacity=cap)
+            ctx.reset_stats()
+            img, _ = render(ctx, s, g.camera, 40, 24, 16, 6, rr=2, seed=9)
+            assert ctx.stats().render_rays == rays, cap
+            assert np.array_equal(img.view(np.uint32), base.view(np.uint32)), cap
+    finally:
+        ctx.set_tuning(queue_capacity=old.queue_capacity)
 
 
 def test_progressive_equals_single_call(ctx, cornell):
