@@ -298,6 +298,9 @@ def main():
     ap.add_argument("--frames-only", type=int, default=0, metavar="N",
                     help="profiling runs: render N plain then N cache frames of the relMSE scene (no reference, "
                          "no traversal legs), print ms per frame, exit")
+    ap.add_argument("--pt-only", action="store_true",
+                    help="profiling runs: build the 10M-triangle scene, run only the 64-spp path-tracing leg, "
+                         "print its result, exit")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="profiling runs: first copy this many MiB device-to-device (a plain copy kernel of "
                          "known size that scales the FETCH_SIZE / WRITE_SIZE counters)")
@@ -339,6 +342,10 @@ def main():
     info = ctx.scene_info(scene)
     log(f"scene built: {info.num_triangles} tris, {info.num_nodes} nodes, depth {info.max_depth}, "
         f"{info.build_ms:.0f} ms, {info.device_bytes / 2**20:.0f} MiB")
+    if args.pt_only:
+        log(f"path tracing @10M tris: {render_throughput(ctx, scene, gen.camera, args.width, args.height, 64, 8, 3)}")
+        ctx.close()
+        return
 
     primary = R.primary_rays(gen.camera, args.width, args.height)
     prim_hits = ctx.trace(scene, primary)

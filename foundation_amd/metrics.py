@@ -1,7 +1,7 @@
 """Image error metric of the path: relative mean squared error.
 
 relMSE(x, y) = mean over pixels and channels of (x - y)^2 / (y^2 + eps),
-eps = 0.01, the form used to judge renders against a converged reference
+eps = 0.01, the form used to compare renders with a converged reference
 (Rousselle et al., "Robust Denoising using Feature and Color Information",
 2013; also the loss of the radiance cache).  Evaluated in float64.
 """

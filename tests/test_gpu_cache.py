@@ -152,7 +152,7 @@ def rel_mse(x, y):
 def test_cache_beats_plain_path_tracing_at_equal_samples(ctx):
     """Cornell box + soup: 32 frames of 1 spp with the cache (trained every
     frame on paths the tracer generates) against 32 spp of plain path
-    tracing, both judged by a 1024-spp plain render in FP64."""
+    tracing, both measured against a 1024-spp plain render in FP64."""
     g = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
     scene = ctx.scene_create(g.desc)
     w = h = 96
