@@ -282,8 +282,16 @@ __device__ __forceinline__ bool start_path(const WfArgs& a, uint64_t g0, uint32_
     return live;
 }
 
+// Records the radiance of the path in `slot`, which has ended, and frees the
+// slot for the next iteration's regeneration (the caller appends it to a free
+// list).
+__device__ __forceinline__ void retire(const WfArgs& a, uint32_t slot, float4 L)
+{
+    a.b.w_L[a.b.p_gid[slot]] = make_float4(L.x, L.y, L.z, 0.0f);
+}
+
 // The window's first n paths into slots 0..n-1; camera rays to extension
-// queue A, padding slots to the ended list.
+// queue A, padding slots (radiance 0) straight to free list 0.
 __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_t n)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) a.b.counters[kCntNext] = n;
@@ -293,20 +301,21 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
         Ray ray{};
         const bool live = i < n && start_path(a, g0, i, i, ray);
         const bool dead = i < n && !live;
+        if (dead) retire(a, i, make_float4(0.f, 0.f, 0.f, 0.f));
         uint32_t slot[2];
-        block_append<2>({live, dead}, {&a.b.counters[kCntExtA], &a.b.counters[kCntEnded]}, slot);
+        block_append<2>({live, dead}, {&a.b.counters[kCntExtA], &a.b.counters[kCntFree0]}, slot);
         if (live) {
             a.b.ext_rays[0][slot[0]] = ray;
             a.b.ext_path[0][slot[0]] = i;
             a.b.ext_scene[0][slot[0]] = a.jobs[a.b.p_job[i]].scene_slot;
         }
-        if (dead) a.b.ended[slot[1]] = i;
+        if (dead) a.b.free_list[0][slot[1]] = i;
     }
 }
 
 // Regeneration: each free slot of list fidx takes the window's next path
 // (while any is left); camera rays to extension queue `next`, padding slots
-// to the ended list.
+// (radiance 0) straight to the other free list.
 __global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t fidx, uint32_t next)
 {
     const uint32_t nf = a.b.counters[kCntFree0 + fidx];
@@ -328,34 +337,15 @@ __global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t f
         Ray ray{};
         const bool live = has && start_path(a, g0, gs[0], slot, ray);
         const bool dead = has && !live;
+        if (dead) retire(a, slot, make_float4(0.f, 0.f, 0.f, 0.f));
         uint32_t q[2];
-        block_append<2>({live, dead}, {&a.b.counters[next], &a.b.counters[kCntEnded]}, q);
+        block_append<2>({live, dead}, {&a.b.counters[next], &a.b.counters[kCntFree0 + (fidx ^ 1u)]}, q);
         if (live) {
             a.b.ext_rays[next][q[0]] = ray;
             a.b.ext_path[next][q[0]] = slot;
             a.b.ext_scene[next][q[0]] = a.jobs[a.b.p_job[slot]].scene_slot;
         }
-        if (dead) a.b.ended[q[1]] = slot;
-    }
-}
-
-// Paths that ended this iteration (their shadow rays resolved): radiance to
-// the window record, slot to free list fidx.
-__global__ __launch_bounds__(256) void k_finalize(WfArgs a, uint32_t fidx)
-{
-    const uint32_t n = a.b.counters[kCntEnded];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const uint32_t i = base + threadIdx.x;
-        const bool v = i < n;
-        if (v) {
-            const uint32_t slot = a.b.ended[i];
-            const float4 L = a.b.p_L[slot];
-            a.b.w_L[a.b.p_gid[slot]] = make_float4(L.x, L.y, L.z, 0.0f);
-        }
-        uint32_t q[2];
-        block_append<2>({v, v}, {&a.b.counters[kCntFree0 + fidx], &a.b.counters[kCntDone]}, q);
-        if (v) a.b.free_list[fidx][q[0]] = a.b.ended[i];
+        if (dead) a.b.free_list[fidx ^ 1u][q[1]] = slot;
     }
 }
 
@@ -371,8 +361,8 @@ __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
 }
 
 // Zeroes the counters one iteration appends to: the next extension queue,
-// the shadow queue, the traversal work counters, the ended list and the free
-// list this iteration's finalize fills.
+// the shadow queue, the traversal work counters and the free list this
+// iteration's retired slots go to.
 __global__ void k_prep(uint32_t* counters, uint32_t next, uint32_t free_zero)
 {
     if (threadIdx.x == 0) {
@@ -380,12 +370,11 @@ __global__ void k_prep(uint32_t* counters, uint32_t next, uint32_t free_zero)
         counters[kCntShadow] = 0u;
         counters[kCntWorkExt] = 0u;
         counters[kCntWorkShadow] = 0u;
-        counters[kCntEnded] = 0u;
         counters[kCntFree0 + free_zero] = 0u;
     }
 }
 
-__global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur)
+__global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t fw)
 {
     const uint32_t n = a.b.counters[cur];
     const uint32_t nxt = cur ^ 1u;
@@ -557,16 +546,19 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur)
                 }
             }
             a.b.p_L[path] = make_float4(L.x, L.y, L.z, __uint_as_float(ncode));
+            // a path that does not continue has ended: retired here, or by the
+            // shadow-ray resolve when its last light sample is still in flight
+            if (!cont && !shadow) retire(a, path, make_float4(L.x, L.y, L.z, 0.0f));
         }
-        // one block-aggregated append for the four lists; a path that does not
-        // continue has ended (its shadow ray, if any, resolves before finalize)
-        const bool ended = i < n && !cont;
+        // one block-aggregated append for the four lists
+        const bool freed = i < n && !cont && !shadow;
         uint32_t slots[4];
-        block_append<4>({cont, query, shadow, ended},
-                        {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow], &a.b.counters[kCntEnded]},
+        block_append<4>({cont, query, shadow, freed},
+                        {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow],
+                         &a.b.counters[kCntFree0 + fw]},
                         slots);
         const uint32_t es = slots[0], qs = slots[1], ss = slots[2];
-        if (ended) a.b.ended[slots[3]] = path;
+        if (freed) a.b.free_list[fw][slots[3]] = path;
         if (cont) {
             a.b.ext_rays[nxt][es] = next_ray;
             a.b.ext_path[nxt][es] = path;
@@ -581,22 +573,40 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur)
             a.b.sh_rays[ss] = sh_ray;
             a.b.sh_path[ss] = path;
             a.b.sh_scene[ss] = scene_slot;
-            a.b.sh_contrib[ss] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, 0.f);
+            // w = 1: the path ended at this vertex, the resolve retires it
+            a.b.sh_contrib[ss] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, cont ? 0.f : 1.f);
         }
     }
 }
 
-// One shadow ray per path per bounce, so the adds below never collide.
-__global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a)
+// One shadow ray per path per bounce, so the adds below never collide.  A
+// path whose last vertex cast this ray is retired after the add, and its slot
+// goes to free list fw.
+__global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a, uint32_t fw)
 {
     const uint32_t n = a.b.counters[kCntShadow];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (__float_as_uint(a.b.sh_hits[i].y) != 0xFFFFFFFFu) continue;
-        const uint32_t path = a.b.sh_path[i];
-        const float4 c = a.b.sh_contrib[i];
-        float4 L = a.b.p_L[path];
-        L.x += c.x; L.y += c.y; L.z += c.z;
-        a.b.p_L[path] = L;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        bool freed = false;
+        uint32_t path = 0;
+        if (i < n) {
+            path = a.b.sh_path[i];
+            const float4 c = a.b.sh_contrib[i];
+            const bool lit = __float_as_uint(a.b.sh_hits[i].y) == 0xFFFFFFFFu;
+            freed = c.w != 0.0f;
+            if (lit || freed) {
+                float4 L = a.b.p_L[path];
+                if (lit) {
+                    L.x += c.x; L.y += c.y; L.z += c.z;
+                    a.b.p_L[path] = L;
+                }
+                if (freed) retire(a, path, L);
+            }
+        }
+        uint32_t q[1];
+        block_append<1>({freed}, {&a.b.counters[kCntFree0 + fw]}, q);
+        if (freed) a.b.free_list[fw][q[0]] = path;
     }
 }
 
@@ -721,10 +731,7 @@ void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, ui
     hipLaunchKernelGGL(k_regen, dim3(grid), dim3(256), 0, s, a, g0, fidx, next);
 }
 
-void launch_finalize(const WfArgs& a, uint32_t fidx, uint32_t grid, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_finalize, dim3(grid), dim3(256), 0, s, a, fidx);
-}
+
 
 // One pixel per thread: mean = sum * inv (the same product as the FP32
 // readback), times exposure; each channel's byte is the number of the 255
@@ -763,14 +770,14 @@ void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t
     hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, next, free_zero);
 }
 
-void launch_shade(const WfArgs& a, uint32_t cur, uint32_t grid, hipStream_t s)
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a, cur);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, a, cur, fw);
 }
 
-void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s)
+void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(256), 0, s, a, fw);
 }
 
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)

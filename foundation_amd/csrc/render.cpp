@@ -27,16 +27,18 @@ using namespace fnd;
 
 namespace fnd {
 
-// Drain polling: after iteration k the window's finished-path count is
-// copied to pinned host memory; the host reads iteration k - kPollLag's copy
-// before it queues iteration k + 1, so the device always has kPollLag
-// iterations queued and at most kPollLag empty ones run after the window drains.
+// Drain polling: after iteration k the counters are copied to pinned host
+// memory; the host reads iteration k - kPollLag's copy before it queues
+// iteration k + 1, so the device always has kPollLag iterations queued and at
+// most kPollLag empty ones run after the window drains.  A window has drained
+// when every path was started and no extension ray is left to shade (a path
+// that ended is retired within its iteration).
 constexpr uint32_t kPollLag = 3;
 constexpr uint32_t kPollRing = kPollLag + 1;
 
 struct RenderState {
     WfBuffers b{};
-    uint32_t* poll_host = nullptr;  // kPollRing words of pinned host memory
+    uint32_t* poll_host = nullptr;  // kPollRing copies of the kCntCount counters, pinned host memory
     hipEvent_t poll_ev[kPollRing] = {};
     fnd_arena frame{};        // reset at every render call, the queues are carved from it
     void* owned = nullptr;    // the library's block (when the caller gives none)
@@ -83,7 +85,7 @@ static bool carve_queues(fnd_arena* ar, uint64_t cap, WfBuffers& b)
     A(&b.p_job, cap); A(&b.p_pixel, cap); A(&b.p_sample, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
     A(&b.p_train, cap); A(&b.p_gid, cap); A(&b.p_bounce, cap);
     A(&b.w_L, cap * kWindowSlots);
-    A(&b.ended, cap); A(&b.free_list[0], cap); A(&b.free_list[1], cap);
+    A(&b.free_list[0], cap); A(&b.free_list[1], cap);
     for (int q = 0; q < 2; ++q) {
         A(&b.ext_rays[q], cap); A(&b.ext_path[q], cap); A(&b.ext_scene[q], cap);
     }
@@ -531,7 +533,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
     // Drain polling resources, made once per render state.
     if (!rs->poll_host) {
         void* h = nullptr;
-        st = check_hip(ctx, hipHostMalloc(&h, kPollRing * sizeof(uint32_t)), "hipHostMalloc");
+        st = check_hip(ctx, hipHostMalloc(&h, kPollRing * kCntCount * sizeof(uint32_t)), "hipHostMalloc");
         if (st != FND_OK) return st;
         rs->poll_host = static_cast<uint32_t*>(h);
         for (auto& e : rs->poll_ev) {
@@ -582,31 +584,29 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             const uint32_t sgrid = std::min(stage_grid, (n0 + 255u) / 256u);  // a window uses n0 slots
             // A window that fits in the slots has no regeneration: slot i holds
             // window path i to its end, so its radiance stays in place and the
-            // bounce count bounds the iterations (no finalize, no polling).
+            // bounce count bounds the iterations (no polling).
             const bool regen = W > slots;
             wa.b.window = W;
             wa.b.w_L = regen ? rs->b.w_L : rs->b.p_L;
             uint32_t cur = kCntExtA;
-            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntDone) | (1u << kCntFree0),
-                              ctx->stream);
+            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntFree0), ctx->stream);
             launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter
             launch_generate(wa, g0, n0, sgrid, ctx->stream);
             fnd_status e = trace(cur, "primary trace");
             if (e != FND_OK) return e;
             if (!regen) {
                 for (uint32_t k = 0; k <= depth; ++k) {
-                    const uint32_t nxt = cur ^ 1u;
-                    launch_prep(wa, nxt, (k & 1u) ^ 1u, ctx->stream);
-                    launch_shade(wa, cur, sgrid, ctx->stream);
+                    const uint32_t nxt = cur ^ 1u, fw = (k & 1u) ^ 1u;  // (the free lists go unused)
+                    launch_prep(wa, nxt, fw, ctx->stream);
+                    launch_shade(wa, cur, fw, sgrid, ctx->stream);
                     if (k < depth) {  // the last shade appends no rays
                         e = trace(nxt, "extension + shadow trace");
                         if (e != FND_OK) return e;
-                        launch_shadow_resolve(wa, sgrid, ctx->stream);
+                        launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
                     }
                     cur = nxt;
                 }
             } else {
-                launch_finalize(wa, 0u, sgrid, ctx->stream);  // padding slots of the first paths
                 // every path is shaded at most depth + 1 times, and while paths are
                 // left no slot stays idle, so the window drains within this many
                 // iterations (plus the polling lag)
@@ -620,15 +620,15 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                     const uint32_t nxt = cur ^ 1u;
                     const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
                     launch_prep(wa, nxt, fw, ctx->stream);
-                    launch_shade(wa, cur, sgrid, ctx->stream);
+                    launch_shade(wa, cur, fw, sgrid, ctx->stream);
                     launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
                     e = trace(nxt, "extension + shadow trace");
                     if (e != FND_OK) return e;
-                    launch_shadow_resolve(wa, sgrid, ctx->stream);
-                    launch_finalize(wa, fw, sgrid, ctx->stream);
+                    launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
                     const uint32_t ring = uint32_t(k % kPollRing);
-                    e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring, &rs->b.counters[kCntDone], sizeof(uint32_t),
-                                                      hipMemcpyDeviceToHost, ctx->stream), "drain poll");
+                    e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount, rs->b.counters,
+                                                      kCntCount * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+                                  "drain poll");
                     if (e == FND_OK) e = check_hip(ctx, hipEventRecord(rs->poll_ev[ring], ctx->stream), "drain poll");
                     if (e != FND_OK) return e;
                     cur = nxt;
@@ -636,7 +636,10 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                         const uint32_t old = uint32_t((k - kPollLag) % kPollRing);
                         e = check_hip(ctx, hipEventSynchronize(rs->poll_ev[old]), "drain poll");
                         if (e != FND_OK) return e;
-                        drained = rs->poll_host[old] >= W;
+                        // queue `old_next` is what iteration k - kPollLag left to shade
+                        const uint32_t* c = rs->poll_host + old * kCntCount;
+                        const uint32_t old_next = uint32_t(((k - kPollLag) & 1u) ? kCntExtA : kCntExtB);
+                        drained = c[old_next] == 0u && c[kCntNext] >= W;
                     }
                 }
             }

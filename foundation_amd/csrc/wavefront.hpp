@@ -71,10 +71,8 @@ enum WfCounter : uint32_t {
     kCntQuery = 5,
     kCntTrain = 6,
     kCntNext = 7,    // next window path to start (may pass the window's end)
-    kCntDone = 8,    // window paths finished
-    kCntEnded = 9,   // slots whose path ended this iteration
-    kCntFree0 = 10,  // free slots, two lists used alternately
-    kCntFree1 = 11,
+    kCntFree0 = 8,   // free slots, two lists used alternately
+    kCntFree1 = 9,
     kCntCount = 16
 };
 
@@ -97,9 +95,9 @@ struct WfBuffers {
     // per window path: final radiance (cache term added after the window drains)
     float4* w_L;
     uint32_t window;          // paths in the current window
-    // slot lists
-    uint32_t* ended;          // slots whose path ended this iteration (kCntEnded)
-    uint32_t* free_list[2];   // slots free for the next iteration (kCntFree0 + i)
+    // slots whose path has ended and been recorded, free for the next
+    // iteration (kCntFree0 + i; the two lists alternate)
+    uint32_t* free_list[2];
     // extension queues (double-buffered) and hits
     Ray* ext_rays[2];
     uint32_t* ext_path[2];
@@ -140,12 +138,11 @@ struct WfArgs {
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, uint32_t grid, hipStream_t s);
-void launch_finalize(const WfArgs& a, uint32_t fidx, uint32_t grid, hipStream_t s);
 void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
                     uint32_t* out, hipStream_t s);
 void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s);
-void launch_shade(const WfArgs& a, uint32_t cur, uint32_t grid, hipStream_t s);
-void launch_shadow_resolve(const WfArgs& a, uint32_t grid, hipStream_t s);
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t grid, hipStream_t s);
+void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t grid, hipStream_t s);
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s);
