@@ -15,6 +15,10 @@ instance, counted by the kernel's counting variant over the same rays) over
 its mean launch time from HIP events; the CPU oracle timed on the same rays
 on the host cores (cpu_baseline); and hit-record parity GPU vs oracle.
 
+extra.traversal_1_5M: the traversal microbenchmark configuration (1.31M-
+triangle displaced icosphere + 200k soup; 1080p coherent, incoherent and
+shadow ray sets) with its own hit-record parity against the oracle.
+
 `value` is the first half of the metric (Mrays/s).  The second half is in
 extra.time_to_rel_mse_0_01: progressive 1 spp/frame path tracing of the
 displaced-sphere scene (1.31M-triangle icosphere + 200k soup, 4 quad lights,
@@ -138,6 +142,55 @@ def frame_arena_sweep(ctx, scene, camera, width, height, depth, rr, frames=3):
         ctx.framebuffer_destroy(fb)
     return {"workload": f"{width}x{height}, 1 spp, depth {depth}, RR from {rr}, 10.08M tris",
             "queue_bytes_per_path": int(pt.queue_bytes(1 << 16) - pt.queue_bytes(1 << 15)) >> 15, "rows": rows}
+
+
+def traversal_1_5m(ctx, steps, warmup, threads, with_oracle):
+    """The traversal microbenchmark configuration: a 1.31M-triangle displaced
+    icosphere (8 subdivisions, 3-octave value-noise displacement of amplitude
+    0.1) plus 200k random-soup triangles, single-level BVH; 1920x1080 coherent
+    primary rays, the incoherent cosine bounce rays from their first hits, and
+    shadow rays from those hits to the centre of the first quad light.  GPU
+    Mrays/s per set (closest hit for primary and incoherent, any hit for
+    shadow) and, with the oracle, hit-record parity on every ray."""
+    import numpy as np
+    from foundation_amd import _abi as A
+    from foundation_amd import pt
+    from foundation_amd import rays as R
+    gen = pt.GeneratedScene(A.GEN_DISPLACED_SPHERE)
+    scene = ctx.scene_create(gen.desc)
+    try:
+        info = ctx.scene_info(scene)
+        primary = R.primary_rays(gen.camera, 1920, 1080)
+        prim_hits = ctx.trace(scene, primary)
+        lookup = gen.triangle_lookup()
+        incoherent, _ = R.bounce_rays(primary, prim_hits, lookup, seed=1)
+        lt = gen.desc.lights[0]
+        centre = [lt.corner[k] + 0.5 * (lt.edge0[k] + lt.edge1[k]) for k in range(3)]
+        shadow, _ = R.shadow_rays(primary, prim_hits, lookup, centre)
+        out = {"scene": "displaced icosphere (1.31M tris) + 200k soup, single-level BVH",
+               "triangles": int(info.num_triangles)}
+        hits = {}
+        for name, rays, flags in (("coherent", primary, 0), ("incoherent", incoherent, 0),
+                                  ("shadow_anyhit", shadow, A.TRACE_ANYHIT)):
+            wall, kernel_ms, h, _ = time_gpu_set(ctx, scene, rays, steps, warmup, flags)
+            out[f"{name}_rays"] = int(rays.shape[0])
+            out[f"{name}_mrays_s"] = round(rays.shape[0] * steps / wall / 1e6, 1)
+            hits[name] = h
+        if with_oracle:
+            from oracle import pt_oracle as O
+            oscene = O.scene_from_generated(gen, threads=threads)
+            par = {}
+            for name, rays in (("coherent", primary), ("incoherent", incoherent)):
+                ref = oscene.trace(rays, threads=threads)
+                par[f"{name}_record_mismatches"] = int(np.any(hits[name].view(np.uint32) != ref.view(np.uint32),
+                                                              axis=1).sum())
+            occ = hits["shadow_anyhit"].view(np.uint32)[:, 1] != 0xFFFFFFFF
+            oocc = oscene.trace(shadow, anyhit=True, threads=threads).view(np.uint32)[:, 1] != 0xFFFFFFFF
+            par["anyhit_mismatches"] = int((occ != oocc).sum())
+            out["parity"] = par
+        return out
+    finally:
+        ctx.scene_destroy(scene)
 
 
 def batch64(ctx, count, spp, depth, cache_path):
@@ -413,6 +466,12 @@ def main():
                   "shadow_rays": int(shadow.shape[0]), "anyhit_mismatches": int((socc != oocc).sum())}
         log(f"cpu: {cpu['value']} Mrays/s on {threads} threads; parity {parity}")
 
+    micro = None
+    if not args.only_incoherent:
+        micro = traversal_1_5m(ctx, max(2, args.steps // 2), args.warmup,
+                               args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu)
+        log(f"traversal 1.5M tris: {micro}")
+
     render = relmse = batch = arena = None
     if not args.no_render and not args.only_incoherent:
         render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 64, 8, 3)
@@ -458,7 +517,7 @@ def main():
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
                   "trace_valu_issue": valu,
                   "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
-                  "path_tracing_10M": render, "frame_arena_10M": arena, "time_to_rel_mse_0_01": relmse,
+                  "traversal_1_5M": micro, "path_tracing_10M": render, "frame_arena_10M": arena, "time_to_rel_mse_0_01": relmse,
                   "batch_64_scenes": batch},
     }
     print(json.dumps(line), flush=True)
