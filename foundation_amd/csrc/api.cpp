@@ -275,6 +275,8 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
     ctx->stats.num_cus = ctx->num_cus;
     st = check_hip(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (st != FND_OK) return fail(st);
+    st = check_hip(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreate");
+    if (st != FND_OK) return fail(st);
     for (auto& ev : ctx->ev) {
         st = check_hip(ctx, hipEventCreate(&ev), "hipEventCreate");
         if (st != FND_OK) return fail(st);
@@ -320,6 +322,10 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : ctx->timing_events) (void)hipEventDestroy(ev);
     ctx->timing_events.clear();
+    if (ctx->side) {
+        (void)hipStreamSynchronize(ctx->side);
+        (void)hipStreamDestroy(ctx->side);
+    }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     const fnd_host_allocator a = ctx->alloc;
     const bool has = ctx->has_alloc;

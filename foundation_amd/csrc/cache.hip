@@ -142,16 +142,17 @@ constexpr int kP = 1;  // 16-sample tiles per wave
 template <bool TRAIN>
 __global__ __launch_bounds__(256) void k_cache_forward(CacheView v, const CacheQuery* q, uint32_t qstride,
                                                        uint32_t n_arg, const uint32_t* n_dev, float4* pred,
-                                                       uint16_t* act, uint32_t stride)
+                                                       uint16_t* act, uint32_t stride, const uint32_t* lo_dev)
 {
     __shared__ h8 wl[kMlpFwdElems / 8];
     for (int i = threadIdx.x; i < kMlpFwdElems / 8; i += 256) wl[i] = reinterpret_cast<const h8*>(v.wfwd)[i];
     __syncthreads();
     const _Float16* w = reinterpret_cast<const _Float16*>(wl);
     const uint32_t n = n_dev ? *n_dev : n_arg;
+    const uint32_t lo = lo_dev ? *lo_dev : 0u;
     const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
     const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), nwaves = gridDim.x * 4;
-    for (uint32_t base = wave * 16u * kP; base < n; base += nwaves * 16u * kP) {
+    for (uint32_t base = lo + wave * 16u * kP; base < n; base += nwaves * 16u * kP) {
         h8 bh[kP];
         h8 bx[kP];
 #pragma unroll
@@ -513,13 +514,15 @@ __global__ __launch_bounds__(256) void k_zero(float* p, uint64_t n)
 }  // namespace
 
 void launch_cache_forward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint32_t* n_dev,
-                          float4* pred, uint16_t* act, uint32_t act_stride, uint32_t grid, hipStream_t s)
+                          float4* pred, uint16_t* act, uint32_t act_stride, uint32_t grid, hipStream_t s,
+                          const uint32_t* lo_dev)
 {
     if (act)
         hipLaunchKernelGGL(k_cache_forward<true>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, n_dev, pred, act,
-                           act_stride);
+                           act_stride, lo_dev);
     else
-        hipLaunchKernelGGL(k_cache_forward<false>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, n_dev, pred, act, 0u);
+        hipLaunchKernelGGL(k_cache_forward<false>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, n_dev, pred, act, 0u,
+                           lo_dev);
 }
 
 void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstride, uint32_t n, uint16_t* dz,

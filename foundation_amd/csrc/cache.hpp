@@ -78,9 +78,11 @@ struct CacheTrainBufs {
     uint32_t cap;               // samples the buffers hold (multiple of 32)
 };
 
-// qstride / tstride: sample s reads q[s * qstride] (and target[s * tstride])
+// qstride / tstride: sample s reads q[s * qstride] (and target[s * tstride]).
+// The samples are [*lo_dev, *n_dev) when those are given, else [0, n).
 void launch_cache_forward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint32_t* n_dev,
-                          float4* pred, uint16_t* act, uint32_t act_stride, uint32_t grid, hipStream_t s);
+                          float4* pred, uint16_t* act, uint32_t act_stride, uint32_t grid, hipStream_t s,
+                          const uint32_t* lo_dev = nullptr);
 void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstride, uint32_t n, uint16_t* dz,
                        uint32_t stride, float* loss, hipStream_t s);
 void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint16_t* act,

@@ -57,6 +57,7 @@ struct fnd_pt_ctx {
     void* log_user = nullptr;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                // radiance-cache inference overlapped with a frame's last bounces
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<hipEvent_t> timing_events;     // pairs (start, end) per pending trace launch
     size_t timing_used = 0;                    // events in use (even)

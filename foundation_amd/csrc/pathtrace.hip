@@ -354,6 +354,11 @@ __global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+__global__ void k_snapshot(uint32_t* counters, uint32_t src, uint32_t dst)
+{
+    if (threadIdx.x == 0) counters[dst] = counters[src];
+}
+
 // Zeroes the counter slots whose bit is set in mask.
 __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
 {
@@ -714,6 +719,11 @@ void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s)
 void launch_cache_collect(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)
 {
     hipLaunchKernelGGL(k_cache_collect, dim3(grid), dim3(256), 0, s, a, g0, n);
+}
+
+void launch_snapshot(const WfArgs& a, uint32_t src, uint32_t dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, s, a.b.counters, src, dst);
 }
 
 void launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s)

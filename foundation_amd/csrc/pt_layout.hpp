@@ -25,7 +25,11 @@
 namespace fnd {
 
 constexpr uint32_t kWideArity = 4;
-constexpr uint32_t kLeafMax = 4;          // triangles per leaf child
+// Triangles per leaf child: one.  Against leaves of up to 4, the traversal
+// tests 2.5x (10M instanced spheres) to 6x (displaced sphere + soup) fewer
+// triangles for 5-15% more node visits and runs 8-31% faster; the tree has
+// 3x the nodes (340 MB at 10M triangles).
+constexpr uint32_t kLeafMax = 1;
 constexpr uint32_t kNodeFlagTop = 1u;     // leaf children are instances
 constexpr uint32_t kMaxTriangles = 1u << 28;
 constexpr uint32_t kMaxNodes = 1u << 30;

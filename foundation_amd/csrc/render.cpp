@@ -40,6 +40,7 @@ struct RenderState {
     WfBuffers b{};
     uint32_t* poll_host = nullptr;  // kPollRing copies of the kCntCount counters, pinned host memory
     hipEvent_t poll_ev[kPollRing] = {};
+    hipEvent_t side_ev[2] = {};     // early cache queries ready / answered
     fnd_arena frame{};        // reset at every render call, the queues are carved from it
     void* owned = nullptr;    // the library's block (when the caller gives none)
     uint64_t owned_bytes = 0;
@@ -58,6 +59,8 @@ void render_state_free(fnd_pt_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     if (rs->poll_host) (void)hipHostFree(rs->poll_host);
     for (auto& e : rs->poll_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : rs->side_ev)
         if (e) (void)hipEventDestroy(e);
     dev_free(ctx, rs->owned, rs->owned_bytes);
     dev_free(ctx, rs->d_jobs, rs->jobs_cap * sizeof(JobDev));
@@ -540,6 +543,10 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
             if (st != FND_OK) return st;
         }
+        for (auto& e : rs->side_ev) {
+            st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            if (st != FND_OK) return st;
+        }
     }
 
     // One pass of the wavefront over a job table already on the device, window
@@ -594,11 +601,30 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             launch_generate(wa, g0, n0, sgrid, ctx->stream);
             fnd_status e = trace(cur, "primary trace");
             if (e != FND_OK) return e;
+            const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (W + 127u) / 128u));
+            bool early = false;  // queries answered on the side stream
             if (!regen) {
+                // After the shade of bounce 1 nearly every path that ends in the
+                // radiance cache has queried it.  Those queries are answered on
+                // the side stream while the main stream traces the last bounces,
+                // launches that hold few rays and leave most CUs idle; the rest
+                // are answered after the loop, on the main stream.
+                const uint32_t ksplit = std::min(1u, depth);
                 for (uint32_t k = 0; k <= depth; ++k) {
                     const uint32_t nxt = cur ^ 1u, fw = (k & 1u) ^ 1u;  // (the free lists go unused)
                     launch_prep(wa, nxt, fw, ctx->stream);
                     launch_shade(wa, cur, fw, sgrid, ctx->stream);
+                    if (apply_cache && k == ksplit && k < depth) {
+                        launch_snapshot(wa, kCntQuery, kCntQuerySplit, ctx->stream);
+                        e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
+                        if (e == FND_OK) e = check_hip(ctx, hipStreamWaitEvent(ctx->side, rs->side_ev[0], 0), "side stream");
+                        if (e != FND_OK) return e;
+                        launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuerySplit], rs->b.q_pred,
+                                             nullptr, 0, cg, ctx->side);
+                        e = check_hip(ctx, hipEventRecord(rs->side_ev[1], ctx->side), "side stream");
+                        if (e != FND_OK) return e;
+                        early = true;
+                    }
                     if (k < depth) {  // the last shade appends no rays
                         e = trace(nxt, "extension + shadow trace");
                         if (e != FND_OK) return e;
@@ -644,9 +670,13 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                 }
             }
             if (apply_cache) {
-                const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (W + 127u) / 128u));
+                // the queries not answered early: [split, count), then wait for the early ones
                 launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
-                                     cg, ctx->stream);
+                                     cg, ctx->stream, early ? &rs->b.counters[kCntQuerySplit] : nullptr);
+                if (early) {
+                    e = check_hip(ctx, hipStreamWaitEvent(ctx->stream, rs->side_ev[1], 0), "side stream");
+                    if (e != FND_OK) return e;
+                }
                 launch_cache_apply(wa, wgrid, ctx->stream);
             }
             if (collect)

@@ -73,6 +73,7 @@ enum WfCounter : uint32_t {
     kCntNext = 7,    // next window path to start (may pass the window's end)
     kCntFree0 = 8,   // free slots, two lists used alternately
     kCntFree1 = 9,
+    kCntQuerySplit = 10,  // radiance-cache queries answered early, on the side stream
     kCntCount = 16
 };
 
@@ -148,5 +149,7 @@ void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s);
 void launch_cache_collect(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
+// counters[dst] = counters[src]
+void launch_snapshot(const WfArgs& a, uint32_t src, uint32_t dst, hipStream_t s);
 
 }  // namespace fnd

@@ -29,7 +29,9 @@
  *     the ray-query entry used by the traversal benchmark and the parity tests.
  *
  * Threading: a context is single-threaded, like the reference's one render
- * thread (src/Editor/Editor.cpp:20-23).  One context owns one HIP stream.
+ * thread (src/Editor/Editor.cpp:20-23).  One context owns one HIP stream for
+ * its work, plus a second one that only answers radiance-cache queries during
+ * a frame's last bounces (joined before fnd_pt_render returns).
  * Calls are synchronous unless FND_ASYNC is set, then fnd_pt_wait is required.
  */
 #ifndef FND_PT_H
