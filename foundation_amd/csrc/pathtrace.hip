@@ -553,10 +553,10 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             a.b.p_L[path] = make_float4(L.x, L.y, L.z, __uint_as_float(ncode));
             // a path that does not continue has ended: retired here, or by the
             // shadow-ray resolve when its last light sample is still in flight
-            if (!cont && !shadow) retire(a, path, make_float4(L.x, L.y, L.z, 0.0f));
+            if (a.b.retire && !cont && !shadow) retire(a, path, make_float4(L.x, L.y, L.z, 0.0f));
         }
         // one block-aggregated append for the four lists
-        const bool freed = i < n && !cont && !shadow;
+        const bool freed = a.b.retire && i < n && !cont && !shadow;
         uint32_t slots[4];
         block_append<4>({cont, query, shadow, freed},
                         {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow],
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             a.b.sh_path[ss] = path;
             a.b.sh_scene[ss] = scene_slot;
             // w = 1: the path ended at this vertex, the resolve retires it
-            a.b.sh_contrib[ss] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, cont ? 0.f : 1.f);
+            a.b.sh_contrib[ss] = make_float4(sh_contrib.x, sh_contrib.y, sh_contrib.z, (a.b.retire && !cont) ? 1.f : 0.f);
         }
     }
 }
@@ -590,6 +590,17 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
 __global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a, uint32_t fw)
 {
     const uint32_t n = a.b.counters[kCntShadow];
+    if (!a.b.retire) {  // a window without regeneration: paths stay in their slots
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+            if (__float_as_uint(a.b.sh_hits[i].y) != 0xFFFFFFFFu) continue;
+            const uint32_t path = a.b.sh_path[i];
+            const float4 c = a.b.sh_contrib[i];
+            float4 L = a.b.p_L[path];
+            L.x += c.x; L.y += c.y; L.z += c.z;
+            a.b.p_L[path] = L;
+        }
+        return;
+    }
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;

@@ -595,6 +595,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             const bool regen = W > slots;
             wa.b.window = W;
             wa.b.w_L = regen ? rs->b.w_L : rs->b.p_L;
+            wa.b.retire = regen ? 1u : 0u;
             uint32_t cur = kCntExtA;
             launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntFree0), ctx->stream);
             launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter

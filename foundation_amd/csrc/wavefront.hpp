@@ -96,6 +96,7 @@ struct WfBuffers {
     // per window path: final radiance (cache term added after the window drains)
     float4* w_L;
     uint32_t window;          // paths in the current window
+    uint32_t retire;          // 1: the window regenerates, ended paths are retired and their slots freed
     // slots whose path has ended and been recorded, free for the next
     // iteration (kCntFree0 + i; the two lists alternate)
     uint32_t* free_list[2];
