@@ -149,6 +149,45 @@ def rel_mse(x, y):
     return float(np.mean((x.astype(np.float64) - y) ** 2 / (y.astype(np.float64) ** 2 + 0.01)))
 
 
+def test_cache_queries_on_two_streams_match_one_window(ctx, params):
+    """Paths that end in the radiance cache (fixed parameters, no training)
+    have most of their queries answered on a second stream while the frame's
+    last bounces trace, and the rest after the bounce loop.  The image is the
+    same bit for bit from run to run, and with 1024 slots (twelve windows that
+    reuse the query buffers) as with one window; and it differs from plain
+    path tracing, so the cache was used."""
+    mlp, grid, _, _ = params
+    g = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    scene = ctx.scene_create(g.desc)
+    w, h = 64, 48
+    old = ctx.tuning()
+
+    def draw(flags):
+        fb = ctx.framebuffer_create(w, h)
+        try:
+            ctx.render([pt.make_job(scene, fb, g.camera, 4, 6, seed=7, flags=flags)])
+            return ctx.framebuffer_read(fb, w, h)[0]
+        finally:
+            ctx.framebuffer_destroy(fb)
+
+    try:
+        configure(ctx, lo=(0, 0, 0), hi=(0, 0, 0))  # bounds from the scene
+        set_params(ctx, mlp, grid)
+        a = draw(A.JOB_USE_CACHE)
+        b = draw(A.JOB_USE_CACHE)
+        ctx.set_tuning(queue_capacity=1024)
+        c = draw(A.JOB_USE_CACHE)
+        ctx.set_tuning(queue_capacity=old.queue_capacity)
+        p = draw(0)
+        assert np.all(np.isfinite(a))
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+        assert not np.array_equal(a.view(np.uint32), p.view(np.uint32))
+    finally:
+        ctx.set_tuning(queue_capacity=old.queue_capacity)
+        ctx.scene_destroy(scene)
+
+
 def test_cache_beats_plain_path_tracing_at_equal_samples(ctx):
     """Cornell box + soup: 32 frames of 1 spp with the cache (trained every
     frame on paths the tracer generates) against 32 spp of plain path
