@@ -315,3 +315,29 @@ def test_library_block_is_allocated_once_per_size(ctx, cornell):
         _render(ctx, s, g.camera, 32, 32, 1, 2, seed=seed)
         assert ctx.frame_arena_info().size == size
         assert ctx.stats().device_bytes == before
+
+
+@pytest.mark.gpu
+def test_device_memory_budget_fails_cleanly():
+    """A context's device-memory budget is enforced as FND_E_OOM_DEVICE, with
+    no crash, and the context stays usable: a scene larger than the budget
+    is refused, and so is a frame whose ray queues exceed it, after which a
+    small frame still renders."""
+    g = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    with pt.Context(0, device_memory_budget=256 << 10) as c:  # 10,034 triangles x 48 B do not fit
+        with pytest.raises(pt.FndError) as e:
+            c.scene_create(g.desc)
+        assert e.value.status == A.E_OOM_DEVICE and "budget" in e.value.message
+    # 128 MiB: the scene, a 1080p framebuffer (24.9 MB) and the traversal
+    # stacks' spill area for the whole persistent grid (about 60 MiB here) fit;
+    # the ray queues of a 1080p 1-spp frame (878 MiB) do not
+    with pt.Context(0, device_memory_budget=128 << 20) as c:
+        s = c.scene_create(g.desc)
+        fb = c.framebuffer_create(1920, 1080)
+        with pytest.raises(pt.FndError) as e:
+            c.render([pt.make_job(s, fb, g.camera, 1, 4)])
+        assert e.value.status == A.E_OOM_DEVICE
+        small = c.framebuffer_create(32, 32)
+        c.render([pt.make_job(s, small, g.camera, 1, 4)])
+        img, n = c.framebuffer_read(small, 32, 32)
+        assert n == 1 and np.all(np.isfinite(img)) and img.max() > 0
