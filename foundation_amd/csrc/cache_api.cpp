@@ -58,6 +58,7 @@ void cache_free(fnd_pt_ctx* ctx)
     auto* cs = static_cast<CacheState*>(ctx->cache);
     if (!cs) return;
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);  // inference may read the weights there
     for (auto& a : cs->allocs) dev_free(ctx, a.first, a.second);
     ctx_delete(ctx, cs);
     ctx->cache = nullptr;

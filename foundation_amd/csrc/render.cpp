@@ -57,6 +57,7 @@ void render_state_free(fnd_pt_ctx* ctx)
     auto* rs = static_cast<RenderState*>(ctx->render_state);
     if (!rs) return;
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     if (rs->poll_host) (void)hipHostFree(rs->poll_host);
     for (auto& e : rs->poll_ev)
         if (e) (void)hipEventDestroy(e);
@@ -398,6 +399,9 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
 {
     if (!ctx || (!jobs && num_jobs)) return FND_E_INVALID_ARG;
     if (num_jobs == 0) return FND_OK;
+    // a call that failed after queueing cache queries on the side stream may
+    // have left them running; they read the queues this call is about to reuse
+    (void)hipStreamSynchronize(ctx->side);
     const uint32_t T = std::max(1u, ctx->tuning.tile);
     std::map<Scene*, uint32_t> slot_of;
     std::vector<Scene*> slots;
@@ -761,6 +765,7 @@ fnd_status fnd_pt_set_frame_arena(fnd_pt_ctx* ctx, void* memory, uint64_t bytes)
                          (unsigned long long)kQueueAlign);
     if (memory && bytes == 0) return set_error(ctx, FND_E_INVALID_ARG, "frame arena of 0 bytes");
     fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "set frame arena");
+    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->side), "set frame arena");
     if (st != FND_OK) return st;
     if (memory) {  // the library's block is no longer used: give it back
         if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
