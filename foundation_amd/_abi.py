@@ -315,6 +315,7 @@ PROTOTYPES = {
     "fnd_pt_framebuffer_read": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_read_f64": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_read_rgba8": (_S, [_P, _H, C.c_float, _P, C.POINTER(C.c_uint32)]),
+    "fnd_pt_framebuffer_present": (_S, [_P, _H, C.c_float, _P, C.c_uint64, C.POINTER(C.c_uint32)]),
     "fnd_pt_render": (_S, [_P, C.POINTER(RenderJob), C.c_uint32]),
     "fnd_arena_reset": (None, [C.POINTER(Arena), _P, C.c_uint64]),
     "fnd_arena_alloc": (_P, [C.POINTER(Arena), C.c_uint64]),

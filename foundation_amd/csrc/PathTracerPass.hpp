@@ -232,6 +232,17 @@ public:
         return fnd_pt_queue_bytes(paths, &b) == FND_OK ? b : 0;
     }
 
+    // The same bytes written into device memory of the caller's (for instance
+    // the swapchain image's memory imported into HIP), rows row_pitch bytes
+    // apart: the frame reaches the presentation engine without host staging
+    // (fnd_pt_framebuffer_present).
+    void PresentTo(void* device_rgba, uint64_t row_pitch, float exposure = 1.0f)
+    {
+        uint32_t n = 0;
+        check(fnd_pt_framebuffer_present(ctx_, fb_, exposure, device_rgba, row_pitch, &n),
+              "fnd_pt_framebuffer_present");
+    }
+
     uint32_t Samples() const { return samples_; }
     const Desc& Description() const { return desc_; }
     fnd_pt_ctx* Handle() const { return ctx_; }

@@ -758,7 +758,8 @@ void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, ui
 // readback), times exposure; each channel's byte is the number of the 255
 // ascending thresholds at or below it (binary search; NaN and x <= 0 give 0).
 __global__ __launch_bounds__(256) void k_present(const float* accum, uint64_t pixels, float inv, float exposure,
-                                                 const float* thresholds, uint32_t* out)
+                                                 const float* thresholds, uint32_t* out, uint32_t width,
+                                                 uint32_t pitch)
 {
     __shared__ float t[256];
     for (uint32_t k = threadIdx.x; k < 256; k += blockDim.x) t[k] = k < 255 ? thresholds[k] : __builtin_inff();
@@ -775,15 +776,16 @@ __global__ __launch_bounds__(256) void k_present(const float* accum, uint64_t pi
                 if (t[b + step - 1] <= x) b += step;
             word |= b << (8 * c);
         }
-        out[i] = word;
+        out[(i / width) * pitch + i % width] = word;  // row pitch in pixels
     }
 }
 
 void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
-                    uint32_t* out, hipStream_t s)
+                    uint32_t* out, uint32_t width, uint32_t pitch, hipStream_t s)
 {
     const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (pixels + 255) / 256)));
-    hipLaunchKernelGGL(k_present, dim3(grid), dim3(256), 0, s, accum, pixels, inv, exposure, thresholds, out);
+    hipLaunchKernelGGL(k_present, dim3(grid), dim3(256), 0, s, accum, pixels, inv, exposure, thresholds, out, width,
+                       pitch);
 }
 
 void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s)

@@ -250,6 +250,20 @@ void camera_basis(const fnd_camera& c, uint32_t w, uint32_t h, JobDev& j)
     j.aspect = float(double(w) / double(h));
 }
 
+// The 255 byte thresholds ((k - 0.5) / 255)^2.2 of the R8G8B8A8 transfer,
+// computed on the host in double precision and uploaded once per context.
+fnd_status present_table(fnd_pt_ctx* ctx)
+{
+    if (ctx->d_present) return FND_OK;
+    float t[255];
+    for (int k = 1; k <= 255; ++k) t[k - 1] = float(std::pow((double(k) - 0.5) / 255.0, 2.2));
+    void* p = nullptr;
+    fnd_status st = dev_alloc(ctx, &p, sizeof t);
+    if (st != FND_OK) return st;
+    ctx->d_present = static_cast<float*>(p);
+    return copy_sync(ctx, ctx->d_present, t, sizeof t, hipMemcpyHostToDevice, "present table");
+}
+
 }  // namespace
 }  // namespace fnd
 
@@ -349,28 +363,47 @@ fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle h, float ex
     Framebuffer* f = find_framebuffer(ctx, h);
     if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
     const uint64_t pixels = uint64_t(f->width) * f->height;
-    fnd_status st = FND_OK;
-    if (!ctx->d_present) {
-        float t[255];
-        for (int k = 1; k <= 255; ++k) t[k - 1] = float(std::pow((double(k) - 0.5) / 255.0, 2.2));
-        void* p = nullptr;
-        if ((st = dev_alloc(ctx, &p, sizeof t)) != FND_OK) return st;
-        ctx->d_present = static_cast<float*>(p);
-        st = copy_sync(ctx, ctx->d_present, t, sizeof t, hipMemcpyHostToDevice, "present table");
-        if (st != FND_OK) return st;
-    }
+    fnd_status st = present_table(ctx);
+    if (st != FND_OK) return st;
     if (!f->rgba8) {
         void* p = nullptr;
         if ((st = dev_alloc(ctx, &p, pixels * sizeof(uint32_t))) != FND_OK) return st;
         f->rgba8 = static_cast<uint32_t*>(p);
     }
     const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
-    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, f->rgba8, ctx->stream);
+    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, f->rgba8, f->width, f->width, ctx->stream);
     st = check_hip(ctx, hipGetLastError(), "present");
     if (st == FND_OK)
         st = check_hip(ctx, hipMemcpyAsync(rgba, f->rgba8, pixels * 4, hipMemcpyDeviceToHost, ctx->stream),
                        "read framebuffer");
     if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
+    if (st != FND_OK) return st;
+    if (out_samples) *out_samples = f->samples;
+    return FND_OK;
+}
+
+// The same conversion written straight into device memory of the caller's
+// (for instance an image imported from the presentation engine), rows
+// `row_pitch` bytes apart: no copy through host memory.
+fnd_status fnd_pt_framebuffer_present(fnd_pt_ctx* ctx, fnd_handle h, float exposure, void* device_rgba,
+                                      uint64_t row_pitch, uint32_t* out_samples)
+{
+    if (!ctx || !device_rgba || !(exposure >= 0.0f)) return FND_E_INVALID_ARG;
+    Framebuffer* f = find_framebuffer(ctx, h);
+    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+    if (reinterpret_cast<uintptr_t>(device_rgba) % 4 || row_pitch % 4 || row_pitch < 4ull * f->width ||
+        row_pitch / 4 > 0xFFFFFFFFull)
+        return set_error(ctx, FND_E_INVALID_ARG,
+                         "present target must be 4-byte aligned with a row pitch of at least %u bytes, a multiple of 4",
+                         4 * f->width);
+    fnd_status st = present_table(ctx);
+    if (st != FND_OK) return st;
+    const uint64_t pixels = uint64_t(f->width) * f->height;
+    const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
+    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, static_cast<uint32_t*>(device_rgba), f->width,
+                   uint32_t(row_pitch / 4), ctx->stream);
+    st = check_hip(ctx, hipGetLastError(), "present");
+    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "present");
     if (st != FND_OK) return st;
     if (out_samples) *out_samples = f->samples;
     return FND_OK;

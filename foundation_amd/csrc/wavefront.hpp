@@ -140,8 +140,9 @@ struct WfArgs {
 
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, uint32_t grid, hipStream_t s);
+// out[y * pitch + x] for pixel y * width + x (pitch in pixels, >= width)
 void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
-                    uint32_t* out, hipStream_t s);
+                    uint32_t* out, uint32_t width, uint32_t pitch, hipStream_t s);
 void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s);
 void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t grid, hipStream_t s);
 void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t grid, hipStream_t s);

@@ -237,8 +237,29 @@ class DeviceMemory:
             hip.hipMalloc.restype = C.c_int
             hip.hipFree.argtypes = [C.c_void_p]
             hip.hipFree.restype = C.c_int
+            hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            hip.hipMemcpy.restype = C.c_int
             DeviceMemory._hip = hip
         return DeviceMemory._hip
+
+    def write(self, data):
+        """Copies host bytes (a numpy array or bytes) to the start of the block."""
+        buf = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8) if isinstance(data, (bytes, bytearray))
+                                   else data).view(np.uint8).reshape(-1)
+        if buf.size > self.nbytes:
+            raise ValueError("more bytes than the block holds")
+        st = DeviceMemory._hip.hipMemcpy(self.ptr, buf.ctypes.data, buf.size, 1)  # hipMemcpyHostToDevice
+        if st != 0:
+            raise FndError(A.E_DEVICE, f"hipMemcpy failed with HIP error {st}")
+
+    def read(self, nbytes=None):
+        """Copies the first nbytes (all by default) of the block to a uint8 array."""
+        n = self.nbytes if nbytes is None else nbytes
+        out = np.zeros(n, np.uint8)
+        st = DeviceMemory._hip.hipMemcpy(out.ctypes.data, self.ptr, n, 2)  # hipMemcpyDeviceToHost
+        if st != 0:
+            raise FndError(A.E_DEVICE, f"hipMemcpy failed with HIP error {st}")
+        return out
 
     def free(self):
         if self.ptr:
@@ -459,6 +480,13 @@ class Context:
         n = C.c_uint32()
         self._ck(self.lib.fnd_pt_framebuffer_read_rgba8(self.h, fb, exposure, out.ctypes.data, C.byref(n)))
         return out, n.value
+
+    def framebuffer_present(self, fb, device_ptr, row_pitch, exposure=1.0):
+        """R8G8B8A8 bytes of the mean radiance written into device memory
+        (rows row_pitch bytes apart); returns the sample count."""
+        n = C.c_uint32()
+        self._ck(self.lib.fnd_pt_framebuffer_present(self.h, fb, exposure, device_ptr, row_pitch, C.byref(n)))
+        return n.value
 
     def render(self, jobs):
         arr = (A.RenderJob * len(jobs))(*jobs)

@@ -296,6 +296,16 @@ fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle fb, double* r
  * the thresholds rounded from double to float once); NaN and x <= 0 give 0. */
 fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle fb, float exposure, uint8_t* rgba,
                                          uint32_t* out_samples);
+/* The same bytes written into device memory of the caller's, for instance
+ * the swapchain image's memory imported into HIP, so the frame reaches the
+ * presentation engine without a round trip through host memory (the
+ * reference stages through a mapped buffer and CopyBufferToImage,
+ * src/Platform/RHI/Resource.hpp:62-68, src/Platform/RHI/Command.hpp:94).
+ * Row y starts at device_rgba + y * row_pitch; row_pitch is a multiple of 4,
+ * at least 4 * width; device_rgba is 4-byte aligned.  Bytes between rows are
+ * not written.  Returns after the device has finished. */
+fnd_status fnd_pt_framebuffer_present(fnd_pt_ctx* ctx, fnd_handle fb, float exposure, void* device_rgba,
+                                      uint64_t row_pitch, uint32_t* out_samples);
 /* Renders all jobs in one wavefront launch sequence: their paths form one path
  * space, processed in windows (see fnd_tuning.queue_capacity). */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);

@@ -138,6 +138,32 @@ def test_rgba8_present_matches_the_declared_transfer(ctx, cornell):
         ctx.framebuffer_destroy(fb)
 
 
+def test_present_into_device_memory_matches_the_host_readback(ctx, cornell):
+    """fnd_pt_framebuffer_present writes the same R8G8B8A8 bytes as the host
+    readback into caller device memory with a row pitch, leaves the bytes
+    between rows alone, and refuses a short pitch or a misaligned target."""
+    g, s = cornell
+    w, h = 37, 21
+    fb = ctx.framebuffer_create(w, h)
+    mem = pt.DeviceMemory(256 * h)
+    try:
+        ctx.render([pt.make_job(s, fb, g.camera, 4, 4, seed=2)])
+        host, n = ctx.framebuffer_read_rgba8(fb, w, h, exposure=1.5)
+        mem.write(np.full(256 * h, 0xAB, np.uint8))
+        assert ctx.framebuffer_present(fb, mem.ptr, 256, exposure=1.5) == n == 4
+        dev = mem.read().reshape(h, 256)
+        assert np.array_equal(dev[:, :4 * w].reshape(h, w, 4), host)
+        assert np.all(dev[:, 4 * w:] == 0xAB)
+        for ptr, pitch in [(mem.ptr, 4 * w - 4), (mem.ptr, 4 * w + 2), (mem.ptr + 2, 256)]:
+            with pytest.raises(pt.FndError) as e:
+                ctx.framebuffer_present(fb, ptr, pitch)
+            assert e.value.status == A.E_INVALID_ARG
+        assert np.array_equal(mem.read().reshape(h, 256), dev)  # the refused calls wrote nothing
+    finally:
+        mem.free()
+        ctx.framebuffer_destroy(fb)
+
+
 def test_images_do_not_depend_on_queue_or_tiles(ctx, cornell):
     g, s = cornell
     base, _ = render(ctx, s, g.camera, 40, 24, 4, 4, seed=3)
