@@ -230,7 +230,7 @@ int8_t choose_exponent(float ext)
     return int8_t(ee);
 }
 
-void encode_node(WideNode& w, const Box* child, const bool* internal, const uint8_t* leaf_meta, int nc)
+void encode_node(WideNode& w, const Box* child, const bool* internal, int nc)
 {
     std::memset(&w, 0, sizeof(w));
     Box nb;
@@ -263,16 +263,11 @@ void encode_node(WideNode& w, const Box* child, const bool* internal, const uint
         w.qhi[a] = hi_word;
     }
     uint8_t imask = 0;
-    uint32_t meta = 0;
     for (int k = 0; k < nc; ++k) {
         imask |= uint8_t(0x10u << k);  // slot k holds a child
-        if (internal[k])
-            imask |= uint8_t(1u << k);
-        else
-            meta |= uint32_t(leaf_meta[k]) << (8 * k);
+        if (internal[k]) imask |= uint8_t(1u << k);
     }
     w.imask = imask;
-    w.meta = meta;
 }
 
 }  // namespace
@@ -335,7 +330,7 @@ bool build_wide_tree(const Box* boxes, uint32_t n, uint32_t leaf_max, unsigned t
         }
         Box cbox[4];
         bool internal[4];
-        uint8_t meta[4] = {0, 0, 0, 0};
+        uint32_t entry[4] = {0, 0, 0, 0};
         int n_int = 0;
         for (int k = 0; k < nc; ++k) {
             cbox[k] = bn[ch[k]].box;
@@ -347,13 +342,15 @@ bool build_wide_tree(const Box* boxes, uint32_t n, uint32_t leaf_max, unsigned t
         for (int k = 0; k < nc; ++k) {
             if (internal[k]) continue;
             const BNode& lf = bn[ch[k]];
-            const uint32_t off = uint32_t(out.order.size()) - prim_base;
-            meta[k] = uint8_t((off << 3) | lf.count);
+            entry[k] = kEntryTri | (uint32_t(out.order.size()) << 3) | lf.count;
             for (uint32_t j = 0; j < lf.count; ++j) out.order.push_back(idx[lf.first + j]);
         }
+        for (int k = 0, rank = 0; k < nc; ++k)
+            if (internal[k]) entry[k] = child_base + uint32_t(rank++);
         out.nodes.resize(out.nodes.size() + n_int);
         WideNode& w = out.nodes[it.wide];
-        encode_node(w, cbox, internal, meta, nc);
+        encode_node(w, cbox, internal, nc);
+        for (int k = 0; k < 4; ++k) w.entry[k] = entry[k];
         w.child_base = child_base;
         w.prim_base = prim_base;
         int rank = n_int;

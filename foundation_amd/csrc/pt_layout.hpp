@@ -11,9 +11,11 @@
 //    Karras, Laine, "Efficient Incoherent Ray Traversal on GPUs Through
 //    Compressed Wide BVHs", HPG 2017), rounded outward so a dequantised box
 //    always contains the exact child box.
-//  * Internal children of a node are contiguous (child_base + rank among the
-//    internal slots); the triangles of its leaf children are contiguous
-//    (prim_base + offset).  So a node needs two indices, not four.
+//  * Each slot holds its finished traversal stack entry (a node index, or a
+//    tagged triangle range or instance), so the node step pushes what it
+//    loads and derives nothing per child.  Internal children of a node are
+//    still contiguous (child_base + rank among the internal slots) and the
+//    triangles of its leaf children too (prim_base + offset).
 //  * LeafTri: 48 bytes, three float4 whose .w of the first carries the
 //    triangle's index within its mesh — a hit never needs a second lookup.
 //  * InstanceRec: 64 bytes per instance in top-level leaf order: the
@@ -30,7 +32,6 @@ constexpr uint32_t kWideArity = 4;
 // triangles for 5-15% more node visits and runs 8-31% faster; the tree has
 // 3x the nodes (340 MB at 10M triangles).
 constexpr uint32_t kLeafMax = 1;
-constexpr uint32_t kNodeFlagTop = 1u;     // leaf children are instances
 constexpr uint32_t kMaxTriangles = 1u << 28;
 constexpr uint32_t kMaxNodes = 1u << 30;
 
@@ -45,13 +46,12 @@ struct alignas(64) WideNode {
     float px, py, pz;        // quantisation origin
     int8_t ex, ey, ez;       // child box scale is 2^e per axis
     uint8_t imask;           // bit k: slot k is an internal node; bit 4 + k: slot k holds a child
-    uint32_t child_base;     // node index of the first internal child
-    uint32_t prim_base;      // first triangle (or instance) of the leaf children
-    uint32_t meta;           // byte k: leaf slot (offset << 3 | count), 0 = empty or internal
-    uint32_t flags;          // kNodeFlagTop
+    uint32_t entry[4];       // slot k's stack entry: child node index, kEntryTri | first << 3 | count,
+                             // or kEntryInstance | index << 3 | 1; 0 for an empty slot
     uint32_t qlo[3];         // byte k of qlo[a]: slot k lower plane on axis a
     uint32_t qhi[3];         // byte k of qhi[a]: slot k upper plane on axis a
-    uint32_t pad[2];
+    uint32_t child_base;     // node index of the first internal child (not read by the traversal)
+    uint32_t prim_base;      // first triangle (or instance) of the leaf children (not read by the traversal)
 };
 static_assert(sizeof(WideNode) == 64, "WideNode must be one 64-byte fetch");
 

@@ -194,6 +194,12 @@ fnd_status encode_scene(const fnd_scene_desc* d, HostScene& out, std::string& er
             WideNode w = tr.nodes[k];
             w.child_base += node_off[m];
             w.prim_base += tri_off[m];
+            for (int c = 0; c < 4; ++c) {
+                if (!((w.imask >> (4 + c)) & 1u)) continue;
+                // node indices move by the mesh's node offset, triangle ranges
+                // by its triangle offset (first << 3 | count)
+                w.entry[c] += ((w.imask >> c) & 1u) ? node_off[m] : tri_off[m] << 3;
+            }
             out.nodes[node_off[m] + k] = w;
         }
         const fnd_mesh_desc& md = d->meshes[m];
@@ -211,7 +217,12 @@ fnd_status encode_scene(const fnd_scene_desc* d, HostScene& out, std::string& er
     for (size_t k = 0; k < top.nodes.size(); ++k) {
         WideNode w = top.nodes[k];
         w.child_base += uint32_t(top_off);
-        w.flags |= kNodeFlagTop;
+        for (int c = 0; c < 4; ++c) {
+            if (!((w.imask >> (4 + c)) & 1u)) continue;
+            // the top tree's leaves are instances, one per leaf: retag them
+            w.entry[c] = ((w.imask >> c) & 1u) ? w.entry[c] + uint32_t(top_off)
+                                               : kEntryInstance | (w.entry[c] & ~kEntryTri);
+        }
         out.nodes[top_off + k] = w;
     }
 

@@ -14,7 +14,8 @@
 //  * Each loop iteration does at most one unit of work per lane: one 64-byte
 //    node (4 quantised slab tests, sort, push), one triangle, or an instance
 //    entry; the kinds that run in an iteration are picked by lanes per cost.
-//    Leaf children are pushed as (first, count) triangle ranges.
+//    A node stores each slot's finished stack entry (child index, or tagged
+//    triangle range or instance), so a node step pushes what it loaded.
 //  * Two-level scenes: an instance entry moves the ray into object space and
 //    pushes a return marker; popping the marker restores the world ray and
 //    the pop continues.
@@ -201,35 +202,27 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
             const int ey = int(int8_t((n0.w >> 8) & 0xffu));
             const int ez = int(int8_t((n0.w >> 16) & 0xffu));
             const uint32_t imask = n0.w >> 24;
-            const uint32_t child_base = n1.x, prim_base = n1.y, meta = n1.z, flags = n1.w;
             const float sx = scale_exp2(r.ix, ex);
             const float sy = scale_exp2(r.iy, ey);
             const float sz = scale_exp2(r.iz, ez);
             const float cx = (__uint_as_float(n0.x) - r.ox) * r.ix;
             const float cy = (__uint_as_float(n0.y) - r.oy) * r.iy;
             const float cz = (__uint_as_float(n0.z) - r.oz) * r.iz;
-            // qlo = (n2.x, n2.y, n2.z), qhi = (n2.w, n3.x, n3.y)
+            // entries = n1; qlo = (n2.x, n2.y, n2.z), qhi = (n2.w, n3.x, n3.y)
             const bool px_ = r.ix >= 0.f, py_ = r.iy >= 0.f, pz_ = r.iz >= 0.f;
             const uint32_t nx = px_ ? n2.x : n2.w, fx = px_ ? n2.w : n2.x;
             const uint32_t ny = py_ ? n2.y : n3.x, fy = py_ ? n3.x : n2.y;
             const uint32_t nz = pz_ ? n2.z : n3.y, fz = pz_ ? n3.y : n2.z;
-            const bool top = TWO && (flags & kNodeFlagTop) != 0u;
-            // a leaf child's entry is tag | ((prim_base + offset) << 3 | count) =
-            // tag | (prim_base << 3) + meta byte, for triangle ranges and instances alike
-            const uint32_t leaf_base = (top ? kEntryInstance : kEntryTri) | (prim_base << 3);
             float tk[4];
-            uint32_t ek[4];
+            uint32_t ek[4] = {n1.x, n1.y, n1.z, n1.w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const float tn = fmaxf(fmaxf(fmaf(ubyte_f(nx, k), sx, cx), fmaf(ubyte_f(ny, k), sy, cy)),
                                        fmaxf(fmaf(ubyte_f(nz, k), sz, cz), tmin));
                 const float tf = fminf(fminf(fmaf(ubyte_f(fx, k), sx, cx), fmaf(ubyte_f(fy, k), sy, cy)),
                                        fminf(fmaf(ubyte_f(fz, k), sz, cz), tbest));
-                const uint32_t m = (meta >> (8 * k)) & 0xffu;
-                const bool internal = ((imask >> k) & 1u) != 0u;
                 const bool hit = ((imask >> (4 + k)) & 1u) != 0u && tn <= tf * kSlack;
                 tk[k] = hit ? tn : __builtin_inff();
-                ek[k] = internal ? child_base + uint32_t(__popc(imask & ((1u << k) - 1u))) : leaf_base + m;
             }
             const bool any_l = MIXED ? lane_any : ANY_CONST;
             if (!any_l) {

@@ -138,8 +138,8 @@ class GeneratedScene:
 
 
 WIDE_NODE_DTYPE = np.dtype([
-    ("p", "<f4", 3), ("e", "i1", 3), ("imask", "u1"), ("child_base", "<u4"), ("prim_base", "<u4"),
-    ("meta", "<u4"), ("flags", "<u4"), ("qlo", "<u4", 3), ("qhi", "<u4", 3), ("pad", "<u4", 2)])
+    ("p", "<f4", 3), ("e", "i1", 3), ("imask", "u1"), ("entry", "<u4", 4), ("qlo", "<u4", 3), ("qhi", "<u4", 3),
+    ("child_base", "<u4"), ("prim_base", "<u4")])
 assert WIDE_NODE_DTYPE.itemsize == 64
 LEAF_TRI_DTYPE = np.dtype([("v0", "<f4", 3), ("local_id", "<u4"), ("v1", "<f4", 3), ("material", "<u4"),
                            ("v2", "<f4", 3), ("light", "<i4")])

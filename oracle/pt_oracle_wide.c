@@ -31,15 +31,14 @@
 #define W_ENTRY_TRI 0x80000000u
 #define W_ENTRY_INST 0x40000000u
 #define W_ENTRY_RETURN 0x7FFFFFFFu
-#define W_FLAG_TOP 1u
 
 typedef struct wnode {          /* 64 bytes */
     float p[3];
     int8_t e[3];
     uint8_t imask;
-    uint32_t child_base, prim_base, meta, flags;
+    uint32_t entry[4];          /* slot k's stack entry, tagged; 0 when empty */
     uint32_t qlo[3], qhi[3];
-    uint32_t pad[2];
+    uint32_t child_base, prim_base;
 } wnode;
 
 typedef struct wtri {           /* 48 bytes */
@@ -161,7 +160,6 @@ static void walk(const wscene* s, const float* ray8, int anyhit, uint32_t* stk_e
                 nw[a] = pos ? n->qlo[a] : n->qhi[a];
                 fw[a] = pos ? n->qhi[a] : n->qlo[a];
             }
-            const int top = s->two_level && (n->flags & W_FLAG_TOP);
             float tk[4];
             uint32_t ek[4];
             for (int k = 0; k < 4; ++k) {
@@ -169,18 +167,9 @@ static void walk(const wscene* s, const float* ray8, int anyhit, uint32_t* stk_e
                                        fmaxf(fmaf(qbyte(nw[2], k), sc[2], cc[2]), tmin));
                 const float tf = fminf(fminf(fmaf(qbyte(fw[0], k), sc[0], cc[0]), fmaf(qbyte(fw[1], k), sc[1], cc[1])),
                                        fminf(fmaf(qbyte(fw[2], k), sc[2], cc[2]), tbest));
-                const uint32_t m = (n->meta >> (8 * k)) & 0xffu;
-                const int internal = (n->imask >> k) & 1u;
-                const int hit = (internal || m != 0u) && tn <= tf * W_SLACK;
-                uint32_t e;
-                if (internal)
-                    e = n->child_base + (uint32_t)__builtin_popcount(n->imask & ((1u << k) - 1u));
-                else if (top) /* one instance per leaf: (index << 3) | 1 */
-                    e = W_ENTRY_INST | ((n->prim_base << 3) + m);
-                else
-                    e = W_ENTRY_TRI | ((n->prim_base + (m >> 3)) << 3) | (m & 7u);
+                const int hit = ((n->imask >> (4 + k)) & 1u) && tn <= tf * W_SLACK;
                 tk[k] = hit ? tn : INFINITY;
-                ek[k] = e;
+                ek[k] = n->entry[k];
             }
             if (!anyhit) {
                 static const int net[5][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {1, 2}};

@@ -134,5 +134,6 @@ def test_encode_single_triangle():
     enc = pt.encode_scene(d)
     assert enc["nodes"].shape == (1,) and enc["tris"].shape == (1,)
     n = enc["nodes"][0]
-    assert n["imask"] == 0x10 and (n["meta"] & 0xff) == 1  # one leaf slot (no internal) with one triangle
+    # one leaf slot (no internal) holding triangle 0, count 1
+    assert n["imask"] == 0x10 and list(n["entry"]) == [0x80000000 | (0 << 3) | 1, 0, 0, 0]
     assert enc["stack_entries"] == 3 * enc["max_depth"] + 2

@@ -237,24 +237,26 @@ def test_encoder_boxes_are_conservative_and_leaves_partition(kind, kw):
         node = nodes[ni]
         lo, hi = _dequant(node)
         imask = int(node["imask"])
-        meta = int(node["meta"])
         child = int(node["child_base"])
         prim = int(node["prim_base"])
         rank = 0
+        leaf_next = prim  # leaf ranges of a node are consecutive from prim_base
         blo = np.full(3, np.inf)
         bhi = np.full(3, -np.inf)
         for k in range(4):
-            # bit 4 + k marks the slots the traversal may enter: internal or a non-empty leaf
-            assert ((imask >> (4 + k)) & 1) == int(bool((imask >> k) & 1) or ((meta >> (8 * k)) & 0xFF) != 0), (ni, k)
+            e = int(node["entry"][k])
+            if not (imask >> (4 + k)) & 1:  # bit 4 + k marks the slots the traversal may enter
+                assert e == 0 and not (imask >> k) & 1, (ni, k)
+                continue
             if (imask >> k) & 1:
-                clo, chi = subtree(child + rank)
+                assert e == child + rank, (ni, k)  # internal: node index, children contiguous
+                clo, chi = subtree(e)
                 rank += 1
             else:
-                m = (meta >> (8 * k)) & 0xFF
-                if m == 0:
-                    continue
-                first, count = prim + (m >> 3), m & 7
-                assert 1 <= count <= 4
+                assert e >> 30 == 0b10, (ni, k)  # a triangle range (single-level scene)
+                first, count = (e >> 3) & 0x0FFFFFFF, e & 7
+                assert first == leaf_next and 1 <= count <= 4, (ni, k)
+                leaf_next += count
                 seen[first:first + count] += 1
                 v = tv[first:first + count].reshape(-1, 3)
                 clo, chi = v.min(0), v.max(0)
