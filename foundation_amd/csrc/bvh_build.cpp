@@ -367,11 +367,12 @@ bool build_wide_tree(const Box* boxes, uint32_t n, uint32_t leaf_max, unsigned t
     return true;
 }
 
-Box transform_box_conservative(const Box& b, const float* m)
+namespace {
+
+void widen_by_mapped_corners(const double* blo, const double* bhi, const float* m, double* lo, double* hi)
 {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int c = 0; c < 8; ++c) {
-        const double p[3] = {(c & 1) ? b.hi[0] : b.lo[0], (c & 2) ? b.hi[1] : b.lo[1], (c & 4) ? b.hi[2] : b.lo[2]};
+        const double p[3] = {(c & 1) ? bhi[0] : blo[0], (c & 2) ? bhi[1] : blo[1], (c & 4) ? bhi[2] : blo[2]};
         for (int i = 0; i < 3; ++i) {
             const double w = double(m[4 * i]) * p[0] + double(m[4 * i + 1]) * p[1] + double(m[4 * i + 2]) * p[2] +
                              double(m[4 * i + 3]);
@@ -379,6 +380,10 @@ Box transform_box_conservative(const Box& b, const float* m)
             hi[i] = std::max(hi[i], w);
         }
     }
+}
+
+Box padded_outward(const double* lo, const double* hi)
+{
     Box out;
     for (int i = 0; i < 3; ++i) {
         const double pad = (hi[i] - lo[i]) * 0x1p-10 + 1e-30;
@@ -386,6 +391,53 @@ Box transform_box_conservative(const Box& b, const float* m)
         out.hi[i] = std::nextafter(float(hi[i] + pad), INFINITY);
     }
     return out;
+}
+
+void collect_cover(const WideTree& t, uint32_t node, int depth, std::vector<DBox>& out)
+{
+    const WideNode& w = t.nodes[node];
+    const int8_t e[3] = {w.ex, w.ey, w.ez};
+    const float p[3] = {w.px, w.py, w.pz};
+    for (int k = 0; k < 4; ++k) {
+        if (!((w.imask >> (4 + k)) & 1u)) continue;
+        if (((w.imask >> k) & 1u) && depth > 1) {
+            collect_cover(t, w.entry[k], depth - 1, out);
+            continue;
+        }
+        // p + q * 2^e is exact in double and bounds the slot's content (the
+        // encoder rounded the planes outward)
+        DBox b;
+        for (int a = 0; a < 3; ++a) {
+            const double s = std::ldexp(1.0, e[a]);
+            b.lo[a] = double(p[a]) + double((w.qlo[a] >> (8 * k)) & 0xffu) * s;
+            b.hi[a] = double(p[a]) + double((w.qhi[a] >> (8 * k)) & 0xffu) * s;
+        }
+        out.push_back(b);
+    }
+}
+
+}  // namespace
+
+std::vector<DBox> cover_boxes(const WideTree& t, int depth)
+{
+    std::vector<DBox> out;
+    if (!t.nodes.empty()) collect_cover(t, 0, std::max(1, depth), out);
+    return out;
+}
+
+Box transform_cover_conservative(const std::vector<DBox>& cover, const float* m)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const DBox& b : cover) widen_by_mapped_corners(b.lo, b.hi, m, lo, hi);
+    return padded_outward(lo, hi);
+}
+
+Box transform_box_conservative(const Box& b, const float* m)
+{
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    const double blo[3] = {b.lo[0], b.lo[1], b.lo[2]}, bhi[3] = {b.hi[0], b.hi[1], b.hi[2]};
+    widen_by_mapped_corners(blo, bhi, m, lo, hi);
+    return padded_outward(lo, hi);
 }
 
 }  // namespace fnd

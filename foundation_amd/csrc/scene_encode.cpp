@@ -150,12 +150,19 @@ fnd_status encode_scene(const fnd_scene_desc* d, HostScene& out, std::string& er
     if (d->num_instances) {
         std::vector<Box> ib(d->num_instances);
         std::vector<uint32_t> id_base(d->num_instances);
+        // Each instance's world box is the mapped union of the (up to 64)
+        // subtree boxes three levels below its mesh's root, not its mapped
+        // root box: a rotated
+        // instance's box then hugs its geometry, and fewer rays enter
+        // instances they miss.
+        std::vector<std::vector<DBox>> cover(d->num_meshes);
+        for (uint32_t m = 0; m < d->num_meshes; ++m) cover[m] = cover_boxes(trees[m], 3);
         uint32_t base = 0;
         for (uint32_t i = 0; i < d->num_instances; ++i) {
             const fnd_instance_desc& in = d->instances[i];
             id_base[i] = base;
             base += d->meshes[in.mesh].num_triangles;
-            ib[i] = transform_box_conservative(trees[in.mesh].bounds, in.object_to_world);
+            ib[i] = transform_cover_conservative(cover[in.mesh], in.object_to_world);
             for (int a = 0; a < 3; ++a) {
                 scene_bounds.lo[a] = std::min(scene_bounds.lo[a], ib[i].lo[a]);
                 scene_bounds.hi[a] = std::max(scene_bounds.hi[a], ib[i].hi[a]);
