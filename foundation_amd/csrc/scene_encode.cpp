@@ -150,13 +150,16 @@ fnd_status encode_scene(const fnd_scene_desc* d, HostScene& out, std::string& er
     if (d->num_instances) {
         std::vector<Box> ib(d->num_instances);
         std::vector<uint32_t> id_base(d->num_instances);
-        // Each instance's world box is the mapped union of the (up to 64)
-        // subtree boxes three levels below its mesh's root, not its mapped
-        // root box: a rotated
-        // instance's box then hugs its geometry, and fewer rays enter
-        // instances they miss.
+        // Each instance's world box is the mapped union of its mesh's subtree
+        // boxes `depth` levels below the root, not its mapped root box: a
+        // rotated instance's box then hugs its geometry, and fewer rays enter
+        // instances they miss.  The depth is the largest in [2, 7] that keeps
+        // the work within 2^22 box transforms, and 2 where none does (16,384
+        // boxes per mesh for 100 instances; 16 for a million).
+        int depth = 2;
+        while (depth < 7 && (uint64_t(d->num_instances) << (2 * (depth + 1))) <= (1ull << 22)) ++depth;
         std::vector<std::vector<DBox>> cover(d->num_meshes);
-        for (uint32_t m = 0; m < d->num_meshes; ++m) cover[m] = cover_boxes(trees[m], 3);
+        for (uint32_t m = 0; m < d->num_meshes; ++m) cover[m] = cover_boxes(trees[m], depth);
         uint32_t base = 0;
         for (uint32_t i = 0; i < d->num_instances; ++i) {
             const fnd_instance_desc& in = d->instances[i];

@@ -269,6 +269,50 @@ def test_encoder_boxes_are_conservative_and_leaves_partition(kind, kw):
     assert np.all(seen == 1)
 
 
+def test_instance_boxes_contain_and_hug_their_geometry():
+    """Every top-level slot names one instance, its (dequantised) box contains
+    that instance's world-space vertices, and the boxes are tighter than the
+    mapped object-space root boxes of randomly rotated instances."""
+    gen = pt.GeneratedScene(A.GEN_INSTANCED_SPHERES, instances=8, subdivision=12)
+    enc = _encode(gen)
+    nodes, recs = enc["nodes"], enc["instances"]
+    ms, insts = gen.meshes(), gen.instances()
+    seen = np.zeros(len(insts), np.int64)
+    area_new = area_root = 0.0
+
+    def area(lo, hi):
+        ex = np.maximum(hi - lo, 0.0)
+        return 2.0 * (ex[0] * ex[1] + ex[1] * ex[2] + ex[2] * ex[0])
+
+    def walk(ni):
+        nonlocal area_new, area_root
+        node = nodes[ni]
+        lo, hi = _dequant(node)
+        for k in range(4):
+            e = int(node["entry"][k])
+            if not (int(node["imask"]) >> (4 + k)) & 1:
+                continue
+            if (int(node["imask"]) >> k) & 1:
+                walk(e)
+                continue
+            assert e >> 30 == 0b01 and e & 7 == 1, (ni, k)  # one instance per top-level leaf
+            i = int(recs[(e >> 3) & 0x07FFFFFF]["orig_index"])
+            seen[i] += 1
+            m = ms[insts[i]["mesh"]]
+            o2w = insts[i]["o2w"].astype(np.float64)
+            v = m["positions"].astype(np.float64) @ o2w[:, :3].T + o2w[:, 3]
+            assert np.all(v >= lo[k]) and np.all(v <= hi[k]), (ni, k)
+            olo, ohi = m["positions"].min(0).astype(np.float64), m["positions"].max(0).astype(np.float64)
+            corners = np.array([[(ohi if c >> a & 1 else olo)[a] for a in range(3)] for c in range(8)])
+            w = corners @ o2w[:, :3].T + o2w[:, 3]
+            area_new += area(lo[k], hi[k])
+            area_root += area(w.min(0), w.max(0))
+
+    walk(int(enc["root"]))
+    assert np.all(seen == 1)
+    assert area_new < 0.85 * area_root, (area_new, area_root)
+
+
 def test_wide_walk_equals_binary_walk(oracle):
     rng = np.random.default_rng(5)
     for kind, kw in [(A.GEN_CORNELL_SOUP, {}), (A.GEN_INSTANCED_SPHERES, {"instances": 6, "subdivision": 8})]:
