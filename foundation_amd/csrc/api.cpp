@@ -188,7 +188,9 @@ void default_tuning(fnd_tuning& t)
     t.block_size = 128;
     t.stack_lds = 12;
     t.waves_per_cu = 24;
-    t.refill = 40;
+    // refill 48: 2.5% more incoherent rays/s at 10.08M triangles than 40, with
+    // the fewer node visits per ray of tight instance boxes; renders unchanged
+    t.refill = 48;
     t.queue_capacity = 1u << 21;
     t.tile = 8;
 }
