@@ -271,14 +271,10 @@ __device__ __forceinline__ bool start_path(const WfArgs& a, uint64_t g0, uint32_
         ray.tmax = 3.0e38f;
         live = true;
     }
-    a.b.p_job[slot] = q.job;
-    a.b.p_pixel[slot] = q.pixel;
-    a.b.p_sample[slot] = q.sample;
+    a.b.p_site[slot] = make_uint4(q.job, q.pixel, q.sample, gid);
+    a.b.p_state[slot] = make_uint2(train, 0u);
     a.b.p_thr[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
     a.b.p_L[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-    a.b.p_train[slot] = train;
-    a.b.p_gid[slot] = gid;
-    a.b.p_bounce[slot] = 0u;
     return live;
 }
 
@@ -287,7 +283,7 @@ __device__ __forceinline__ bool start_path(const WfArgs& a, uint64_t g0, uint32_
 // list).
 __device__ __forceinline__ void retire(const WfArgs& a, uint32_t slot, float4 L)
 {
-    a.b.w_L[a.b.p_gid[slot]] = make_float4(L.x, L.y, L.z, 0.0f);
+    a.b.w_L[a.b.p_site[slot].w] = make_float4(L.x, L.y, L.z, 0.0f);
 }
 
 // The window's first n paths into slots 0..n-1; camera rays to extension
@@ -307,7 +303,7 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
         if (live) {
             a.b.ext_rays[0][slot[0]] = ray;
             a.b.ext_path[0][slot[0]] = i;
-            a.b.ext_scene[0][slot[0]] = a.jobs[a.b.p_job[i]].scene_slot;
+            a.b.ext_scene[0][slot[0]] = a.jobs[a.b.p_site[i].x].scene_slot;
         }
         if (dead) a.b.free_list[0][slot[1]] = i;
     }
@@ -343,7 +339,7 @@ __global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t f
         if (live) {
             a.b.ext_rays[next][q[0]] = ray;
             a.b.ext_path[next][q[0]] = slot;
-            a.b.ext_scene[next][q[0]] = a.jobs[a.b.p_job[slot]].scene_slot;
+            a.b.ext_scene[next][q[0]] = a.jobs[a.b.p_site[slot].x].scene_slot;
         }
         if (dead) a.b.free_list[fidx ^ 1u][q[1]] = slot;
     }
@@ -391,11 +387,14 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
         V3 sh_contrib = v3(0, 0, 0);
         CacheQuery cq{};
         V3 q_w = v3(0, 0, 0);
-        uint32_t path = 0, scene_slot = 0;
+        uint32_t path = 0, scene_slot = 0, gid = 0;
         if (i < n) {
             path = a.b.ext_path[cur][i];
-            const uint32_t bounce = a.b.p_bounce[path];
-            const JobDev& job = a.jobs[a.b.p_job[path]];
+            const uint4 site = a.b.p_site[path];
+            const uint2 state = a.b.p_state[path];
+            gid = site.w;
+            const uint32_t bounce = state.y;
+            const JobDev& job = a.jobs[site.x];
             scene_slot = job.scene_slot;
             const ShadeScene& sc = a.scenes[scene_slot];
             const float4 hit = a.b.ext_hits[i];
@@ -406,7 +405,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
             V3 L = v3(L4.x, L4.y, L4.z);
             const float pdf_prev = thr4.w;
             const V3 dir = v3(ray.dir[0], ray.dir[1], ray.dir[2]);
-            const uint32_t tslot = a.b.p_train[path];
+            const uint32_t tslot = state.x;
             const bool record = tslot != kNoTrainSlot;
             uint32_t rc = 0;
             if (record) {
@@ -457,7 +456,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                     L = L + thr * Le * w;
                 }
                 if (!emissive && bounce < job.max_depth) {
-                    Rng rng = rng_for(job.seed, a.b.p_sample[path], a.b.p_pixel[path], bounce + 1u);
+                    Rng rng = rng_for(job.seed, site.z, site.y, bounce + 1u);
                     const Onb frame = onb_from(nf);
                     const V3 wo = to_local(frame, -dir);
                     const V3 albedo = v3(mat.albedo[0], mat.albedo[1], mat.albedo[2]);
@@ -545,7 +544,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
                             thr = nthr;
                             cont = true;
                             a.b.p_thr[path] = make_float4(thr.x, thr.y, thr.z, bs.pdf);
-                            a.b.p_bounce[path] = bounce + 1u;
+                            a.b.p_state[path] = make_uint2(tslot, bounce + 1u);
                         }
                     }
                 }
@@ -571,7 +570,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
         }
         if (query) {
             a.b.q_query[qs] = cq;
-            a.b.q_path[qs] = a.b.p_gid[path];
+            a.b.q_path[qs] = gid;
             a.b.q_weight[qs] = make_float4(q_w.x, q_w.y, q_w.z, 0.f);
         }
         if (shadow) {

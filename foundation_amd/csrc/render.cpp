@@ -86,8 +86,7 @@ static bool carve_queues(fnd_arena* ar, uint64_t cap, WfBuffers& b)
         ok = ok && q != nullptr;
         *p = static_cast<T*>(q);
     };
-    A(&b.p_job, cap); A(&b.p_pixel, cap); A(&b.p_sample, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
-    A(&b.p_train, cap); A(&b.p_gid, cap); A(&b.p_bounce, cap);
+    A(&b.p_site, cap); A(&b.p_state, cap); A(&b.p_thr, cap); A(&b.p_L, cap);
     A(&b.w_L, cap * kWindowSlots);
     A(&b.free_list[0], cap); A(&b.free_list[1], cap);
     for (int q = 0; q < 2; ++q) {

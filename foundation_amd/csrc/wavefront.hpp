@@ -85,14 +85,14 @@ constexpr uint32_t kWindowSlots = 8;
 
 struct WfBuffers {
     // per path slot
-    uint32_t* p_job;
-    uint32_t* p_pixel;        // y * W + x, or 0xFFFFFFFF for padding
-    uint32_t* p_sample;
+    // The small fields are packed per slot: slots are read and written in
+    // queue order, i.e. scattered, so one record touches one line where
+    // separate arrays touched one line each.
+    uint4* p_site;            // (job, pixel: y * W + x or 0xFFFFFFFF for padding, sample,
+                              //  gid: the slot's path, its index in the window)
+    uint2* p_state;           // (training record slot or kNoTrainSlot, vertices shaded so far)
     float4* p_thr;            // throughput, w = solid-angle pdf of the last BSDF sample (0: camera)
     float4* p_L;              // radiance
-    uint32_t* p_train;        // training record slot, or kNoTrainSlot
-    uint32_t* p_gid;          // the slot's path: index in the window
-    uint32_t* p_bounce;       // vertices shaded so far
     // per window path: final radiance (cache term added after the window drains)
     float4* w_L;
     uint32_t window;          // paths in the current window
