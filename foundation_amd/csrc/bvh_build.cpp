@@ -432,12 +432,4 @@ Box transform_cover_conservative(const std::vector<DBox>& cover, const float* m)
     return padded_outward(lo, hi);
 }
 
-Box transform_box_conservative(const Box& b, const float* m)
-{
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    const double blo[3] = {b.lo[0], b.lo[1], b.lo[2]}, bhi[3] = {b.hi[0], b.hi[1], b.hi[2]};
-    widen_by_mapped_corners(blo, bhi, m, lo, hi);
-    return padded_outward(lo, hi);
-}
-
 }  // namespace fnd

@@ -26,10 +26,6 @@ struct WideTree {
 bool build_wide_tree(const Box* boxes, uint32_t n, uint32_t leaf_max, unsigned threads, WideTree& out,
                      std::string& err);
 
-// Widens a box to contain the 8 mapped corners of `b` under a row-major 3x4
-// map, rounded outward and padded by 2^-10 of the extent.
-Box transform_box_conservative(const Box& b, const float* m34);
-
 // An object-space box in double precision (exact dequantised planes).
 struct DBox {
     double lo[3];
@@ -42,9 +38,9 @@ struct DBox {
 std::vector<DBox> cover_boxes(const WideTree& t, int depth);
 
 // The box of the mapped corners of every box in `cover` under a row-major 3x4
-// map, rounded outward and padded by 2^-10 of the extent, as
-// transform_box_conservative does for one box.  A rotated instance's world
-// box then follows its geometry instead of its rotated object-space box.
+// map (in double), rounded outward to float and padded by 2^-10 of the
+// extent.  A rotated instance's world box then follows its geometry instead
+// of its rotated object-space box.
 Box transform_cover_conservative(const std::vector<DBox>& cover, const float* m34);
 
 }  // namespace fnd
