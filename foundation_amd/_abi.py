@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ABI_VERSION = 5  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
+ABI_VERSION = 6  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
 LIB_PATH = os.path.join(_HERE, "libfnd_pt.so")
 
 fnd_handle = C.c_uint64
@@ -317,6 +317,9 @@ PROTOTYPES = {
     "fnd_pt_framebuffer_read_rgba8": (_S, [_P, _H, C.c_float, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_present": (_S, [_P, _H, C.c_float, _P, C.c_uint64, C.POINTER(C.c_uint32)]),
     "fnd_pt_render": (_S, [_P, C.POINTER(RenderJob), C.c_uint32]),
+    "fnd_camera_basis": (_S, [C.POINTER(Camera), C.c_uint32, C.c_uint32, _P]),
+    "fnd_pt_camera_rays": (_S, [_P, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, _P]),
+    "fnd_pt_rng_draws": (_S, [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
     "fnd_arena_reset": (None, [C.POINTER(Arena), _P, C.c_uint64]),
     "fnd_arena_alloc": (_P, [C.POINTER(Arena), C.c_uint64]),
     "fnd_arena_alloc_aligned": (_P, [C.POINTER(Arena), C.c_uint64, C.c_uint64]),

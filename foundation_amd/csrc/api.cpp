@@ -31,12 +31,12 @@ namespace fnd {
 fnd_status set_error(fnd_pt_ctx* ctx, fnd_status st, const char* fmt, ...)
 {
     if (!ctx) return st;
-    char buf[512];
+    char buf[sizeof ctx->last_error];
     va_list ap;
     va_start(ap, fmt);
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
-    ctx->last_error = buf;
+    std::memcpy(ctx->last_error, buf, sizeof buf);
     if (ctx->log) ctx->log(ctx->log_user, 3, buf);
     return st;
 }
@@ -238,11 +238,7 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
     if (allocator && (!allocator->alloc || !allocator->free)) return FND_E_INVALID_ARG;
     void* mem = host_alloc(allocator, sizeof(fnd_pt_ctx), 64);
     if (!mem) return FND_E_OOM_HOST;
-    fnd_pt_ctx* ctx = new (mem) fnd_pt_ctx();
-    if (allocator) {
-        ctx->alloc = *allocator;
-        ctx->has_alloc = true;
-    }
+    fnd_pt_ctx* ctx = new (mem) fnd_pt_ctx(allocator);
     if (desc) {
         ctx->device = desc->device;
         ctx->budget = desc->device_memory_budget;
@@ -252,7 +248,7 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
     default_tuning(ctx->tuning);
     auto fail = [&](fnd_status st) {
         // keep the message for the caller: there is no context to ask
-        fprintf(stderr, "fnd_pt_create: %s\n", ctx->last_error.c_str());
+        fprintf(stderr, "fnd_pt_create: %s\n", ctx->last_error);
         fnd_pt_destroy(ctx);
         return st;
     };
@@ -273,7 +269,7 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
     st = check_hip(ctx, hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
     if (st != FND_OK) return fail(st);
     ctx->num_cus = uint32_t(prop.multiProcessorCount);
-    ctx->arch = prop.gcnArchName;
+    snprintf(ctx->arch, sizeof ctx->arch, "%s", prop.gcnArchName);
     ctx->stats.num_cus = ctx->num_cus;
     st = check_hip(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (st != FND_OK) return fail(st);
@@ -338,206 +334,239 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
 const char* fnd_pt_last_error(const fnd_pt_ctx* ctx)
 {
     if (!ctx) return "null context";
-    return ctx->last_error.c_str();
+    return ctx->last_error;
 }
 
 fnd_status fnd_pt_wait(fnd_pt_ctx* ctx)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
-    if (st == FND_OK) resolve_timings(ctx);
-    return st;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        if (st == FND_OK) resolve_timings(ctx);
+        return st;
+    });
 }
 
 fnd_status fnd_pt_get_stats(fnd_pt_ctx* ctx, fnd_pt_stats* out)
 {
-    if (!ctx || !out) return FND_E_INVALID_ARG;
-    ctx->stats.device_bytes = ctx->used;
-    *out = ctx->stats;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        ctx->stats.device_bytes = ctx->used;
+        *out = ctx->stats;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_reset_stats(fnd_pt_ctx* ctx)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    const uint32_t cus = ctx->stats.num_cus;
-    std::memset(&ctx->stats, 0, sizeof(ctx->stats));
-    ctx->stats.num_cus = cus;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        const uint32_t cus = ctx->stats.num_cus;
+        std::memset(&ctx->stats, 0, sizeof(ctx->stats));
+        ctx->stats.num_cus = cus;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_set_tuning(fnd_pt_ctx* ctx, const fnd_tuning* t)
 {
-    if (!ctx || !t) return FND_E_INVALID_ARG;
-    fnd_tuning n = ctx->tuning;
-    fnd_tuning d;
-    default_tuning(d);
-    n.block_size = t->block_size ? t->block_size : d.block_size;
-    n.stack_lds = t->stack_lds ? t->stack_lds : d.stack_lds;
-    n.waves_per_cu = t->waves_per_cu ? t->waves_per_cu : d.waves_per_cu;
-    n.refill = t->refill ? t->refill : d.refill;
-    n.queue_capacity = t->queue_capacity ? t->queue_capacity : d.queue_capacity;
-    n.tile = t->tile ? t->tile : d.tile;
-    if (!trace_config_supported(n.block_size, n.stack_lds))
-        return set_error(ctx, FND_E_UNSUPPORTED, "no traversal kernel for block %u / stack %u", n.block_size,
-                         n.stack_lds);
-    if (n.refill > 64 || n.waves_per_cu > 64)
-        return set_error(ctx, FND_E_INVALID_ARG, "refill must be 1..64 and waves_per_cu 1..64");
-    if (n.queue_capacity < 1024 || n.queue_capacity > (1u << 26))
-        return set_error(ctx, FND_E_INVALID_ARG, "queue_capacity must be in [1024, 2^26]");
-    if (n.tile > 256) return set_error(ctx, FND_E_INVALID_ARG, "tile must be <= 256");
-    ctx->tuning = n;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !t) return FND_E_INVALID_ARG;
+        fnd_tuning n = ctx->tuning;
+        fnd_tuning d;
+        default_tuning(d);
+        n.block_size = t->block_size ? t->block_size : d.block_size;
+        n.stack_lds = t->stack_lds ? t->stack_lds : d.stack_lds;
+        n.waves_per_cu = t->waves_per_cu ? t->waves_per_cu : d.waves_per_cu;
+        n.refill = t->refill ? t->refill : d.refill;
+        n.queue_capacity = t->queue_capacity ? t->queue_capacity : d.queue_capacity;
+        n.tile = t->tile ? t->tile : d.tile;
+        if (!trace_config_supported(n.block_size, n.stack_lds))
+            return set_error(ctx, FND_E_UNSUPPORTED, "no traversal kernel for block %u / stack %u", n.block_size,
+                             n.stack_lds);
+        if (n.refill > 64 || n.waves_per_cu > 64)
+            return set_error(ctx, FND_E_INVALID_ARG, "refill must be 1..64 and waves_per_cu 1..64");
+        if (n.queue_capacity < 1024 || n.queue_capacity > (1u << 26))
+            return set_error(ctx, FND_E_INVALID_ARG, "queue_capacity must be in [1024, 2^26]");
+        if (n.tile > 256) return set_error(ctx, FND_E_INVALID_ARG, "tile must be <= 256");
+        ctx->tuning = n;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_get_tuning(fnd_pt_ctx* ctx, fnd_tuning* out)
 {
-    if (!ctx || !out) return FND_E_INVALID_ARG;
-    *out = ctx->tuning;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        *out = ctx->tuning;
+        return FND_OK;
+    });
 }
 
 // ---------------------------------------------------------------- scenes
 
 fnd_status fnd_pt_scene_create(fnd_pt_ctx* ctx, const fnd_scene_desc* d, fnd_handle* out)
 {
-    if (!ctx || !d || !out) return ctx ? set_error(ctx, FND_E_INVALID_ARG, "null argument") : FND_E_INVALID_ARG;
-    *out = FND_INVALID_HANDLE;
-    const double t0 = now_ms();
-    HostScene hs;
-    std::string err;
-    fnd_status st = encode_scene(d, hs, err);
-    if (st != FND_OK) return set_error(ctx, st, "%s", err.c_str());
-    Scene* s = ctx_new<Scene>(ctx);
-    if (!s) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
-    auto bail = [&](fnd_status e) {
-        free_scene(ctx, s);
-        return e;
-    };
-    s->sz_nodes = hs.nodes.size() * sizeof(WideNode);
-    s->sz_tris = hs.tris.size() * sizeof(LeafTri);
-    s->sz_insts = std::max<size_t>(1, hs.insts.size()) * sizeof(InstanceRec);
-    s->sz_materials = hs.materials.size() * sizeof(DeviceMaterial);
-    s->sz_lights = hs.lights.size() * sizeof(DeviceLight);
-    void* p = nullptr;
-    st = dev_alloc(ctx, &p, s->sz_nodes);
-    if (st != FND_OK) return bail(st);
-    s->nodes = static_cast<WideNode*>(p);
-    st = dev_alloc(ctx, &p, s->sz_tris);
-    if (st != FND_OK) return bail(st);
-    s->tris = static_cast<float4*>(p);
-    st = dev_alloc(ctx, &p, s->sz_insts);
-    if (st != FND_OK) return bail(st);
-    s->insts = static_cast<InstanceRec*>(p);
-    st = dev_alloc(ctx, &p, s->sz_materials);
-    if (st != FND_OK) return bail(st);
-    s->materials = static_cast<DeviceMaterial*>(p);
-    st = dev_alloc(ctx, &p, s->sz_lights);
-    if (st != FND_OK) return bail(st);
-    s->lights = static_cast<DeviceLight*>(p);
-    st = copy_sync(ctx, s->nodes, hs.nodes.data(), s->sz_nodes, hipMemcpyHostToDevice, "upload nodes");
-    if (st == FND_OK) st = copy_sync(ctx, s->tris, hs.tris.data(), s->sz_tris, hipMemcpyHostToDevice, "upload triangles");
-    if (st == FND_OK && !hs.insts.empty())
-        st = copy_sync(ctx, s->insts, hs.insts.data(), hs.insts.size() * sizeof(InstanceRec), hipMemcpyHostToDevice,
-                       "upload instances");
-    if (st == FND_OK)
-        st = copy_sync(ctx, s->materials, hs.materials.data(), s->sz_materials, hipMemcpyHostToDevice, "upload materials");
-    if (st == FND_OK) st = copy_sync(ctx, s->lights, hs.lights.data(), s->sz_lights, hipMemcpyHostToDevice, "upload lights");
-    if (st != FND_OK) return bail(st);
-    s->num_lights = hs.num_lights;
-    s->num_materials = uint32_t(hs.materials.size());
-    std::memcpy(s->background, hs.background, 12);
-    s->two_level = hs.two_level;
-    s->root = hs.root;
-    s->info = hs.info;
-    s->info.build_ms = now_ms() - t0;
-    const fnd_handle h = make_handle(kHandleScene, ctx->next_id++);
-    ctx->scenes[h] = s;
-    *out = h;
-    if (ctx->log) {
-        char msg[256];
-        snprintf(msg, sizeof msg, "scene: %llu tris, %llu nodes, depth %u, %.1f ms",
-                 (unsigned long long)s->info.num_triangles, (unsigned long long)s->info.num_nodes, s->info.max_depth,
-                 s->info.build_ms);
-        ctx->log(ctx->log_user, 1, msg);
-    }
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !d || !out) return ctx ? set_error(ctx, FND_E_INVALID_ARG, "null argument") : FND_E_INVALID_ARG;
+        *out = FND_INVALID_HANDLE;
+        const double t0 = now_ms();
+        HostScene hs;
+        std::string err;
+        fnd_status st = encode_scene(d, hs, err);
+        if (st != FND_OK) return set_error(ctx, st, "%s", err.c_str());
+        Scene* s = ctx_new<Scene>(ctx);
+        if (!s) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+        auto bail = [&](fnd_status e) {
+            free_scene(ctx, s);
+            return e;
+        };
+        s->sz_nodes = hs.nodes.size() * sizeof(WideNode);
+        s->sz_tris = hs.tris.size() * sizeof(LeafTri);
+        s->sz_insts = std::max<size_t>(1, hs.insts.size()) * sizeof(InstanceRec);
+        s->sz_materials = hs.materials.size() * sizeof(DeviceMaterial);
+        s->sz_lights = hs.lights.size() * sizeof(DeviceLight);
+        void* p = nullptr;
+        st = dev_alloc(ctx, &p, s->sz_nodes);
+        if (st != FND_OK) return bail(st);
+        s->nodes = static_cast<WideNode*>(p);
+        st = dev_alloc(ctx, &p, s->sz_tris);
+        if (st != FND_OK) return bail(st);
+        s->tris = static_cast<float4*>(p);
+        st = dev_alloc(ctx, &p, s->sz_insts);
+        if (st != FND_OK) return bail(st);
+        s->insts = static_cast<InstanceRec*>(p);
+        st = dev_alloc(ctx, &p, s->sz_materials);
+        if (st != FND_OK) return bail(st);
+        s->materials = static_cast<DeviceMaterial*>(p);
+        st = dev_alloc(ctx, &p, s->sz_lights);
+        if (st != FND_OK) return bail(st);
+        s->lights = static_cast<DeviceLight*>(p);
+        st = copy_sync(ctx, s->nodes, hs.nodes.data(), s->sz_nodes, hipMemcpyHostToDevice, "upload nodes");
+        if (st == FND_OK) st = copy_sync(ctx, s->tris, hs.tris.data(), s->sz_tris, hipMemcpyHostToDevice, "upload triangles");
+        if (st == FND_OK && !hs.insts.empty())
+            st = copy_sync(ctx, s->insts, hs.insts.data(), hs.insts.size() * sizeof(InstanceRec), hipMemcpyHostToDevice,
+                           "upload instances");
+        if (st == FND_OK)
+            st = copy_sync(ctx, s->materials, hs.materials.data(), s->sz_materials, hipMemcpyHostToDevice, "upload materials");
+        if (st == FND_OK) st = copy_sync(ctx, s->lights, hs.lights.data(), s->sz_lights, hipMemcpyHostToDevice, "upload lights");
+        if (st != FND_OK) return bail(st);
+        s->num_lights = hs.num_lights;
+        s->num_materials = uint32_t(hs.materials.size());
+        std::memcpy(s->background, hs.background, 12);
+        s->two_level = hs.two_level;
+        s->root = hs.root;
+        s->info = hs.info;
+        s->info.build_ms = now_ms() - t0;
+        const fnd_handle h = make_handle(kHandleScene, ctx->next_id++);
+        try {
+            ctx->scenes[h] = s;
+        } catch (...) {
+            free_scene(ctx, s);
+            throw;
+        }
+        *out = h;
+        if (ctx->log) {
+            char msg[256];
+            snprintf(msg, sizeof msg, "scene: %llu tris, %llu nodes, depth %u, %.1f ms",
+                     (unsigned long long)s->info.num_triangles, (unsigned long long)s->info.num_nodes, s->info.max_depth,
+                     s->info.build_ms);
+            ctx->log(ctx->log_user, 1, msg);
+        }
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_scene_destroy(fnd_pt_ctx* ctx, fnd_handle h)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    Scene* s = find_scene(ctx, h);
-    if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
-    (void)hipStreamSynchronize(ctx->stream);
-    free_scene(ctx, s);
-    ctx->scenes.erase(h);
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        Scene* s = find_scene(ctx, h);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
+        (void)hipStreamSynchronize(ctx->stream);
+        free_scene(ctx, s);
+        ctx->scenes.erase(h);
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_scene_info(fnd_pt_ctx* ctx, fnd_handle h, fnd_scene_info* out)
 {
-    if (!ctx || !out) return FND_E_INVALID_ARG;
-    Scene* s = find_scene(ctx, h);
-    if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
-    *out = s->info;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        Scene* s = find_scene(ctx, h);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
+        *out = s->info;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_scene_export(fnd_pt_ctx* ctx, fnd_handle h, void* nodes, uint64_t* nodes_bytes, void* tris,
                                uint64_t* tris_bytes)
 {
-    if (!ctx || !nodes_bytes || !tris_bytes) return FND_E_INVALID_ARG;
-    Scene* s = find_scene(ctx, h);
-    if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
-    const uint64_t nb = s->info.num_nodes * sizeof(WideNode), tb = s->info.num_triangles * sizeof(LeafTri);
-    if (nodes) {
-        if (*nodes_bytes < nb) return set_error(ctx, FND_E_INVALID_ARG, "node buffer too small");
-        fnd_status st = copy_sync(ctx, nodes, s->nodes, nb, hipMemcpyDeviceToHost, "export nodes");
-        if (st != FND_OK) return st;
-    }
-    if (tris) {
-        if (*tris_bytes < tb) return set_error(ctx, FND_E_INVALID_ARG, "triangle buffer too small");
-        fnd_status st = copy_sync(ctx, tris, s->tris, tb, hipMemcpyDeviceToHost, "export triangles");
-        if (st != FND_OK) return st;
-    }
-    *nodes_bytes = nb;
-    *tris_bytes = tb;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !nodes_bytes || !tris_bytes) return FND_E_INVALID_ARG;
+        Scene* s = find_scene(ctx, h);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
+        const uint64_t nb = s->info.num_nodes * sizeof(WideNode), tb = s->info.num_triangles * sizeof(LeafTri);
+        if (nodes) {
+            if (*nodes_bytes < nb) return set_error(ctx, FND_E_INVALID_ARG, "node buffer too small");
+            fnd_status st = copy_sync(ctx, nodes, s->nodes, nb, hipMemcpyDeviceToHost, "export nodes");
+            if (st != FND_OK) return st;
+        }
+        if (tris) {
+            if (*tris_bytes < tb) return set_error(ctx, FND_E_INVALID_ARG, "triangle buffer too small");
+            fnd_status st = copy_sync(ctx, tris, s->tris, tb, hipMemcpyDeviceToHost, "export triangles");
+            if (st != FND_OK) return st;
+        }
+        *nodes_bytes = nb;
+        *tris_bytes = tb;
+        return FND_OK;
+    });
 }
 
 // ---------------------------------------------------------------- buffers
 
 fnd_status fnd_pt_buffer_create(fnd_pt_ctx* ctx, uint64_t bytes, fnd_handle* out)
 {
-    if (!ctx || !out) return FND_E_INVALID_ARG;
-    *out = FND_INVALID_HANDLE;
-    if (bytes == 0) return set_error(ctx, FND_E_INVALID_ARG, "zero-sized buffer");
-    Buffer* b = ctx_new<Buffer>(ctx);
-    if (!b) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
-    fnd_status st = dev_alloc(ctx, &b->ptr, bytes);
-    if (st != FND_OK) {
-        ctx_delete(ctx, b);
-        return st;
-    }
-    b->bytes = bytes;
-    const fnd_handle h = make_handle(kHandleBuffer, ctx->next_id++);
-    ctx->buffers[h] = b;
-    *out = h;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        *out = FND_INVALID_HANDLE;
+        if (bytes == 0) return set_error(ctx, FND_E_INVALID_ARG, "zero-sized buffer");
+        Buffer* b = ctx_new<Buffer>(ctx);
+        if (!b) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+        fnd_status st = dev_alloc(ctx, &b->ptr, bytes);
+        if (st != FND_OK) {
+            ctx_delete(ctx, b);
+            return st;
+        }
+        b->bytes = bytes;
+        const fnd_handle h = make_handle(kHandleBuffer, ctx->next_id++);
+        try {
+            ctx->buffers[h] = b;
+        } catch (...) {
+            dev_free(ctx, b->ptr, b->bytes);
+            ctx_delete(ctx, b);
+            throw;
+        }
+        *out = h;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_buffer_destroy(fnd_pt_ctx* ctx, fnd_handle h)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    auto it = ctx->buffers.find(h);
-    if (handle_kind(h) != kHandleBuffer || it == ctx->buffers.end())
-        return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
-    (void)hipStreamSynchronize(ctx->stream);
-    dev_free(ctx, it->second->ptr, it->second->bytes);
-    ctx_delete(ctx, it->second);
-    ctx->buffers.erase(it);
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        auto it = ctx->buffers.find(h);
+        if (handle_kind(h) != kHandleBuffer || it == ctx->buffers.end())
+            return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
+        (void)hipStreamSynchronize(ctx->stream);
+        dev_free(ctx, it->second->ptr, it->second->bytes);
+        ctx_delete(ctx, it->second);
+        ctx->buffers.erase(it);
+        return FND_OK;
+    });
 }
 
 static Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h)
@@ -549,22 +578,26 @@ static Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h)
 
 fnd_status fnd_pt_buffer_write(fnd_pt_ctx* ctx, fnd_handle h, uint64_t offset, const void* src, uint64_t bytes)
 {
-    if (!ctx || (!src && bytes)) return FND_E_INVALID_ARG;
-    Buffer* b = find_buffer(ctx, h);
-    if (!b) return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
-    if (offset > b->bytes || bytes > b->bytes - offset) return set_error(ctx, FND_E_INVALID_ARG, "write out of range");
-    if (!bytes) return FND_OK;
-    return copy_sync(ctx, static_cast<char*>(b->ptr) + offset, src, bytes, hipMemcpyHostToDevice, "buffer write");
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || (!src && bytes)) return FND_E_INVALID_ARG;
+        Buffer* b = find_buffer(ctx, h);
+        if (!b) return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
+        if (offset > b->bytes || bytes > b->bytes - offset) return set_error(ctx, FND_E_INVALID_ARG, "write out of range");
+        if (!bytes) return FND_OK;
+        return copy_sync(ctx, static_cast<char*>(b->ptr) + offset, src, bytes, hipMemcpyHostToDevice, "buffer write");
+    });
 }
 
 fnd_status fnd_pt_buffer_read(fnd_pt_ctx* ctx, fnd_handle h, uint64_t offset, void* dst, uint64_t bytes)
 {
-    if (!ctx || (!dst && bytes)) return FND_E_INVALID_ARG;
-    Buffer* b = find_buffer(ctx, h);
-    if (!b) return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
-    if (offset > b->bytes || bytes > b->bytes - offset) return set_error(ctx, FND_E_INVALID_ARG, "read out of range");
-    if (!bytes) return FND_OK;
-    return copy_sync(ctx, dst, static_cast<const char*>(b->ptr) + offset, bytes, hipMemcpyDeviceToHost, "buffer read");
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || (!dst && bytes)) return FND_E_INVALID_ARG;
+        Buffer* b = find_buffer(ctx, h);
+        if (!b) return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
+        if (offset > b->bytes || bytes > b->bytes - offset) return set_error(ctx, FND_E_INVALID_ARG, "read out of range");
+        if (!bytes) return FND_OK;
+        return copy_sync(ctx, dst, static_cast<const char*>(b->ptr) + offset, bytes, hipMemcpyDeviceToHost, "buffer read");
+    });
 }
 
 // ---------------------------------------------------------------- ray queries
@@ -629,49 +662,53 @@ static fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float
 fnd_status fnd_pt_trace_buffers(fnd_pt_ctx* ctx, fnd_handle scene, fnd_handle rays, fnd_handle hits, uint32_t n,
                                 uint32_t flags)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    Scene* s = find_scene(ctx, scene);
-    if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
-    Buffer* rb = find_buffer(ctx, rays);
-    Buffer* hb = find_buffer(ctx, hits);
-    if (!rb || !hb) return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
-    if (flags & ~(FND_TRACE_ANYHIT | FND_TRACE_COUNT | FND_ASYNC))
-        return set_error(ctx, FND_E_INVALID_ARG, "unknown trace flags 0x%x", flags);
-    if (uint64_t(n) * sizeof(Ray) > rb->bytes || uint64_t(n) * sizeof(Hit) > hb->bytes)
-        return set_error(ctx, FND_E_INVALID_ARG, "buffers too small for %u rays", n);
-    if (n == 0) return FND_OK;
-    return trace_device(ctx, s, static_cast<const Ray*>(rb->ptr), static_cast<float4*>(hb->ptr), n, flags);
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        Scene* s = find_scene(ctx, scene);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
+        Buffer* rb = find_buffer(ctx, rays);
+        Buffer* hb = find_buffer(ctx, hits);
+        if (!rb || !hb) return set_error(ctx, FND_E_NOT_FOUND, "unknown buffer handle");
+        if (flags & ~(FND_TRACE_ANYHIT | FND_TRACE_COUNT | FND_ASYNC))
+            return set_error(ctx, FND_E_INVALID_ARG, "unknown trace flags 0x%x", flags);
+        if (uint64_t(n) * sizeof(Ray) > rb->bytes || uint64_t(n) * sizeof(Hit) > hb->bytes)
+            return set_error(ctx, FND_E_INVALID_ARG, "buffers too small for %u rays", n);
+        if (n == 0) return FND_OK;
+        return trace_device(ctx, s, static_cast<const Ray*>(rb->ptr), static_cast<float4*>(hb->ptr), n, flags);
+    });
 }
 
 fnd_status fnd_pt_trace(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, uint32_t n, fnd_hit* hits, uint32_t flags)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    if (n && (!rays || !hits)) return set_error(ctx, FND_E_INVALID_ARG, "null ray or hit array");
-    Scene* s = find_scene(ctx, scene);
-    if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
-    if (flags & ~(FND_TRACE_ANYHIT | FND_TRACE_COUNT))
-        return set_error(ctx, FND_E_INVALID_ARG, "unknown trace flags 0x%x (FND_ASYNC needs device buffers)", flags);
-    if (n == 0) return FND_OK;
-    const uint64_t need = uint64_t(n) * (sizeof(Ray) + sizeof(Hit));
-    if (need > ctx->tmp_bytes) {
-        dev_free(ctx, ctx->d_tmp, ctx->tmp_bytes);
-        ctx->d_tmp = nullptr;
-        ctx->tmp_bytes = 0;
-        fnd_status st = dev_alloc(ctx, &ctx->d_tmp, need);
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        if (n && (!rays || !hits)) return set_error(ctx, FND_E_INVALID_ARG, "null ray or hit array");
+        Scene* s = find_scene(ctx, scene);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
+        if (flags & ~(FND_TRACE_ANYHIT | FND_TRACE_COUNT))
+            return set_error(ctx, FND_E_INVALID_ARG, "unknown trace flags 0x%x (FND_ASYNC needs device buffers)", flags);
+        if (n == 0) return FND_OK;
+        const uint64_t need = uint64_t(n) * (sizeof(Ray) + sizeof(Hit));
+        if (need > ctx->tmp_bytes) {
+            dev_free(ctx, ctx->d_tmp, ctx->tmp_bytes);
+            ctx->d_tmp = nullptr;
+            ctx->tmp_bytes = 0;
+            fnd_status st = dev_alloc(ctx, &ctx->d_tmp, need);
+            if (st != FND_OK) return st;
+            ctx->tmp_bytes = need;
+        }
+        Ray* dr = static_cast<Ray*>(ctx->d_tmp);
+        float4* dh = reinterpret_cast<float4*>(dr + n);
+        fnd_status st = check_hip(ctx, hipMemcpyAsync(dr, rays, uint64_t(n) * sizeof(Ray), hipMemcpyHostToDevice, ctx->stream),
+                                  "upload rays");
         if (st != FND_OK) return st;
-        ctx->tmp_bytes = need;
-    }
-    Ray* dr = static_cast<Ray*>(ctx->d_tmp);
-    float4* dh = reinterpret_cast<float4*>(dr + n);
-    fnd_status st = check_hip(ctx, hipMemcpyAsync(dr, rays, uint64_t(n) * sizeof(Ray), hipMemcpyHostToDevice, ctx->stream),
-                              "upload rays");
-    if (st != FND_OK) return st;
-    st = trace_device(ctx, s, dr, dh, n, flags);
-    if (st != FND_OK) return st;
-    st = check_hip(ctx, hipMemcpyAsync(hits, dh, uint64_t(n) * sizeof(Hit), hipMemcpyDeviceToHost, ctx->stream),
-                   "download hits");
-    if (st != FND_OK) return st;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+        st = trace_device(ctx, s, dr, dh, n, flags);
+        if (st != FND_OK) return st;
+        st = check_hip(ctx, hipMemcpyAsync(hits, dh, uint64_t(n) * sizeof(Hit), hipMemcpyDeviceToHost, ctx->stream),
+                       "download hits");
+        if (st != FND_OK) return st;
+        return check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+    });
 }
 
 }  // extern "C"

@@ -123,127 +123,129 @@ bool write_cache(const char* path, const std::string& key, const fnd_tuning& t, 
 extern "C" fnd_status fnd_pt_autotune(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs,
                                       const char* cache_path, uint32_t flags, fnd_autotune_result* out)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    if (!jobs || !num_jobs || !out) return set_error(ctx, FND_E_INVALID_ARG, "autotune needs jobs and a result");
-    if (flags & ~FND_AUTOTUNE_FORCE) return set_error(ctx, FND_E_INVALID_ARG, "unknown autotune flags 0x%x", flags);
-    std::memset(out, 0, sizeof(*out));
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        if (!jobs || !num_jobs || !out) return set_error(ctx, FND_E_INVALID_ARG, "autotune needs jobs and a result");
+        if (flags & ~FND_AUTOTUNE_FORCE) return set_error(ctx, FND_E_INVALID_ARG, "unknown autotune flags 0x%x", flags);
+        std::memset(out, 0, sizeof(*out));
 
-    // workload key and the timing jobs (1 spp, no training, scratch framebuffers)
-    uint64_t tris = 0, pixels = 0;
-    uint32_t depth = 0;
-    std::vector<fnd_render_job> tj(jobs, jobs + num_jobs);
-    std::vector<fnd_handle> fbs;
-    auto release = [&]() {
-        for (fnd_handle h : fbs) (void)fnd_pt_framebuffer_destroy(ctx, h);
-        fbs.clear();
-    };
-    std::set<fnd_handle> scenes;
-    for (uint32_t i = 0; i < num_jobs; ++i) {
-        fnd_scene_info info{};
-        fnd_status st = fnd_pt_scene_info(ctx, jobs[i].scene, &info);
-        if (st != FND_OK) {
-            release();
-            return st;
+        // workload key and the timing jobs (1 spp, no training, scratch framebuffers)
+        uint64_t tris = 0, pixels = 0;
+        uint32_t depth = 0;
+        std::vector<fnd_render_job> tj(jobs, jobs + num_jobs);
+        std::vector<fnd_handle> fbs;
+        auto release = [&]() {
+            for (fnd_handle h : fbs) (void)fnd_pt_framebuffer_destroy(ctx, h);
+            fbs.clear();
+        };
+        std::set<fnd_handle> scenes;
+        for (uint32_t i = 0; i < num_jobs; ++i) {
+            fnd_scene_info info{};
+            fnd_status st = fnd_pt_scene_info(ctx, jobs[i].scene, &info);
+            if (st != FND_OK) {
+                release();
+                return st;
+            }
+            if (scenes.insert(jobs[i].scene).second) tris += info.num_triangles;
+            Framebuffer* f = find_framebuffer(ctx, jobs[i].framebuffer);
+            if (!f) {
+                release();
+                return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown framebuffer handle", i);
+            }
+            pixels += uint64_t(f->width) * f->height;
+            depth = std::max(depth, jobs[i].max_depth);
+            fnd_handle h = FND_INVALID_HANDLE;
+            st = fnd_pt_framebuffer_create(ctx, f->width, f->height, 0, &h);
+            if (st != FND_OK) {
+                release();
+                return st;
+            }
+            fbs.push_back(h);
+            tj[i].framebuffer = h;
+            tj[i].spp = 1;
+            tj[i].flags &= ~FND_JOB_TRAIN_CACHE;
         }
-        if (scenes.insert(jobs[i].scene).second) tris += info.num_triangles;
-        Framebuffer* f = find_framebuffer(ctx, jobs[i].framebuffer);
-        if (!f) {
-            release();
-            return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown framebuffer handle", i);
-        }
-        pixels += uint64_t(f->width) * f->height;
-        depth = std::max(depth, jobs[i].max_depth);
-        fnd_handle h = FND_INVALID_HANDLE;
-        st = fnd_pt_framebuffer_create(ctx, f->width, f->height, 0, &h);
-        if (st != FND_OK) {
-            release();
-            return st;
-        }
-        fbs.push_back(h);
-        tj[i].framebuffer = h;
-        tj[i].spp = 1;
-        tj[i].flags &= ~FND_JOB_TRAIN_CACHE;
-    }
-    char key[128];
-    std::snprintf(key, sizeof key, "%s/cu%u/tri%d/px%d/d%u/j%d", ctx->arch.c_str(), ctx->num_cus, ilog2(tris),
-                  ilog2(pixels), depth, ilog2(num_jobs));
-    std::snprintf(out->key, sizeof out->key, "%s", key);
+        char key[128];
+        std::snprintf(key, sizeof key, "%s/cu%u/tri%d/px%d/d%u/j%d", ctx->arch, ctx->num_cus, ilog2(tris),
+                      ilog2(pixels), depth, ilog2(num_jobs));
+        std::snprintf(out->key, sizeof out->key, "%s", key);
 
-    fnd_tuning def{};
-    {
-        fnd_tuning zero{};
-        const fnd_tuning keep = ctx->tuning;
-        (void)fnd_pt_set_tuning(ctx, &zero);  // zeros select the library defaults
-        def = ctx->tuning;
-        ctx->tuning = keep;
-    }
-
-    if (!(flags & FND_AUTOTUNE_FORCE)) {
-        fnd_tuning t{};
-        double ms = 0;
-        if (read_cache(cache_path, key, t, ms) && fnd_pt_set_tuning(ctx, &t) == FND_OK) {
-            release();
-            out->tuning = ctx->tuning;
-            out->best_ms = ms;
-            out->from_cache = 1;
-            return FND_OK;
+        fnd_tuning def{};
+        {
+            fnd_tuning zero{};
+            const fnd_tuning keep = ctx->tuning;
+            (void)fnd_pt_set_tuning(ctx, &zero);  // zeros select the library defaults
+            def = ctx->tuning;
+            ctx->tuning = keep;
         }
-    }
 
-    // one timing measurement: a warm-up render (buffers sized, caches warm)
-    // then `reps` renders; the mean device time of those
-    auto time_one = [&](Candidate& c, uint32_t reps) -> fnd_status {
-        fnd_status st = fnd_pt_set_tuning(ctx, &c.t);
-        if (st != FND_OK) return st;
-        double sum = 0;
-        for (uint32_t r = 0; r <= reps; ++r) {
-            for (uint32_t i = 0; i < num_jobs; ++i) tj[i].sample_offset = jobs[i].sample_offset + r;
-            st = fnd_pt_render(ctx, tj.data(), num_jobs);
+        if (!(flags & FND_AUTOTUNE_FORCE)) {
+            fnd_tuning t{};
+            double ms = 0;
+            if (read_cache(cache_path, key, t, ms) && fnd_pt_set_tuning(ctx, &t) == FND_OK) {
+                release();
+                out->tuning = ctx->tuning;
+                out->best_ms = ms;
+                out->from_cache = 1;
+                return FND_OK;
+            }
+        }
+
+        // one timing measurement: a warm-up render (buffers sized, caches warm)
+        // then `reps` renders; the mean device time of those
+        auto time_one = [&](Candidate& c, uint32_t reps) -> fnd_status {
+            fnd_status st = fnd_pt_set_tuning(ctx, &c.t);
             if (st != FND_OK) return st;
-            out->renders += 1;
-            if (r > 0) sum += ctx->stats.last_call_ms;
-        }
-        c.ms = sum / reps;
-        return FND_OK;
-    };
+            double sum = 0;
+            for (uint32_t r = 0; r <= reps; ++r) {
+                for (uint32_t i = 0; i < num_jobs; ++i) tj[i].sample_offset = jobs[i].sample_offset + r;
+                st = fnd_pt_render(ctx, tj.data(), num_jobs);
+                if (st != FND_OK) return st;
+                out->renders += 1;
+                if (r > 0) sum += ctx->stats.last_call_ms;
+            }
+            c.ms = sum / reps;
+            return FND_OK;
+        };
 
-    const fnd_tuning keep = ctx->tuning;
-    std::vector<Candidate> pool = candidates(def, 27);
-    out->candidates = uint32_t(pool.size());
-    uint32_t reps = 1;
-    bool first = true;
-    fnd_status st = FND_OK;
-    while (st == FND_OK) {
-        for (auto& c : pool) {
-            st = time_one(c, reps);
+        const fnd_tuning keep = ctx->tuning;
+        std::vector<Candidate> pool = candidates(def, 27);
+        out->candidates = uint32_t(pool.size());
+        uint32_t reps = 1;
+        bool first = true;
+        fnd_status st = FND_OK;
+        while (st == FND_OK) {
+            for (auto& c : pool) {
+                st = time_one(c, reps);
+                if (st != FND_OK) break;
+            }
             if (st != FND_OK) break;
+            std::sort(pool.begin(), pool.end(), [](const Candidate& a, const Candidate& b) { return a.ms < b.ms; });
+            if (first) {
+                out->median_ms = pool[pool.size() / 2].ms;
+                out->median = pool[pool.size() / 2].t;
+                for (const auto& c : pool)
+                    if (c.t.block_size == def.block_size && c.t.stack_lds == def.stack_lds &&
+                        c.t.waves_per_cu == def.waves_per_cu && c.t.refill == def.refill &&
+                        c.t.queue_capacity == def.queue_capacity && c.t.tile == def.tile)
+                        out->default_ms = c.ms;
+                first = false;
+            }
+            if (pool.size() == 1) break;
+            pool.resize((pool.size() + 2) / 3);
+            reps *= 2;
         }
-        if (st != FND_OK) break;
-        std::sort(pool.begin(), pool.end(), [](const Candidate& a, const Candidate& b) { return a.ms < b.ms; });
-        if (first) {
-            out->median_ms = pool[pool.size() / 2].ms;
-            out->median = pool[pool.size() / 2].t;
-            for (const auto& c : pool)
-                if (c.t.block_size == def.block_size && c.t.stack_lds == def.stack_lds &&
-                    c.t.waves_per_cu == def.waves_per_cu && c.t.refill == def.refill &&
-                    c.t.queue_capacity == def.queue_capacity && c.t.tile == def.tile)
-                    out->default_ms = c.ms;
-            first = false;
+        release();
+        if (st != FND_OK) {
+            ctx->tuning = keep;
+            return st;
         }
-        if (pool.size() == 1) break;
-        pool.resize((pool.size() + 2) / 3);
-        reps *= 2;
-    }
-    release();
-    if (st != FND_OK) {
-        ctx->tuning = keep;
-        return st;
-    }
-    st = fnd_pt_set_tuning(ctx, &pool[0].t);
-    if (st != FND_OK) return st;
-    out->tuning = ctx->tuning;
-    out->best_ms = pool[0].ms;
-    if (cache_path && !write_cache(cache_path, key, pool[0].t, pool[0].ms))
-        return set_error(ctx, FND_E_INVALID_ARG, "cannot write autotune cache %s", cache_path);
-    return FND_OK;
+        st = fnd_pt_set_tuning(ctx, &pool[0].t);
+        if (st != FND_OK) return st;
+        out->tuning = ctx->tuning;
+        out->best_ms = pool[0].ms;
+        if (cache_path && !write_cache(cache_path, key, pool[0].t, pool[0].ms))
+            return set_error(ctx, FND_E_INVALID_ARG, "cannot write autotune cache %s", cache_path);
+        return FND_OK;
+    });
 }

@@ -6,6 +6,7 @@
 // has 4 slots (as in Wald et al., "Getting Rid of Packets", IEEE RT 2008);
 // quantisation after Ylitie et al., HPG 2017, with outward rounding.
 #include "bvh_build.hpp"
+#include "host_pool.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -112,10 +113,7 @@ public:
             }
         };
         const unsigned nt = std::max(1u, std::min<unsigned>(threads, unsigned(frontier.size())));
-        std::vector<std::thread> pool;
-        for (unsigned i = 1; i < nt; ++i) pool.emplace_back(worker);
-        worker();
-        for (auto& th : pool) th.join();
+        run_pool(nt, worker);
     }
 
     const std::vector<BNode>& nodes() const { return nodes_; }

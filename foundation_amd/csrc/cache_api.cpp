@@ -202,196 +202,210 @@ fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* d)
 
 fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* d)
 {
-    if (!ctx || !d) return FND_E_INVALID_ARG;
-    if (!d->enable) {
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !d) return FND_E_INVALID_ARG;
+        if (!d->enable) {
+            cache_free(ctx);
+            return FND_OK;
+        }
+        fnd_status st = validate(ctx, *d);
+        if (st != FND_OK) return st;
         cache_free(ctx);
-        return FND_OK;
-    }
-    fnd_status st = validate(ctx, *d);
-    if (st != FND_OK) return st;
-    cache_free(ctx);
-    auto* cs = ctx_new<CacheState>(ctx);
-    if (!cs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
-    ctx->cache = cs;
-    cs->desc = *d;
-    auto fail = [&](fnd_status e) {
-        cache_free(ctx);
-        return e;
-    };
-    if ((st = calloc_dev(ctx, cs, &cs->w_master, kMlpParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->w_m, kMlpParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->w_v, kMlpParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->wfwd, kMlpFwdElems)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->wbwd, kMlpBwdElems)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->g_master, kGridParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->g_m, kGridParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->g_v, kGridParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->grid16, kGridParams / 2)) != FND_OK) return fail(st);
-    CacheTrainBufs& tb = cs->tb;
-    tb.cap = kCacheBatch;
-    if ((st = calloc_dev(ctx, cs, &tb.act, uint64_t(kActRows) * tb.cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &tb.dz, uint64_t(kDzRows) * tb.cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &tb.pred, tb.cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &tb.grad_mlp, kMlpParams)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &tb.grad_grid, kGridParams / 2)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &tb.loss, 1)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->io_query, tb.cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->io_target, tb.cap)) != FND_OK) return fail(st);
-    cs->sample_cap = kCacheTrainMaxPaths * d->train_depth;
-    if ((st = calloc_dev(ctx, cs, &cs->s_query, cs->sample_cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->s_target, cs->sample_cap)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->rec, uint64_t(kCacheTrainMaxPaths) * kCacheMaxRecords * kCacheRecordF4)) != FND_OK)
-        return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->rec_count, kCacheTrainMaxPaths)) != FND_OK) return fail(st);
-    if ((st = calloc_dev(ctx, cs, &cs->t_list, kCacheTrainMaxPaths)) != FND_OK) return fail(st);
-    // level resolutions: N_l = floor(N_min * b^l), b = (N_max / N_min)^(1 / (L - 1))
-    const double b = std::exp((std::log(2048.0) - std::log(16.0)) / double(kCacheLevels - 1));
-    cs->view.dense_mask = 0;
-    for (int l = 0; l < kCacheLevels; ++l) {
-        const int r = int(std::floor(16.0 * std::pow(b, double(l)) + 1e-9));
-        cs->view.res[l] = r;
-        const double cells = double(r + 1) * double(r + 1) * double(r + 1);
-        if (cells <= double(kCacheT)) cs->view.dense_mask |= 1u << l;
-    }
-    cs->view.grid16 = cs->grid16;
-    cs->view.wfwd = cs->wfwd;
-    cs->view.wbwd = cs->wbwd;
-    if (d->bounds_hi[0] > d->bounds_lo[0] || d->bounds_hi[1] > d->bounds_lo[1] || d->bounds_hi[2] > d->bounds_lo[2])
-        cache_set_bounds(cs, d->bounds_lo, d->bounds_hi, false);
-    // initial parameters: MLP uniform +-sqrt(6 / (fan_in + fan_out)) (Glorot), grid uniform +-1e-4
-    std::mt19937_64 rng(d->seed);
-    std::vector<float> w(kMlpParams);
-    auto fill = [&](int off, int fin, int fout) {
-        const float lim = std::sqrt(6.0f / float(fin + fout));
-        std::uniform_real_distribution<float> U(-lim, lim);
-        for (int i = 0; i < fin * fout; ++i) w[off + i] = U(rng);
-    };
-    fill(kW0, kCacheIn, kCacheHid);
-    for (int l = 0; l < 4; ++l) fill(kW1 + l * 4096, kCacheHid, kCacheHid);
-    fill(kW5, kCacheHid, 16);
-    std::vector<float> gridv(kGridParams);
-    std::uniform_real_distribution<float> G(-1e-4f, 1e-4f);
-    for (auto& x : gridv) x = G(rng);
-    return fnd_pt_cache_set_params(ctx, w.data(), gridv.data());
+        auto* cs = ctx_new<CacheState>(ctx);
+        if (!cs) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+        ctx->cache = cs;
+        cs->desc = *d;
+        auto fail = [&](fnd_status e) {
+            cache_free(ctx);
+            return e;
+        };
+        if ((st = calloc_dev(ctx, cs, &cs->w_master, kMlpParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->w_m, kMlpParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->w_v, kMlpParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->wfwd, kMlpFwdElems)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->wbwd, kMlpBwdElems)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->g_master, kGridParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->g_m, kGridParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->g_v, kGridParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->grid16, kGridParams / 2)) != FND_OK) return fail(st);
+        CacheTrainBufs& tb = cs->tb;
+        tb.cap = kCacheBatch;
+        if ((st = calloc_dev(ctx, cs, &tb.act, uint64_t(kActRows) * tb.cap)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &tb.dz, uint64_t(kDzRows) * tb.cap)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &tb.pred, tb.cap)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &tb.grad_mlp, kMlpParams)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &tb.grad_grid, kGridParams / 2)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &tb.loss, 1)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->io_query, tb.cap)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->io_target, tb.cap)) != FND_OK) return fail(st);
+        cs->sample_cap = kCacheTrainMaxPaths * d->train_depth;
+        if ((st = calloc_dev(ctx, cs, &cs->s_query, cs->sample_cap)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->s_target, cs->sample_cap)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->rec, uint64_t(kCacheTrainMaxPaths) * kCacheMaxRecords * kCacheRecordF4)) != FND_OK)
+            return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->rec_count, kCacheTrainMaxPaths)) != FND_OK) return fail(st);
+        if ((st = calloc_dev(ctx, cs, &cs->t_list, kCacheTrainMaxPaths)) != FND_OK) return fail(st);
+        // level resolutions: N_l = floor(N_min * b^l), b = (N_max / N_min)^(1 / (L - 1))
+        const double b = std::exp((std::log(2048.0) - std::log(16.0)) / double(kCacheLevels - 1));
+        cs->view.dense_mask = 0;
+        for (int l = 0; l < kCacheLevels; ++l) {
+            const int r = int(std::floor(16.0 * std::pow(b, double(l)) + 1e-9));
+            cs->view.res[l] = r;
+            const double cells = double(r + 1) * double(r + 1) * double(r + 1);
+            if (cells <= double(kCacheT)) cs->view.dense_mask |= 1u << l;
+        }
+        cs->view.grid16 = cs->grid16;
+        cs->view.wfwd = cs->wfwd;
+        cs->view.wbwd = cs->wbwd;
+        if (d->bounds_hi[0] > d->bounds_lo[0] || d->bounds_hi[1] > d->bounds_lo[1] || d->bounds_hi[2] > d->bounds_lo[2])
+            cache_set_bounds(cs, d->bounds_lo, d->bounds_hi, false);
+        // initial parameters: MLP uniform +-sqrt(6 / (fan_in + fan_out)) (Glorot), grid uniform +-1e-4
+        std::mt19937_64 rng(d->seed);
+        std::vector<float> w(kMlpParams);
+        auto fill = [&](int off, int fin, int fout) {
+            const float lim = std::sqrt(6.0f / float(fin + fout));
+            std::uniform_real_distribution<float> U(-lim, lim);
+            for (int i = 0; i < fin * fout; ++i) w[off + i] = U(rng);
+        };
+        fill(kW0, kCacheIn, kCacheHid);
+        for (int l = 0; l < 4; ++l) fill(kW1 + l * 4096, kCacheHid, kCacheHid);
+        fill(kW5, kCacheHid, 16);
+        std::vector<float> gridv(kGridParams);
+        std::uniform_real_distribution<float> G(-1e-4f, 1e-4f);
+        for (auto& x : gridv) x = G(rng);
+        return fnd_pt_cache_set_params(ctx, w.data(), gridv.data());
+    });
 }
 
 fnd_status fnd_pt_cache_param_count(fnd_pt_ctx* ctx, uint64_t* mlp, uint64_t* grid)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    if (mlp) *mlp = kMlpParams;
-    if (grid) *grid = kGridParams;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        if (mlp) *mlp = kMlpParams;
+        if (grid) *grid = kGridParams;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_cache_set_params(fnd_pt_ctx* ctx, const float* mlp, const float* grid)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    CacheState* cs = cache_get(ctx);
-    if (!cs) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
-    fnd_status st = FND_OK;
-    if (mlp) {
-        st = check_hip(ctx, hipMemcpyAsync(cs->w_master, mlp, kMlpParams * 4, hipMemcpyHostToDevice, ctx->stream),
-                       "upload weights");
-        if (st != FND_OK) return st;
-        launch_arrange(cs->w_master, cs->wfwd, cs->wbwd, ctx->stream);
-    }
-    if (grid) {
-        std::vector<uint32_t> h16(kGridParams / 2);
-        for (uint64_t i = 0; i < kGridParams / 2; ++i) {
-            const _Float16 a = _Float16(grid[2 * i]), b = _Float16(grid[2 * i + 1]);
-            uint16_t ua, ub;
-            std::memcpy(&ua, &a, 2);
-            std::memcpy(&ub, &b, 2);
-            h16[i] = uint32_t(ua) | (uint32_t(ub) << 16);
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        CacheState* cs = cache_get(ctx);
+        if (!cs) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
+        fnd_status st = FND_OK;
+        if (mlp) {
+            st = check_hip(ctx, hipMemcpyAsync(cs->w_master, mlp, kMlpParams * 4, hipMemcpyHostToDevice, ctx->stream),
+                           "upload weights");
+            if (st != FND_OK) return st;
+            launch_arrange(cs->w_master, cs->wfwd, cs->wbwd, ctx->stream);
         }
-        st = check_hip(ctx, hipMemcpyAsync(cs->g_master, grid, kGridParams * 4, hipMemcpyHostToDevice, ctx->stream),
-                       "upload grid");
-        if (st == FND_OK)
-            st = check_hip(ctx, hipMemcpyAsync(cs->grid16, h16.data(), kGridParams * 2, hipMemcpyHostToDevice, ctx->stream),
+        if (grid) {
+            std::vector<uint32_t> h16(kGridParams / 2);
+            for (uint64_t i = 0; i < kGridParams / 2; ++i) {
+                const _Float16 a = _Float16(grid[2 * i]), b = _Float16(grid[2 * i + 1]);
+                uint16_t ua, ub;
+                std::memcpy(&ua, &a, 2);
+                std::memcpy(&ub, &b, 2);
+                h16[i] = uint32_t(ua) | (uint32_t(ub) << 16);
+            }
+            st = check_hip(ctx, hipMemcpyAsync(cs->g_master, grid, kGridParams * 4, hipMemcpyHostToDevice, ctx->stream),
                            "upload grid");
-        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "upload grid");
-        if (st != FND_OK) return st;
-    }
-    // a new parameter set restarts the optimiser
-    launch_zero(cs->w_m, kMlpParams, ctx->stream);
-    launch_zero(cs->w_v, kMlpParams, ctx->stream);
-    launch_zero(cs->g_m, kGridParams, ctx->stream);
-    launch_zero(cs->g_v, kGridParams, ctx->stream);
-    cs->step = 0;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "set params");
+            if (st == FND_OK)
+                st = check_hip(ctx, hipMemcpyAsync(cs->grid16, h16.data(), kGridParams * 2, hipMemcpyHostToDevice, ctx->stream),
+                               "upload grid");
+            if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "upload grid");
+            if (st != FND_OK) return st;
+        }
+        // a new parameter set restarts the optimiser
+        launch_zero(cs->w_m, kMlpParams, ctx->stream);
+        launch_zero(cs->w_v, kMlpParams, ctx->stream);
+        launch_zero(cs->g_m, kGridParams, ctx->stream);
+        launch_zero(cs->g_v, kGridParams, ctx->stream);
+        cs->step = 0;
+        return check_hip(ctx, hipStreamSynchronize(ctx->stream), "set params");
+    });
 }
 
 fnd_status fnd_pt_cache_get_params(fnd_pt_ctx* ctx, float* mlp, float* grid)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    CacheState* cs = cache_get(ctx);
-    if (!cs) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
-    fnd_status st = FND_OK;
-    if (mlp)
-        st = check_hip(ctx, hipMemcpyAsync(mlp, cs->w_master, kMlpParams * 4, hipMemcpyDeviceToHost, ctx->stream),
-                       "read weights");
-    if (st == FND_OK && grid)
-        st = check_hip(ctx, hipMemcpyAsync(grid, cs->g_master, kGridParams * 4, hipMemcpyDeviceToHost, ctx->stream),
-                       "read grid");
-    if (st != FND_OK) return st;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "read params");
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        CacheState* cs = cache_get(ctx);
+        if (!cs) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
+        fnd_status st = FND_OK;
+        if (mlp)
+            st = check_hip(ctx, hipMemcpyAsync(mlp, cs->w_master, kMlpParams * 4, hipMemcpyDeviceToHost, ctx->stream),
+                           "read weights");
+        if (st == FND_OK && grid)
+            st = check_hip(ctx, hipMemcpyAsync(grid, cs->g_master, kGridParams * 4, hipMemcpyDeviceToHost, ctx->stream),
+                           "read grid");
+        if (st != FND_OK) return st;
+        return check_hip(ctx, hipStreamSynchronize(ctx->stream), "read params");
+    });
 }
 
 fnd_status fnd_pt_cache_infer(fnd_pt_ctx* ctx, const float* queries, uint32_t n, float* out)
 {
-    CacheState* cs;
-    fnd_status st = need_cache(ctx, &cs);
-    if (st != FND_OK) return st;
-    if (n && (!queries || !out)) return set_error(ctx, FND_E_INVALID_ARG, "null array");
-    for (uint32_t b0 = 0; b0 < n; b0 += cs->tb.cap) {
-        const uint32_t m = std::min(cs->tb.cap, n - b0);
-        st = upload_queries(ctx, cs, queries + 10ull * b0, nullptr, m);
+    return guard(ctx, [&]() -> fnd_status {
+        CacheState* cs;
+        fnd_status st = need_cache(ctx, &cs);
         if (st != FND_OK) return st;
-        const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (m + 127u) / 128u));
-        launch_cache_forward(cs->view, cs->io_query, 1, m, nullptr, cs->tb.pred, nullptr, 0, grid, ctx->stream);
-        std::vector<float4> hp(m);
-        st = check_hip(ctx, hipMemcpyAsync(hp.data(), cs->tb.pred, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream),
-                       "read predictions");
-        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "infer");
-        if (st != FND_OK) return st;
-        for (uint32_t i = 0; i < m; ++i) {
-            out[3ull * (b0 + i)] = hp[i].x;
-            out[3ull * (b0 + i) + 1] = hp[i].y;
-            out[3ull * (b0 + i) + 2] = hp[i].z;
+        if (n && (!queries || !out)) return set_error(ctx, FND_E_INVALID_ARG, "null array");
+        for (uint32_t b0 = 0; b0 < n; b0 += cs->tb.cap) {
+            const uint32_t m = std::min(cs->tb.cap, n - b0);
+            st = upload_queries(ctx, cs, queries + 10ull * b0, nullptr, m);
+            if (st != FND_OK) return st;
+            const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (m + 127u) / 128u));
+            launch_cache_forward(cs->view, cs->io_query, 1, m, nullptr, cs->tb.pred, nullptr, 0, grid, ctx->stream);
+            std::vector<float4> hp(m);
+            st = check_hip(ctx, hipMemcpyAsync(hp.data(), cs->tb.pred, m * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream),
+                           "read predictions");
+            if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "infer");
+            if (st != FND_OK) return st;
+            for (uint32_t i = 0; i < m; ++i) {
+                out[3ull * (b0 + i)] = hp[i].x;
+                out[3ull * (b0 + i) + 1] = hp[i].y;
+                out[3ull * (b0 + i) + 2] = hp[i].z;
+            }
         }
-    }
-    return FND_OK;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_cache_gradients(fnd_pt_ctx* ctx, const float* queries, const float* targets, uint32_t n, float* loss,
                                   float* dmlp, float* dgrid)
 {
-    CacheState* cs;
-    fnd_status st = need_cache(ctx, &cs);
-    if (st != FND_OK) return st;
-    if (!queries || !targets || n == 0) return set_error(ctx, FND_E_INVALID_ARG, "empty batch");
-    st = upload_queries(ctx, cs, queries, targets, n);
-    if (st != FND_OK) return st;
-    float l = 0;
-    st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, false, dmlp, dgrid, &l);
-    if (loss) *loss = l;
-    return st;
+    return guard(ctx, [&]() -> fnd_status {
+        CacheState* cs;
+        fnd_status st = need_cache(ctx, &cs);
+        if (st != FND_OK) return st;
+        if (!queries || !targets || n == 0) return set_error(ctx, FND_E_INVALID_ARG, "empty batch");
+        st = upload_queries(ctx, cs, queries, targets, n);
+        if (st != FND_OK) return st;
+        float l = 0;
+        st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, false, dmlp, dgrid, &l);
+        if (loss) *loss = l;
+        return st;
+    });
 }
 
 fnd_status fnd_pt_cache_train(fnd_pt_ctx* ctx, const float* queries, const float* targets, uint32_t n, uint32_t steps,
                               float* loss)
 {
-    CacheState* cs;
-    fnd_status st = need_cache(ctx, &cs);
-    if (st != FND_OK) return st;
-    if (!queries || !targets || n == 0) return set_error(ctx, FND_E_INVALID_ARG, "empty batch");
-    st = upload_queries(ctx, cs, queries, targets, n);
-    if (st != FND_OK) return st;
-    float l = 0;
-    for (uint32_t k = 0; k < steps && st == FND_OK; ++k)
-        st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, true, nullptr, nullptr, k + 1 == steps ? &l : nullptr);
-    if (loss) *loss = l;
-    return st;
+    return guard(ctx, [&]() -> fnd_status {
+        CacheState* cs;
+        fnd_status st = need_cache(ctx, &cs);
+        if (st != FND_OK) return st;
+        if (!queries || !targets || n == 0) return set_error(ctx, FND_E_INVALID_ARG, "empty batch");
+        st = upload_queries(ctx, cs, queries, targets, n);
+        if (st != FND_OK) return st;
+        float l = 0;
+        for (uint32_t k = 0; k < steps && st == FND_OK; ++k)
+            st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, true, nullptr, nullptr, k + 1 == steps ? &l : nullptr);
+        if (loss) *loss = l;
+        return st;
+    });
 }
 
 }  // extern "C"

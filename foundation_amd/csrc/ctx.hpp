@@ -4,8 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <functional>
 #include <map>
-#include <string>
+#include <stdexcept>
 #include <vector>
 
 #include "../../include/fnd_pt.h"
@@ -48,9 +49,54 @@ struct Framebuffer {
     uint32_t samples = 0;
 };
 
+// Host memory of a context comes from the caller's allocator (malloc/free
+// when none was given); every context-owned object is made and released here.
+void* host_alloc(const fnd_host_allocator* a, size_t size, size_t align);
+void host_free(const fnd_host_allocator* a, void* p);
+
+// Standard-library allocator over the caller's fnd_host_allocator (null:
+// malloc), so the context's tables and a call's temporaries take their
+// memory where the caller said.  A refused allocation throws std::bad_alloc,
+// which the entry point's guard turns into FND_E_OOM_HOST.
+template <class T>
+struct CtxAlloc {
+    using value_type = T;
+    const fnd_host_allocator* a = nullptr;
+    CtxAlloc() = default;
+    explicit CtxAlloc(const fnd_host_allocator* x) : a(x) {}
+    template <class U>
+    CtxAlloc(const CtxAlloc<U>& o) : a(o.a) {}
+    T* allocate(size_t n)
+    {
+        if (n > size_t(-1) / sizeof(T)) throw std::bad_alloc();
+        void* p = host_alloc(a, n * sizeof(T), alignof(T) < 16 ? 16 : alignof(T));
+        if (!p) throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { host_free(a, p); }
+    template <class U>
+    bool operator==(const CtxAlloc<U>& o) const { return a == o.a; }
+    template <class U>
+    bool operator!=(const CtxAlloc<U>& o) const { return a != o.a; }
+};
+
+template <class T>
+using CtxVector = std::vector<T, CtxAlloc<T>>;
+template <class K, class V>
+using CtxMap = std::map<K, V, std::less<K>, CtxAlloc<std::pair<const K, V>>>;
+
 }  // namespace fnd
 
 struct fnd_pt_ctx {
+    // `a`: the caller's allocator, copied (null: malloc/free)
+    explicit fnd_pt_ctx(const fnd_host_allocator* a)
+        : alloc(a ? *a : fnd_host_allocator{}), has_alloc(a != nullptr), timing_events(host_alloc_ref()),
+          scenes(host_alloc_ref()), buffers(host_alloc_ref()), framebuffers(host_alloc_ref())
+    {
+    }
+    const fnd_host_allocator* host_alloc_ptr() const { return has_alloc ? &alloc : nullptr; }
+    fnd::CtxAlloc<char> host_alloc_ref() const { return fnd::CtxAlloc<char>(host_alloc_ptr()); }
+
     fnd_host_allocator alloc{};
     bool has_alloc = false;
     fnd_log_fn log = nullptr;
@@ -59,19 +105,19 @@ struct fnd_pt_ctx {
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                // radiance-cache inference overlapped with a frame's last bounces
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::vector<hipEvent_t> timing_events;     // pairs (start, end) per pending trace launch
+    fnd::CtxVector<hipEvent_t> timing_events;  // pairs (start, end) per pending trace launch
     size_t timing_used = 0;                    // events in use (even)
-    std::string last_error;
+    char last_error[512] = {};                 // fixed storage: recording a failure allocates nothing
     uint64_t budget = 0;
     uint64_t used = 0;
     uint32_t num_cus = 0;
-    std::string arch;
+    char arch[64] = {};
     fnd_tuning tuning{};
     fnd_pt_stats stats{};
     uint64_t next_id = 1;
-    std::map<uint64_t, fnd::Scene*> scenes;
-    std::map<uint64_t, fnd::Buffer*> buffers;
-    std::map<uint64_t, fnd::Framebuffer*> framebuffers;
+    fnd::CtxMap<uint64_t, fnd::Scene*> scenes;
+    fnd::CtxMap<uint64_t, fnd::Buffer*> buffers;
+    fnd::CtxMap<uint64_t, fnd::Framebuffer*> framebuffers;
     // scratch owned by the context
     float* d_present = nullptr;                // 255 byte thresholds of the RGBA8 transfer
     uint32_t* d_counters = nullptr;            // 64 work counters
@@ -103,10 +149,25 @@ fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);
 void render_state_free(fnd_pt_ctx* ctx);
 void cache_free(fnd_pt_ctx* ctx);
 
-// Host memory of a context comes from the caller's allocator (malloc/free
-// when none was given); every context-owned object is made and released here.
-void* host_alloc(const fnd_host_allocator* a, size_t size, size_t align);
-void host_free(const fnd_host_allocator* a, void* p);
+// Runs the body of an entry point.  No exception crosses the C boundary: a
+// host allocation failure anywhere below (a container on the caller's
+// allocator, a worker thread of a builder) becomes FND_E_OOM_HOST with a
+// message, anything else FND_E_DEVICE.
+template <class F>
+fnd_status guard(fnd_pt_ctx* ctx, F&& body) noexcept
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+    } catch (const std::length_error&) {
+        return set_error(ctx, FND_E_OOM_HOST, "out of host memory (container size limit)");
+    } catch (const std::exception& e) {
+        return set_error(ctx, FND_E_DEVICE, "internal error: %s", e.what());
+    } catch (...) {
+        return set_error(ctx, FND_E_DEVICE, "internal error");
+    }
+}
 
 template <class T>
 T* ctx_new(fnd_pt_ctx* ctx)

@@ -14,11 +14,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "../../include/fnd_pt.h"
+#include "host_pool.hpp"
 
 namespace {
 
@@ -114,70 +116,65 @@ extern "C" fnd_status fnd_obj_load(const char* path, fnd_obj_mesh** out, char* e
     if (err && err_len) err[0] = '\0';
     if (!path || !out) return fail(err, err_len, FND_E_INVALID_ARG, "null path or output");
     *out = nullptr;
-    std::FILE* fp = std::fopen(path, "rb");
-    if (!fp) return fail(err, err_len, FND_E_NOT_FOUND, "cannot open %s", path);
-    fnd_obj_mesh* m = new (std::nothrow) fnd_obj_mesh();
-    if (!m) {
-        std::fclose(fp);
-        return FND_E_OOM_HOST;
-    }
-    std::vector<char> line(1 << 16);
-    uint64_t lineno = 0;
-    std::vector<uint32_t> face;
-    fnd_status st = FND_OK;
-    while (std::fgets(line.data(), int(line.size()), fp)) {
-        ++lineno;
-        if (!std::strchr(line.data(), '\n') && !std::feof(fp)) {
-            st = fail(err, err_len, FND_E_INVALID_ARG, "%s:%llu: line longer than %zu bytes", path,
-                      (unsigned long long)lineno, line.size() - 1);
-            break;
-        }
-        const char* s = line.data();
-        while (*s == ' ' || *s == '\t') ++s;
-        if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
-            float p[3];
-            if (std::sscanf(s + 2, "%f %f %f", &p[0], &p[1], &p[2]) != 3) {
-                st = fail(err, err_len, FND_E_INVALID_ARG, "%s:%llu: vertex needs three coordinates", path,
-                          (unsigned long long)lineno);
+    std::unique_ptr<std::FILE, int (*)(std::FILE*)> file(std::fopen(path, "rb"), &std::fclose);
+    if (!file) return fail(err, err_len, FND_E_NOT_FOUND, "cannot open %s", path);
+    std::FILE* fp = file.get();
+    return fnd::guard_host(err, err_len, [&]() -> fnd_status {
+        std::unique_ptr<fnd_obj_mesh> m(new fnd_obj_mesh());
+        std::vector<char> line(1 << 16);
+        uint64_t lineno = 0;
+        std::vector<uint32_t> face;
+        fnd_status st = FND_OK;
+        while (std::fgets(line.data(), int(line.size()), fp)) {
+            ++lineno;
+            if (!std::strchr(line.data(), '\n') && !std::feof(fp)) {
+                st = fail(err, err_len, FND_E_INVALID_ARG, "%s:%llu: line longer than %zu bytes", path,
+                          (unsigned long long)lineno, line.size() - 1);
                 break;
             }
-            m->positions.insert(m->positions.end(), p, p + 3);
-        } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
-            s += 2;
-            face.clear();
-            const long long nverts = (long long)(m->positions.size() / 3);
-            bool ok = true;
-            for (;;) {
-                while (*s == ' ' || *s == '\t') ++s;
-                if (!*s || *s == '\r' || *s == '\n' || *s == '#') break;
-                uint32_t v;
-                if (!parse_ref(s, nverts, v)) {
-                    ok = false;
+            const char* s = line.data();
+            while (*s == ' ' || *s == '\t') ++s;
+            if (s[0] == 'v' && (s[1] == ' ' || s[1] == '\t')) {
+                float p[3];
+                if (std::sscanf(s + 2, "%f %f %f", &p[0], &p[1], &p[2]) != 3) {
+                    st = fail(err, err_len, FND_E_INVALID_ARG, "%s:%llu: vertex needs three coordinates", path,
+                              (unsigned long long)lineno);
                     break;
                 }
-                face.push_back(v);
+                m->positions.insert(m->positions.end(), p, p + 3);
+            } else if (s[0] == 'f' && (s[1] == ' ' || s[1] == '\t')) {
+                s += 2;
+                face.clear();
+                const long long nverts = (long long)(m->positions.size() / 3);
+                bool ok = true;
+                for (;;) {
+                    while (*s == ' ' || *s == '\t') ++s;
+                    if (!*s || *s == '\r' || *s == '\n' || *s == '#') break;
+                    uint32_t v;
+                    if (!parse_ref(s, nverts, v)) {
+                        ok = false;
+                        break;
+                    }
+                    face.push_back(v);
+                }
+                if (!ok || face.size() < 3) {
+                    st = fail(err, err_len, FND_E_INVALID_ARG, "%s:%llu: bad face", path, (unsigned long long)lineno);
+                    break;
+                }
+                for (size_t k = 1; k + 1 < face.size(); ++k) {  // fan
+                    m->indices.push_back(face[0]);
+                    m->indices.push_back(face[k]);
+                    m->indices.push_back(face[k + 1]);
+                }
             }
-            if (!ok || face.size() < 3) {
-                st = fail(err, err_len, FND_E_INVALID_ARG, "%s:%llu: bad face", path, (unsigned long long)lineno);
-                break;
-            }
-            for (size_t k = 1; k + 1 < face.size(); ++k) {  // fan
-                m->indices.push_back(face[0]);
-                m->indices.push_back(face[k]);
-                m->indices.push_back(face[k + 1]);
-            }
+            // other statements (vt, vn, o, g, usemtl, s, comments) carry nothing this path uses
         }
-        // other statements (vt, vn, o, g, usemtl, s, comments) carry nothing this path uses
-    }
-    std::fclose(fp);
-    if (st == FND_OK && m->positions.size() / 3 > 0xFFFFFFFFull)
-        st = fail(err, err_len, FND_E_INVALID_ARG, "%s: too many vertices", path);
-    if (st != FND_OK) {
-        delete m;
-        return st;
-    }
-    *out = m;
-    return FND_OK;
+        if (st == FND_OK && m->positions.size() / 3 > 0xFFFFFFFFull)
+            st = fail(err, err_len, FND_E_INVALID_ARG, "%s: too many vertices", path);
+        if (st != FND_OK) return st;
+        *out = m.release();
+        return FND_OK;
+    });
 }
 
 extern "C" fnd_status fnd_obj_mesh_get(const fnd_obj_mesh* m, fnd_mesh_desc* out)

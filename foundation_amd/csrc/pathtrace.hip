@@ -139,7 +139,7 @@ __device__ __forceinline__ float light_weight(const DeviceLight& lt, V3 p, V3 n)
     for (int k = 0; k < 4; ++k) {
         const V3 v = cs[k] - p;
         const float l2 = dot(v, v);
-        if (l2 > 0.0f) cx = fmaxf(cx, dot(n, v) * rsqrtf(l2));
+        if (l2 > 0.0f) cx = fmaxf(cx, dot(n, v) / sqrtf(l2));
     }
     if (!(cx > 0.0f)) return 0.0f;
     const V3 v = c0 + (e0 + e1) * 0.5f - p;
@@ -198,7 +198,10 @@ __device__ __forceinline__ float light_pick_pdf(const ShadeScene& sc, uint32_t l
         total += w;
         if (l == li) wl = w;
     }
-    return total > 0.0f ? 0.9f * wl / total + 0.1f * sc.lights[li].pdf_select : sc.lights[li].pdf_select;
+    // the same products pick_light forms, so the two agree to the bit
+    const float inv = total > 0.0f ? 0.9f / total : 0.0f;
+    const float floor_share = total > 0.0f ? 0.1f : 1.0f;
+    return wl * inv + floor_share * sc.lights[li].pdf_select;
 }
 
 __device__ __forceinline__ float ray_eps(V3 p)
@@ -230,6 +233,27 @@ __device__ __forceinline__ PathSite locate(const WfArgs& a, uint64_t g)
     return q;
 }
 
+// Camera ray of pixel (x, y) = pixel index `pixel` for `sample`: glm::lookAt
+// basis of the job (formed on the host in double), vertical field of view,
+// row 0 at the top; the jitter is the first two draws of the path's bounce-0
+// generator.
+__device__ __forceinline__ void camera_ray(const JobDev& job, uint32_t x, uint32_t y, uint32_t sample, uint32_t pixel,
+                                           Ray& ray)
+{
+    Rng rng = rng_for(job.seed, sample, pixel, 0u);
+    const float jx = rng_float(rng), jy = rng_float(rng);
+    const float sx = (2.0f * (float(x) + jx) / float(job.width) - 1.0f) * job.tan_half * job.aspect;
+    const float sy = (1.0f - 2.0f * (float(y) + jy) / float(job.height)) * job.tan_half;
+    const V3 f = v3(job.fwd[0], job.fwd[1], job.fwd[2]);
+    const V3 r = v3(job.right[0], job.right[1], job.right[2]);
+    const V3 u = v3(job.up[0], job.up[1], job.up[2]);
+    const V3 d = normalize(f + r * sx + u * sy);
+    ray.org[0] = job.eye[0]; ray.org[1] = job.eye[1]; ray.org[2] = job.eye[2];
+    ray.tmin = 0.0f;
+    ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
+    ray.tmax = 3.0e38f;
+}
+
 // Starts window path `gid` (global path g0 + gid) in `slot`: the slot's
 // state, and the camera ray.  False for tile padding (no ray; radiance 0).
 __device__ __forceinline__ bool start_path(const WfArgs& a, uint64_t g0, uint32_t gid, uint32_t slot, Ray& ray)
@@ -257,18 +281,7 @@ __device__ __forceinline__ bool start_path(const WfArgs& a, uint64_t g0, uint32_
                 }
             }
         }
-        Rng rng = rng_for(job.seed, q.sample, q.pixel, 0u);
-        const float jx = rng_float(rng), jy = rng_float(rng);
-        const float sx = (2.0f * (float(q.x) + jx) / float(job.width) - 1.0f) * job.tan_half * job.aspect;
-        const float sy = (1.0f - 2.0f * (float(q.y) + jy) / float(job.height)) * job.tan_half;
-        const V3 f = v3(job.fwd[0], job.fwd[1], job.fwd[2]);
-        const V3 r = v3(job.right[0], job.right[1], job.right[2]);
-        const V3 u = v3(job.up[0], job.up[1], job.up[2]);
-        const V3 d = normalize(f + r * sx + u * sy);
-        ray.org[0] = job.eye[0]; ray.org[1] = job.eye[1]; ray.org[2] = job.eye[2];
-        ray.tmin = 0.0f;
-        ray.dir[0] = d.x; ray.dir[1] = d.y; ray.dir[2] = d.z;
-        ray.tmax = 3.0e38f;
+        camera_ray(job, q.x, q.y, q.sample, q.pixel, ray);
         live = true;
     }
     a.b.p_site[slot] = make_uint4(q.job, q.pixel, q.sample, gid);
@@ -714,6 +727,28 @@ __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint64_t g0, ui
     }
 }
 
+// Every pixel's camera ray for one sample, as the renderer starts its paths.
+__global__ __launch_bounds__(256) void k_camera_rays(JobDev job, uint32_t sample, Ray* out)
+{
+    const uint32_t n = job.width * job.height;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        Ray ray;
+        camera_ray(job, i % job.width, i / job.width, sample, i, ray);
+        out[i] = ray;
+    }
+}
+
+// The first `ndraws` outputs of the generator of (seed, sample, pixel, bounce)
+// for pixels pixel0 .. pixel0 + npix - 1.
+__global__ __launch_bounds__(256) void k_rng_draws(uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t npix,
+                                                   uint32_t bounce, uint32_t ndraws, uint32_t* out)
+{
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        Rng r = rng_for(seed, sample, pixel0 + p, bounce);
+        for (uint32_t k = 0; k < ndraws; ++k) out[size_t(p) * ndraws + k] = rng_next(r);
+    }
+}
+
 }  // namespace
 
 void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s)
@@ -805,6 +840,20 @@ void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t grid, hipStrea
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)
 {
     hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, s, a, g0, n);
+}
+
+void launch_camera_rays(const JobDev& job, uint32_t sample, Ray* out, hipStream_t s)
+{
+    const uint64_t n = uint64_t(job.width) * job.height;
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(4096, (n + 255) / 256)));
+    hipLaunchKernelGGL(k_camera_rays, dim3(grid), dim3(256), 0, s, job, sample, out);
+}
+
+void launch_rng_draws(uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t npix, uint32_t bounce, uint32_t ndraws,
+                      uint32_t* out, hipStream_t s)
+{
+    const uint32_t grid = std::max(1u, std::min(4096u, (npix + 255u) / 256u));
+    hipLaunchKernelGGL(k_rng_draws, dim3(grid), dim3(256), 0, s, seed, sample, pixel0, npix, bounce, ndraws, out);
 }
 
 }  // namespace fnd

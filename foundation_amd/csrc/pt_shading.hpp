@@ -104,12 +104,47 @@ __device__ __forceinline__ Onb onb_from(V3 n)
 __device__ __forceinline__ V3 to_local(const Onb& o, V3 v) { return v3(dot(v, o.t), dot(v, o.b), dot(v, o.n)); }
 __device__ __forceinline__ V3 to_world(const Onb& o, V3 v) { return o.t * v.x + o.b * v.y + o.n * v.z; }
 
+// sin and cos of 2*pi*u for u in [0, 1), u a multiple of 2^-24 (the output
+// of rng_float).  Written out so that any host evaluating the same fmaf
+// sequence gets the same bits (a library sincosf differs between the device
+// and a host libm by an ulp here and there):
+//  * exact reduction to the nearest quarter turn: q = round(4u), r = u - q/4
+//    is exact (both are multiples of 2^-24) with |r| <= 1/8;
+//  * x = 2*pi*r in [-pi/4, pi/4], Taylor polynomials in Horner form through
+//    x^11 (sin) and x^12 (cos); the first omitted terms are below 2.2e-9 and
+//    1.1e-10 relative, so the results are within a few ulp of the true values;
+//  * the quadrant q mod 4 rotates (sin x, cos x) by q quarter turns.
+__host__ __device__ __forceinline__ void sincos_2pi(float u, float& s, float& c)
+{
+    const float qf = floorf(fmaf(u, 4.0f, 0.5f));
+    const float r = fmaf(qf, -0.25f, u);
+    const uint32_t q = uint32_t(qf) & 3u;
+    const float x = r * 6.28318530717958647692f;
+    const float x2 = x * x;
+    float ps = fmaf(x2, 2.7557319223985893e-06f, -1.9841269841269841e-04f);  // 1/9!, -1/7!
+    ps = fmaf(x2, ps, 8.3333333333333333e-03f);                               // 1/5!
+    ps = fmaf(x2, ps, -1.6666666666666667e-01f);                              // -1/3!
+    const float sx = fmaf(x * x2, ps, x);
+    float pc = fmaf(x2, 2.0876756987868099e-09f, -2.7557319223985891e-07f);  // 1/12!, -1/10!
+    pc = fmaf(x2, pc, 2.4801587301587302e-05f);                               // 1/8!
+    pc = fmaf(x2, pc, -1.3888888888888889e-03f);                              // -1/6!
+    pc = fmaf(x2, pc, 4.1666666666666667e-02f);                               // 1/4!
+    pc = fmaf(x2, pc, -0.5f);                                                 // -1/2!
+    const float cx = fmaf(x2, pc, 1.0f);
+    s = (q & 1u) ? cx : sx;
+    c = (q & 1u) ? -sx : cx;
+    if (q & 2u) {
+        s = -s;
+        c = -c;
+    }
+}
+
 // Malley: uniform disk point (polar), lifted to the hemisphere; pdf = cos / pi.
 __device__ __forceinline__ V3 sample_cosine(float u1, float u2)
 {
     const float r = sqrtf(u1);
     float s, c;
-    sincosf(2.0f * kPi * u2, &s, &c);
+    sincos_2pi(u2, s, c);
     return v3(r * c, r * s, sqrtf(fmaxf(0.0f, 1.0f - u1)));
 }
 
@@ -145,7 +180,7 @@ __device__ __forceinline__ V3 ggx_sample_vndf(V3 wo, float alpha, float u1, floa
     const V3 t2 = cross(vh, t1);
     const float r = sqrtf(u1);
     float s, c;
-    sincosf(2.0f * kPi * u2, &s, &c);
+    sincos_2pi(u2, s, c);
     const float p1 = r * c;
     float p2 = r * s;
     const float sh = 0.5f * (1.0f + vh.z);

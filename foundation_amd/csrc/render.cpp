@@ -15,7 +15,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <map>
 #include <type_traits>
 #include <vector>
 
@@ -227,6 +226,30 @@ void select_training(JobDev& j, uint32_t spp, uint32_t base, uint32_t count, uin
     j.train_count = uint32_t(std::min<uint64_t>(count, std::max<uint64_t>(1, j.train_mod / kTrainShare)));
 }
 
+// Why a camera cannot form a basis, or null: non-finite values, eye == at, a
+// field of view outside (0, 180) degrees, or an up vector of no length or
+// (nearly) parallel to the view direction (the right vector would be the
+// normalisation of a zero cross product, NaN for every ray).
+const char* camera_problem(const fnd_camera& c)
+{
+    double f[3], up[3];
+    for (int i = 0; i < 3; ++i) {
+        if (!std::isfinite(c.eye[i]) || !std::isfinite(c.at[i]) || !std::isfinite(c.up[i]))
+            return "non-finite camera";
+        f[i] = double(c.at[i]) - double(c.eye[i]);
+        up[i] = c.up[i];
+    }
+    if (!(c.fov_y_deg > 0.0f && c.fov_y_deg < 180.0f)) return "field of view outside (0, 180) degrees";
+    const double lf = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    const double lu = std::sqrt(up[0] * up[0] + up[1] * up[1] + up[2] * up[2]);
+    if (!(lf > 0.0)) return "eye == at";
+    if (!(lu > 0.0)) return "up vector of zero length";
+    const double r[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2], f[0] * up[1] - f[1] * up[0]};
+    const double lr = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+    if (!(lr > 1e-6 * lf * lu)) return "up vector parallel to the view direction";
+    return nullptr;
+}
+
 void camera_basis(const fnd_camera& c, uint32_t w, uint32_t h, JobDev& j)
 {
     double f[3], up[3] = {c.up[0], c.up[1], c.up[2]};
@@ -270,84 +293,99 @@ extern "C" {
 
 fnd_status fnd_pt_framebuffer_create(fnd_pt_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags, fnd_handle* out)
 {
-    if (!ctx || !out) return FND_E_INVALID_ARG;
-    *out = FND_INVALID_HANDLE;
-    if (width == 0 || height == 0 || width > 16384 || height > 16384)
-        return set_error(ctx, FND_E_INVALID_ARG, "framebuffer size %ux%u out of range", width, height);
-    if (flags & ~FND_FB_FP64) return set_error(ctx, FND_E_INVALID_ARG, "unknown framebuffer flags");
-    auto* f = ctx_new<Framebuffer>(ctx);
-    if (!f) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
-    f->width = width;
-    f->height = height;
-    f->flags = flags;
-    const uint64_t n = uint64_t(width) * height * 3;
-    void* p = nullptr;
-    fnd_status st = dev_alloc(ctx, &p, n * sizeof(float));
-    if (st != FND_OK) {
-        ctx_delete(ctx, f);
-        return st;
-    }
-    f->accum = static_cast<float*>(p);
-    if (flags & FND_FB_FP64) {
-        st = dev_alloc(ctx, &p, n * sizeof(double));
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        *out = FND_INVALID_HANDLE;
+        if (width == 0 || height == 0 || width > 16384 || height > 16384)
+            return set_error(ctx, FND_E_INVALID_ARG, "framebuffer size %ux%u out of range", width, height);
+        if (flags & ~FND_FB_FP64) return set_error(ctx, FND_E_INVALID_ARG, "unknown framebuffer flags");
+        auto* f = ctx_new<Framebuffer>(ctx);
+        if (!f) return set_error(ctx, FND_E_OOM_HOST, "out of host memory");
+        f->width = width;
+        f->height = height;
+        f->flags = flags;
+        const uint64_t n = uint64_t(width) * height * 3;
+        void* p = nullptr;
+        fnd_status st = dev_alloc(ctx, &p, n * sizeof(float));
         if (st != FND_OK) {
-            dev_free(ctx, f->accum, n * sizeof(float));
             ctx_delete(ctx, f);
             return st;
         }
-        f->accum64 = static_cast<double*>(p);
-    }
-    const fnd_handle h = make_handle(kHandleFramebuffer, ctx->next_id++);
-    ctx->framebuffers[h] = f;
-    *out = h;
-    return fnd_pt_framebuffer_clear(ctx, h);
+        f->accum = static_cast<float*>(p);
+        if (flags & FND_FB_FP64) {
+            st = dev_alloc(ctx, &p, n * sizeof(double));
+            if (st != FND_OK) {
+                dev_free(ctx, f->accum, n * sizeof(float));
+                ctx_delete(ctx, f);
+                return st;
+            }
+            f->accum64 = static_cast<double*>(p);
+        }
+        const fnd_handle h = make_handle(kHandleFramebuffer, ctx->next_id++);
+        try {
+            ctx->framebuffers[h] = f;
+        } catch (...) {
+            dev_free(ctx, f->accum, n * sizeof(float));
+            dev_free(ctx, f->accum64, n * sizeof(double));
+            ctx_delete(ctx, f);
+            throw;
+        }
+        *out = h;
+        return fnd_pt_framebuffer_clear(ctx, h);
+    });
 }
 
 fnd_status fnd_pt_framebuffer_destroy(fnd_pt_ctx* ctx, fnd_handle h)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    Framebuffer* f = find_framebuffer(ctx, h);
-    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-    (void)hipStreamSynchronize(ctx->stream);
-    const uint64_t n = uint64_t(f->width) * f->height * 3;
-    dev_free(ctx, f->accum, n * sizeof(float));
-    dev_free(ctx, f->accum64, n * sizeof(double));
-    dev_free(ctx, f->rgba8, n / 3 * sizeof(uint32_t));
-    ctx_delete(ctx, f);
-    ctx->framebuffers.erase(h);
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        (void)hipStreamSynchronize(ctx->stream);
+        const uint64_t n = uint64_t(f->width) * f->height * 3;
+        dev_free(ctx, f->accum, n * sizeof(float));
+        dev_free(ctx, f->accum64, n * sizeof(double));
+        dev_free(ctx, f->rgba8, n / 3 * sizeof(uint32_t));
+        ctx_delete(ctx, f);
+        ctx->framebuffers.erase(h);
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_framebuffer_clear(fnd_pt_ctx* ctx, fnd_handle h)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    Framebuffer* f = find_framebuffer(ctx, h);
-    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-    const uint64_t n = uint64_t(f->width) * f->height * 3;
-    fnd_status st = check_hip(ctx, hipMemsetAsync(f->accum, 0, n * sizeof(float), ctx->stream), "clear");
-    if (st == FND_OK && f->accum64)
-        st = check_hip(ctx, hipMemsetAsync(f->accum64, 0, n * sizeof(double), ctx->stream), "clear");
-    if (st != FND_OK) return st;
-    f->samples = 0;
-    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "clear");
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        const uint64_t n = uint64_t(f->width) * f->height * 3;
+        fnd_status st = check_hip(ctx, hipMemsetAsync(f->accum, 0, n * sizeof(float), ctx->stream), "clear");
+        if (st == FND_OK && f->accum64)
+            st = check_hip(ctx, hipMemsetAsync(f->accum64, 0, n * sizeof(double), ctx->stream), "clear");
+        if (st != FND_OK) return st;
+        f->samples = 0;
+        return check_hip(ctx, hipStreamSynchronize(ctx->stream), "clear");
+    });
 }
 
 fnd_status fnd_pt_framebuffer_read(fnd_pt_ctx* ctx, fnd_handle h, float* rgb, uint32_t* out_samples)
 {
-    if (!ctx || !rgb) return FND_E_INVALID_ARG;
-    Framebuffer* f = find_framebuffer(ctx, h);
-    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-    const uint64_t n = uint64_t(f->width) * f->height * 3;
-    fnd_status st = check_hip(ctx, hipMemcpyAsync(rgb, f->accum, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
-                              "read framebuffer");
-    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
-    if (st != FND_OK) return st;
-    if (f->samples) {
-        const float inv = 1.0f / float(f->samples);
-        for (uint64_t i = 0; i < n; ++i) rgb[i] *= inv;
-    }
-    if (out_samples) *out_samples = f->samples;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !rgb) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        const uint64_t n = uint64_t(f->width) * f->height * 3;
+        fnd_status st = check_hip(ctx, hipMemcpyAsync(rgb, f->accum, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                                  "read framebuffer");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
+        if (st != FND_OK) return st;
+        if (f->samples) {
+            const float inv = 1.0f / float(f->samples);
+            for (uint64_t i = 0; i < n; ++i) rgb[i] *= inv;
+        }
+        if (out_samples) *out_samples = f->samples;
+        return FND_OK;
+    });
 }
 
 // Present path: the reference presents R8G8B8A8_UNORM at 1920x1080
@@ -358,27 +396,29 @@ fnd_status fnd_pt_framebuffer_read(fnd_pt_ctx* ctx, fnd_handle h, float* rgb, ui
 fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle h, float exposure, uint8_t* rgba,
                                          uint32_t* out_samples)
 {
-    if (!ctx || !rgba || !(exposure >= 0.0f)) return FND_E_INVALID_ARG;
-    Framebuffer* f = find_framebuffer(ctx, h);
-    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-    const uint64_t pixels = uint64_t(f->width) * f->height;
-    fnd_status st = present_table(ctx);
-    if (st != FND_OK) return st;
-    if (!f->rgba8) {
-        void* p = nullptr;
-        if ((st = dev_alloc(ctx, &p, pixels * sizeof(uint32_t))) != FND_OK) return st;
-        f->rgba8 = static_cast<uint32_t*>(p);
-    }
-    const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
-    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, f->rgba8, f->width, f->width, ctx->stream);
-    st = check_hip(ctx, hipGetLastError(), "present");
-    if (st == FND_OK)
-        st = check_hip(ctx, hipMemcpyAsync(rgba, f->rgba8, pixels * 4, hipMemcpyDeviceToHost, ctx->stream),
-                       "read framebuffer");
-    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
-    if (st != FND_OK) return st;
-    if (out_samples) *out_samples = f->samples;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !rgba || !(exposure >= 0.0f)) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        const uint64_t pixels = uint64_t(f->width) * f->height;
+        fnd_status st = present_table(ctx);
+        if (st != FND_OK) return st;
+        if (!f->rgba8) {
+            void* p = nullptr;
+            if ((st = dev_alloc(ctx, &p, pixels * sizeof(uint32_t))) != FND_OK) return st;
+            f->rgba8 = static_cast<uint32_t*>(p);
+        }
+        const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
+        launch_present(f->accum, pixels, inv, exposure, ctx->d_present, f->rgba8, f->width, f->width, ctx->stream);
+        st = check_hip(ctx, hipGetLastError(), "present");
+        if (st == FND_OK)
+            st = check_hip(ctx, hipMemcpyAsync(rgba, f->rgba8, pixels * 4, hipMemcpyDeviceToHost, ctx->stream),
+                           "read framebuffer");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
+        if (st != FND_OK) return st;
+        if (out_samples) *out_samples = f->samples;
+        return FND_OK;
+    });
 }
 
 // The same conversion written straight into device memory of the caller's
@@ -387,399 +427,468 @@ fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle h, float ex
 fnd_status fnd_pt_framebuffer_present(fnd_pt_ctx* ctx, fnd_handle h, float exposure, void* device_rgba,
                                       uint64_t row_pitch, uint32_t* out_samples)
 {
-    if (!ctx || !device_rgba || !(exposure >= 0.0f)) return FND_E_INVALID_ARG;
-    Framebuffer* f = find_framebuffer(ctx, h);
-    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-    if (reinterpret_cast<uintptr_t>(device_rgba) % 4 || row_pitch % 4 || row_pitch < 4ull * f->width ||
-        row_pitch / 4 > 0xFFFFFFFFull)
-        return set_error(ctx, FND_E_INVALID_ARG,
-                         "present target must be 4-byte aligned with a row pitch of at least %u bytes, a multiple of 4",
-                         4 * f->width);
-    fnd_status st = present_table(ctx);
-    if (st != FND_OK) return st;
-    const uint64_t pixels = uint64_t(f->width) * f->height;
-    const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
-    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, static_cast<uint32_t*>(device_rgba), f->width,
-                   uint32_t(row_pitch / 4), ctx->stream);
-    st = check_hip(ctx, hipGetLastError(), "present");
-    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "present");
-    if (st != FND_OK) return st;
-    if (out_samples) *out_samples = f->samples;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !device_rgba || !(exposure >= 0.0f)) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        if (reinterpret_cast<uintptr_t>(device_rgba) % 4 || row_pitch % 4 || row_pitch < 4ull * f->width ||
+            row_pitch / 4 > 0xFFFFFFFFull)
+            return set_error(ctx, FND_E_INVALID_ARG,
+                             "present target must be 4-byte aligned with a row pitch of at least %u bytes, a multiple of 4",
+                             4 * f->width);
+        fnd_status st = present_table(ctx);
+        if (st != FND_OK) return st;
+        const uint64_t pixels = uint64_t(f->width) * f->height;
+        const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
+        launch_present(f->accum, pixels, inv, exposure, ctx->d_present, static_cast<uint32_t*>(device_rgba), f->width,
+                       uint32_t(row_pitch / 4), ctx->stream);
+        st = check_hip(ctx, hipGetLastError(), "present");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "present");
+        if (st != FND_OK) return st;
+        if (out_samples) *out_samples = f->samples;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle h, double* rgb, uint32_t* out_samples)
 {
-    if (!ctx || !rgb) return FND_E_INVALID_ARG;
-    Framebuffer* f = find_framebuffer(ctx, h);
-    if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-    if (!f->accum64) return set_error(ctx, FND_E_INVALID_ARG, "framebuffer was created without FND_FB_FP64");
-    const uint64_t n = uint64_t(f->width) * f->height * 3;
-    fnd_status st = check_hip(ctx, hipMemcpyAsync(rgb, f->accum64, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
-                              "read framebuffer");
-    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
-    if (st != FND_OK) return st;
-    if (f->samples) {
-        const double inv = 1.0 / double(f->samples);
-        for (uint64_t i = 0; i < n; ++i) rgb[i] *= inv;
-    }
-    if (out_samples) *out_samples = f->samples;
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !rgb) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        if (!f->accum64) return set_error(ctx, FND_E_INVALID_ARG, "framebuffer was created without FND_FB_FP64");
+        const uint64_t n = uint64_t(f->width) * f->height * 3;
+        fnd_status st = check_hip(ctx, hipMemcpyAsync(rgb, f->accum64, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
+                                  "read framebuffer");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read framebuffer");
+        if (st != FND_OK) return st;
+        if (f->samples) {
+            const double inv = 1.0 / double(f->samples);
+            for (uint64_t i = 0; i < n; ++i) rgb[i] *= inv;
+        }
+        if (out_samples) *out_samples = f->samples;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs)
 {
-    if (!ctx || (!jobs && num_jobs)) return FND_E_INVALID_ARG;
-    if (num_jobs == 0) return FND_OK;
-    // a call that failed after queueing cache queries on the side stream may
-    // have left them running; they read the queues this call is about to reuse
-    (void)hipStreamSynchronize(ctx->side);
-    const uint32_t T = std::max(1u, ctx->tuning.tile);
-    std::map<Scene*, uint32_t> slot_of;
-    std::vector<Scene*> slots;
-    std::vector<JobDev> hj(num_jobs);
-    std::vector<Framebuffer*> fbs(num_jobs);
-    uint64_t total = 0;
-    uint32_t max_depth = 0, max_stack = 0;
-    bool any_use = false, any_train = false;
-    Scene* cache_scene = nullptr;
-    for (uint32_t i = 0; i < num_jobs; ++i) {
-        const fnd_render_job& in = jobs[i];
-        Scene* s = find_scene(ctx, in.scene);
-        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown scene handle", i);
-        Framebuffer* fb = find_framebuffer(ctx, in.framebuffer);
-        if (!fb) return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown framebuffer handle", i);
-        for (uint32_t k = 0; k < i; ++k)
-            if (fbs[k] == fb)
-                return set_error(ctx, FND_E_INVALID_ARG, "jobs %u and %u accumulate into one framebuffer", k, i);
-        if (in.spp == 0 || in.max_depth > 64)
-            return set_error(ctx, FND_E_INVALID_ARG, "job %u: spp must be >= 1 and max_depth <= 64", i);
-        if (in.flags & ~(FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE))
-            return set_error(ctx, FND_E_INVALID_ARG, "job %u: unknown flags 0x%x", i, in.flags);
-        if (in.flags & (FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE)) {
-            if (!cache_get(ctx))
-                return set_error(ctx, FND_E_INVALID_ARG, "job %u: radiance cache flags need fnd_pt_cache_configure", i);
-            if (!cache_scene) cache_scene = s;
-            any_use |= (in.flags & FND_JOB_USE_CACHE) != 0;
-            any_train |= (in.flags & FND_JOB_TRAIN_CACHE) != 0;
-        }
-        const float dx = in.camera.at[0] - in.camera.eye[0], dy = in.camera.at[1] - in.camera.eye[1],
-                    dz = in.camera.at[2] - in.camera.eye[2];
-        if (!(dx * dx + dy * dy + dz * dz > 0.0f) || !(in.camera.fov_y_deg > 0.0f && in.camera.fov_y_deg < 180.0f))
-            return set_error(ctx, FND_E_INVALID_ARG, "job %u: degenerate camera", i);
-        if (uint64_t(in.sample_offset) + in.spp > 0xFFFFFFFFull)
-            return set_error(ctx, FND_E_INVALID_ARG, "job %u: sample index overflows 32 bits", i);
-        auto it = slot_of.find(s);
-        if (it == slot_of.end()) {
-            it = slot_of.emplace(s, uint32_t(slots.size())).first;
-            slots.push_back(s);
-        }
-        JobDev& j = hj[i];
-        std::memset(&j, 0, sizeof(j));
-        j.scene_slot = it->second;
-        j.width = fb->width;
-        j.height = fb->height;
-        j.tile = T;
-        j.tiles_x = (fb->width + T - 1) / T;
-        const uint64_t padded = uint64_t(j.tiles_x) * ((fb->height + T - 1) / T) * T * T;
-        if (padded >= (1ull << 32)) return set_error(ctx, FND_E_INVALID_ARG, "job %u: image too large", i);
-        j.padded = uint32_t(padded);
-        j.max_depth = in.max_depth;
-        j.rr_depth = in.rr_depth;
-        j.sample_offset = in.sample_offset;
-        j.flags = in.flags & FND_JOB_USE_CACHE;
-        j.seed = in.seed;
-        j.path_base = total;
-        total += padded * in.spp;
-        j.path_end = total;
-        camera_basis(in.camera, fb->width, fb->height, j);
-        j.accum = fb->accum;
-        j.accum64 = fb->accum64;
-        fbs[i] = fb;
-        max_depth = std::max(max_depth, in.max_depth);
-        max_stack = std::max(max_stack, s->info.stack_entries);
-    }
-    CacheState* cs = cache_get(ctx);
-    if (cache_scene && !cs->bounds_set) cache_set_bounds(cs, cache_scene->info.bounds_lo, cache_scene->info.bounds_hi, true);
-    if (any_train) {
-        // the frame's training paths: desc.train_paths split over the training jobs
-        uint32_t ntrain = 0;
-        for (uint32_t i = 0; i < num_jobs; ++i) ntrain += (jobs[i].flags & FND_JOB_TRAIN_CACHE) ? 1u : 0u;
-        const uint32_t per = std::max(1u, cs->desc.train_paths / ntrain);
-        uint32_t base = 0;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || (!jobs && num_jobs)) return FND_E_INVALID_ARG;
+        if (num_jobs == 0) return FND_OK;
+        // a call that failed after queueing cache queries on the side stream may
+        // have left them running; they read the queues this call is about to reuse
+        (void)hipStreamSynchronize(ctx->side);
+        const uint32_t T = std::max(1u, ctx->tuning.tile);
+        // the call's temporaries come from the caller's allocator, like the context's own objects
+        const CtxAlloc<char> ha = ctx->host_alloc_ref();
+        CtxMap<Scene*, uint32_t> slot_of(ha);
+        CtxVector<Scene*> slots(ha);
+        CtxVector<JobDev> hj(num_jobs, JobDev{}, ha);
+        CtxVector<Framebuffer*> fbs(num_jobs, nullptr, ha);
+        uint64_t total = 0;
+        uint32_t max_depth = 0, max_stack = 0;
+        bool any_use = false, any_train = false;
+        Scene* cache_scene = nullptr;
         for (uint32_t i = 0; i < num_jobs; ++i) {
-            if (!(jobs[i].flags & FND_JOB_TRAIN_CACHE) || base >= kCacheTrainMaxPaths) continue;
-            select_training(hj[i], jobs[i].spp, base, std::min(per, kCacheTrainMaxPaths - base), cs->frames);
-            base += hj[i].train_count;
+            const fnd_render_job& in = jobs[i];
+            Scene* s = find_scene(ctx, in.scene);
+            if (!s) return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown scene handle", i);
+            Framebuffer* fb = find_framebuffer(ctx, in.framebuffer);
+            if (!fb) return set_error(ctx, FND_E_NOT_FOUND, "job %u: unknown framebuffer handle", i);
+            for (uint32_t k = 0; k < i; ++k)
+                if (fbs[k] == fb)
+                    return set_error(ctx, FND_E_INVALID_ARG, "jobs %u and %u accumulate into one framebuffer", k, i);
+            if (in.spp == 0 || in.max_depth > 64)
+                return set_error(ctx, FND_E_INVALID_ARG, "job %u: spp must be >= 1 and max_depth <= 64", i);
+            if (in.flags & ~(FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE))
+                return set_error(ctx, FND_E_INVALID_ARG, "job %u: unknown flags 0x%x", i, in.flags);
+            if (in.flags & (FND_JOB_USE_CACHE | FND_JOB_TRAIN_CACHE)) {
+                if (!cache_get(ctx))
+                    return set_error(ctx, FND_E_INVALID_ARG, "job %u: radiance cache flags need fnd_pt_cache_configure", i);
+                if (!cache_scene) cache_scene = s;
+                any_use |= (in.flags & FND_JOB_USE_CACHE) != 0;
+                any_train |= (in.flags & FND_JOB_TRAIN_CACHE) != 0;
+            }
+            if (const char* why = camera_problem(in.camera))
+                return set_error(ctx, FND_E_INVALID_ARG, "job %u: degenerate camera (%s)", i, why);
+            if (uint64_t(in.sample_offset) + in.spp > 0xFFFFFFFFull)
+                return set_error(ctx, FND_E_INVALID_ARG, "job %u: sample index overflows 32 bits", i);
+            auto it = slot_of.find(s);
+            if (it == slot_of.end()) {
+                it = slot_of.emplace(s, uint32_t(slots.size())).first;
+                slots.push_back(s);
+            }
+            JobDev& j = hj[i];
+            std::memset(&j, 0, sizeof(j));
+            j.scene_slot = it->second;
+            j.width = fb->width;
+            j.height = fb->height;
+            j.tile = T;
+            j.tiles_x = (fb->width + T - 1) / T;
+            const uint64_t padded = uint64_t(j.tiles_x) * ((fb->height + T - 1) / T) * T * T;
+            if (padded >= (1ull << 32)) return set_error(ctx, FND_E_INVALID_ARG, "job %u: image too large", i);
+            j.padded = uint32_t(padded);
+            j.max_depth = in.max_depth;
+            j.rr_depth = in.rr_depth;
+            j.sample_offset = in.sample_offset;
+            j.flags = in.flags & FND_JOB_USE_CACHE;
+            j.seed = in.seed;
+            j.path_base = total;
+            total += padded * in.spp;
+            j.path_end = total;
+            camera_basis(in.camera, fb->width, fb->height, j);
+            j.accum = fb->accum;
+            j.accum64 = fb->accum64;
+            fbs[i] = fb;
+            max_depth = std::max(max_depth, in.max_depth);
+            max_stack = std::max(max_stack, s->info.stack_entries);
         }
-    }
-    const uint32_t want = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
-    uint32_t cap = 0;
-    fnd_status st = begin_frame(ctx, want, uint32_t(std::min<uint64_t>(1024u, total)), &cap);
-    if (st != FND_OK) return st;
-    st = ensure_tables(ctx, num_jobs, uint32_t(slots.size()));
-    if (st != FND_OK) return st;
-    auto* rs = static_cast<RenderState*>(ctx->render_state);
-    std::vector<ShadeScene> hs(slots.size());
-    std::vector<SceneDev> hd(slots.size());
-    for (size_t k = 0; k < slots.size(); ++k) {
-        const Scene* s = slots[k];
-        hd[k] = SceneDev{s->nodes, s->tris, s->insts, s->root, s->two_level ? 1u : 0u};
-        ShadeScene& z = hs[k];
-        z.tris = s->tris;
-        z.insts = s->insts;
-        z.materials = s->materials;
-        z.lights = s->lights;
-        z.num_lights = s->num_lights;
-        z.two_level = s->two_level ? 1u : 0u;
-        std::memcpy(z.background, s->background, 12);
-        z.pad = 0;
-    }
-    st = check_hip(ctx, hipMemcpyAsync(rs->d_shade, hs.data(), hs.size() * sizeof(ShadeScene), hipMemcpyHostToDevice,
-                                       ctx->stream), "upload scenes");
-    if (st == FND_OK)
-        st = check_hip(ctx, hipMemcpyAsync(rs->d_scenes, hd.data(), hd.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
-                                           ctx->stream), "upload scenes");
-    if (st != FND_OK) return st;
-
-    TraceLaunch tl{};
-    tl.two_level = true;
-    tl.multi = true;
-    tl.block = ctx->tuning.block_size;
-    tl.stack_lds = ctx->tuning.stack_lds;
-    tl.mode = kModeMixedInternal;  // the mode every launch below uses
-    tl.grid = trace_grid(ctx, tl);
-    st = ensure_spill(ctx, max_stack, tl.stack_lds, tl.grid * tl.block);
-    if (st != FND_OK) return st;
-
-    WfArgs wa{};
-    wa.jobs = rs->d_jobs;
-    wa.scenes = rs->d_shade;
-    wa.b = rs->b;
-    if (cs) {
-        wa.b.rec = cs->rec;
-        wa.b.rec_count = cs->rec_count;
-        wa.b.t_list = cs->t_list;
-        wa.b.rec_max = std::min(cs->desc.train_depth, kCacheMaxRecords);
-        wa.b.glossy_min = cs->desc.glossy_min_roughness;
-        wa.b.glossy_any_bounce = cs->desc.glossy_any_bounce;
-        wa.b.t_query = cs->s_query;
-        wa.b.t_target = cs->s_target;
-        wa.b.t_cap = cs->sample_cap;
-    }
-    const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
-
-    // Drain polling resources, made once per render state.
-    if (!rs->poll_host) {
-        void* h = nullptr;
-        st = check_hip(ctx, hipHostMalloc(&h, kPollRing * kCntCount * sizeof(uint32_t)), "hipHostMalloc");
+        CacheState* cs = cache_get(ctx);
+        if (cache_scene && !cs->bounds_set) cache_set_bounds(cs, cache_scene->info.bounds_lo, cache_scene->info.bounds_hi, true);
+        if (any_train) {
+            // the frame's training paths: desc.train_paths split over the training jobs
+            uint32_t ntrain = 0;
+            for (uint32_t i = 0; i < num_jobs; ++i) ntrain += (jobs[i].flags & FND_JOB_TRAIN_CACHE) ? 1u : 0u;
+            const uint32_t per = std::max(1u, cs->desc.train_paths / ntrain);
+            uint32_t base = 0;
+            for (uint32_t i = 0; i < num_jobs; ++i) {
+                if (!(jobs[i].flags & FND_JOB_TRAIN_CACHE) || base >= kCacheTrainMaxPaths) continue;
+                select_training(hj[i], jobs[i].spp, base, std::min(per, kCacheTrainMaxPaths - base), cs->frames);
+                base += hj[i].train_count;
+            }
+        }
+        const uint32_t want = uint32_t(std::min<uint64_t>(std::max(1024u, ctx->tuning.queue_capacity), total));
+        uint32_t cap = 0;
+        fnd_status st = begin_frame(ctx, want, uint32_t(std::min<uint64_t>(1024u, total)), &cap);
         if (st != FND_OK) return st;
-        rs->poll_host = static_cast<uint32_t*>(h);
-        for (auto& e : rs->poll_ev) {
-            st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            if (st != FND_OK) return st;
+        st = ensure_tables(ctx, num_jobs, uint32_t(slots.size()));
+        if (st != FND_OK) return st;
+        auto* rs = static_cast<RenderState*>(ctx->render_state);
+        CtxVector<ShadeScene> hs(slots.size(), ShadeScene{}, ha);
+        CtxVector<SceneDev> hd(slots.size(), SceneDev{}, ha);
+        for (size_t k = 0; k < slots.size(); ++k) {
+            const Scene* s = slots[k];
+            hd[k] = SceneDev{s->nodes, s->tris, s->insts, s->root, s->two_level ? 1u : 0u};
+            ShadeScene& z = hs[k];
+            z.tris = s->tris;
+            z.insts = s->insts;
+            z.materials = s->materials;
+            z.lights = s->lights;
+            z.num_lights = s->num_lights;
+            z.two_level = s->two_level ? 1u : 0u;
+            std::memcpy(z.background, s->background, 12);
+            z.pad = 0;
         }
-        for (auto& e : rs->side_ev) {
-            st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            if (st != FND_OK) return st;
-        }
-    }
+        st = check_hip(ctx, hipMemcpyAsync(rs->d_shade, hs.data(), hs.size() * sizeof(ShadeScene), hipMemcpyHostToDevice,
+                                           ctx->stream), "upload scenes");
+        if (st == FND_OK)
+            st = check_hip(ctx, hipMemcpyAsync(rs->d_scenes, hd.data(), hd.size() * sizeof(SceneDev), hipMemcpyHostToDevice,
+                                               ctx->stream), "upload scenes");
+        if (st != FND_OK) return st;
 
-    // One pass of the wavefront over a job table already on the device, window
-    // by window.  A window's first paths fill the slots; then every iteration
-    // shades the traced extension rays, gives the slots freed in the previous
-    // iteration the window's next paths (their camera rays join the same
-    // queue), traces extension and shadow rays in one launch, adds the shadow
-    // results and retires the paths that ended.  Paths ending into the
-    // radiance cache are queried once the window has drained, so a window
-    // that uses the cache holds at most one path per slot.
-    auto run = [&](uint32_t njobs, uint64_t paths, uint32_t slots, uint32_t depth, bool apply_cache,
-                   bool collect) -> fnd_status {
-        wa.num_jobs = njobs;
-        const uint64_t wmax = (apply_cache || collect) ? uint64_t(slots) : uint64_t(slots) * kWindowSlots;
-        TraceArgs ta{};
-        ta.scenes = rs->d_scenes;
-        ta.hits = rs->b.ext_hits;
-        ta.counter = &rs->b.counters[kCntWorkExt];
-        ta.spill = ctx->d_spill;
-        ta.spill_stride = tl.grid * tl.block;
-        ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
-        ta.rays2 = rs->b.sh_rays;
-        ta.hits2 = rs->b.sh_hits;
-        ta.ray_scene2 = rs->b.sh_scene;
-        ta.nrays2_dev = &rs->b.counters[kCntShadow];
-        ta.ray_total = rs->b.rays;
-        TraceLaunch l = tl;
-        l.mode = kModeMixedInternal;
-        // One traversal launch per iteration: the extension rays (closest hit)
-        // and the shadow rays of the vertices just shaded (any hit) share it,
-        // so their tails overlap.
-        auto trace = [&](uint32_t q, const char* what) -> fnd_status {
-            ta.rays = rs->b.ext_rays[q];
-            ta.ray_scene = rs->b.ext_scene[q];
-            ta.nrays_dev = &rs->b.counters[q];
-            return check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
-        };
-        for (uint64_t g0 = 0; g0 < paths; g0 += wmax) {
-            const uint32_t W = uint32_t(std::min<uint64_t>(wmax, paths - g0));
-            const uint32_t n0 = std::min(W, slots);
-            const uint32_t wgrid = std::min(stage_grid, (W + 255u) / 256u);
-            const uint32_t sgrid = std::min(stage_grid, (n0 + 255u) / 256u);  // a window uses n0 slots
-            // A window that fits in the slots has no regeneration: slot i holds
-            // window path i to its end, so its radiance stays in place and the
-            // bounce count bounds the iterations (no polling).
-            const bool regen = W > slots;
-            wa.b.window = W;
-            wa.b.w_L = regen ? rs->b.w_L : rs->b.p_L;
-            wa.b.retire = regen ? 1u : 0u;
-            uint32_t cur = kCntExtA;
-            launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntFree0), ctx->stream);
-            launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter
-            launch_generate(wa, g0, n0, sgrid, ctx->stream);
-            fnd_status e = trace(cur, "primary trace");
-            if (e != FND_OK) return e;
-            const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (W + 127u) / 128u));
-            bool early = false;  // queries answered on the side stream
-            if (!regen) {
-                // After the shade of bounce 1 nearly every path that ends in the
-                // radiance cache has queried it.  Those queries are answered on
-                // the side stream while the main stream traces the last bounces,
-                // launches that hold few rays and leave most CUs idle; the rest
-                // are answered after the loop, on the main stream.
-                const uint32_t ksplit = std::min(1u, depth);
-                for (uint32_t k = 0; k <= depth; ++k) {
-                    const uint32_t nxt = cur ^ 1u, fw = (k & 1u) ^ 1u;  // (the free lists go unused)
-                    launch_prep(wa, nxt, fw, ctx->stream);
-                    launch_shade(wa, cur, fw, sgrid, ctx->stream);
-                    if (apply_cache && k == ksplit && k < depth) {
-                        launch_snapshot(wa, kCntQuery, kCntQuerySplit, ctx->stream);
-                        e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
-                        if (e == FND_OK) e = check_hip(ctx, hipStreamWaitEvent(ctx->side, rs->side_ev[0], 0), "side stream");
-                        if (e != FND_OK) return e;
-                        launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuerySplit], rs->b.q_pred,
-                                             nullptr, 0, cg, ctx->side);
-                        e = check_hip(ctx, hipEventRecord(rs->side_ev[1], ctx->side), "side stream");
-                        if (e != FND_OK) return e;
-                        early = true;
+        TraceLaunch tl{};
+        tl.two_level = true;
+        tl.multi = true;
+        tl.block = ctx->tuning.block_size;
+        tl.stack_lds = ctx->tuning.stack_lds;
+        tl.mode = kModeMixedInternal;  // the mode every launch below uses
+        tl.grid = trace_grid(ctx, tl);
+        st = ensure_spill(ctx, max_stack, tl.stack_lds, tl.grid * tl.block);
+        if (st != FND_OK) return st;
+
+        WfArgs wa{};
+        wa.jobs = rs->d_jobs;
+        wa.scenes = rs->d_shade;
+        wa.b = rs->b;
+        if (cs) {
+            wa.b.rec = cs->rec;
+            wa.b.rec_count = cs->rec_count;
+            wa.b.t_list = cs->t_list;
+            wa.b.rec_max = std::min(cs->desc.train_depth, kCacheMaxRecords);
+            wa.b.glossy_min = cs->desc.glossy_min_roughness;
+            wa.b.glossy_any_bounce = cs->desc.glossy_any_bounce;
+            wa.b.t_query = cs->s_query;
+            wa.b.t_target = cs->s_target;
+            wa.b.t_cap = cs->sample_cap;
+        }
+        const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
+
+        // Drain polling resources, made once per render state.
+        if (!rs->poll_host) {
+            void* h = nullptr;
+            st = check_hip(ctx, hipHostMalloc(&h, kPollRing * kCntCount * sizeof(uint32_t)), "hipHostMalloc");
+            if (st != FND_OK) return st;
+            rs->poll_host = static_cast<uint32_t*>(h);
+            for (auto& e : rs->poll_ev) {
+                st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                if (st != FND_OK) return st;
+            }
+            for (auto& e : rs->side_ev) {
+                st = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                if (st != FND_OK) return st;
+            }
+        }
+
+        // One pass of the wavefront over a job table already on the device, window
+        // by window.  A window's first paths fill the slots; then every iteration
+        // shades the traced extension rays, gives the slots freed in the previous
+        // iteration the window's next paths (their camera rays join the same
+        // queue), traces extension and shadow rays in one launch, adds the shadow
+        // results and retires the paths that ended.  Paths ending into the
+        // radiance cache are queried once the window has drained, so a window
+        // that uses the cache holds at most one path per slot.
+        auto run = [&](uint32_t njobs, uint64_t paths, uint32_t slots, uint32_t depth, bool apply_cache,
+                       bool collect) -> fnd_status {
+            wa.num_jobs = njobs;
+            const uint64_t wmax = (apply_cache || collect) ? uint64_t(slots) : uint64_t(slots) * kWindowSlots;
+            TraceArgs ta{};
+            ta.scenes = rs->d_scenes;
+            ta.hits = rs->b.ext_hits;
+            ta.counter = &rs->b.counters[kCntWorkExt];
+            ta.spill = ctx->d_spill;
+            ta.spill_stride = tl.grid * tl.block;
+            ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
+            ta.rays2 = rs->b.sh_rays;
+            ta.hits2 = rs->b.sh_hits;
+            ta.ray_scene2 = rs->b.sh_scene;
+            ta.nrays2_dev = &rs->b.counters[kCntShadow];
+            ta.ray_total = rs->b.rays;
+            TraceLaunch l = tl;
+            l.mode = kModeMixedInternal;
+            // One traversal launch per iteration: the extension rays (closest hit)
+            // and the shadow rays of the vertices just shaded (any hit) share it,
+            // so their tails overlap.
+            auto trace = [&](uint32_t q, const char* what) -> fnd_status {
+                ta.rays = rs->b.ext_rays[q];
+                ta.ray_scene = rs->b.ext_scene[q];
+                ta.nrays_dev = &rs->b.counters[q];
+                return check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
+            };
+            for (uint64_t g0 = 0; g0 < paths; g0 += wmax) {
+                const uint32_t W = uint32_t(std::min<uint64_t>(wmax, paths - g0));
+                const uint32_t n0 = std::min(W, slots);
+                const uint32_t wgrid = std::min(stage_grid, (W + 255u) / 256u);
+                const uint32_t sgrid = std::min(stage_grid, (n0 + 255u) / 256u);  // a window uses n0 slots
+                // A window that fits in the slots has no regeneration: slot i holds
+                // window path i to its end, so its radiance stays in place and the
+                // bounce count bounds the iterations (no polling).
+                const bool regen = W > slots;
+                wa.b.window = W;
+                wa.b.w_L = regen ? rs->b.w_L : rs->b.p_L;
+                wa.b.retire = regen ? 1u : 0u;
+                uint32_t cur = kCntExtA;
+                launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntFree0), ctx->stream);
+                launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter
+                launch_generate(wa, g0, n0, sgrid, ctx->stream);
+                fnd_status e = trace(cur, "primary trace");
+                if (e != FND_OK) return e;
+                const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (W + 127u) / 128u));
+                bool early = false;  // queries answered on the side stream
+                if (!regen) {
+                    // After the shade of bounce 1 nearly every path that ends in the
+                    // radiance cache has queried it.  Those queries are answered on
+                    // the side stream while the main stream traces the last bounces,
+                    // launches that hold few rays and leave most CUs idle; the rest
+                    // are answered after the loop, on the main stream.
+                    const uint32_t ksplit = std::min(1u, depth);
+                    for (uint32_t k = 0; k <= depth; ++k) {
+                        const uint32_t nxt = cur ^ 1u, fw = (k & 1u) ^ 1u;  // (the free lists go unused)
+                        launch_prep(wa, nxt, fw, ctx->stream);
+                        launch_shade(wa, cur, fw, sgrid, ctx->stream);
+                        if (apply_cache && k == ksplit && k < depth) {
+                            launch_snapshot(wa, kCntQuery, kCntQuerySplit, ctx->stream);
+                            e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
+                            if (e == FND_OK) e = check_hip(ctx, hipStreamWaitEvent(ctx->side, rs->side_ev[0], 0), "side stream");
+                            if (e != FND_OK) return e;
+                            launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuerySplit], rs->b.q_pred,
+                                                 nullptr, 0, cg, ctx->side);
+                            e = check_hip(ctx, hipEventRecord(rs->side_ev[1], ctx->side), "side stream");
+                            if (e != FND_OK) return e;
+                            early = true;
+                        }
+                        if (k < depth) {  // the last shade appends no rays
+                            e = trace(nxt, "extension + shadow trace");
+                            if (e != FND_OK) return e;
+                            launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
+                        }
+                        cur = nxt;
                     }
-                    if (k < depth) {  // the last shade appends no rays
+                } else {
+                    // every path is shaded at most depth + 1 times, and while paths are
+                    // left no slot stays idle, so the window drains within this many
+                    // iterations (plus the polling lag)
+                    const uint64_t max_iters = (uint64_t(W) / slots + 2u) * (uint64_t(depth) + 3u) + kPollLag + 2u;
+                    bool drained = false;
+                    for (uint64_t k = 0; !drained; ++k) {
+                        if (k >= max_iters)
+                            return set_error(ctx, FND_E_DEVICE,
+                                             "wavefront window of %u paths did not drain in %llu iterations", W,
+                                             (unsigned long long)max_iters);
+                        const uint32_t nxt = cur ^ 1u;
+                        const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
+                        launch_prep(wa, nxt, fw, ctx->stream);
+                        launch_shade(wa, cur, fw, sgrid, ctx->stream);
+                        launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
                         e = trace(nxt, "extension + shadow trace");
                         if (e != FND_OK) return e;
                         launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
-                    }
-                    cur = nxt;
-                }
-            } else {
-                // every path is shaded at most depth + 1 times, and while paths are
-                // left no slot stays idle, so the window drains within this many
-                // iterations (plus the polling lag)
-                const uint64_t max_iters = (uint64_t(W) / slots + 2u) * (uint64_t(depth) + 3u) + kPollLag + 2u;
-                bool drained = false;
-                for (uint64_t k = 0; !drained; ++k) {
-                    if (k >= max_iters)
-                        return set_error(ctx, FND_E_DEVICE,
-                                         "wavefront window of %u paths did not drain in %llu iterations", W,
-                                         (unsigned long long)max_iters);
-                    const uint32_t nxt = cur ^ 1u;
-                    const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
-                    launch_prep(wa, nxt, fw, ctx->stream);
-                    launch_shade(wa, cur, fw, sgrid, ctx->stream);
-                    launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
-                    e = trace(nxt, "extension + shadow trace");
-                    if (e != FND_OK) return e;
-                    launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
-                    const uint32_t ring = uint32_t(k % kPollRing);
-                    e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount, rs->b.counters,
-                                                      kCntCount * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
-                                  "drain poll");
-                    if (e == FND_OK) e = check_hip(ctx, hipEventRecord(rs->poll_ev[ring], ctx->stream), "drain poll");
-                    if (e != FND_OK) return e;
-                    cur = nxt;
-                    if (k >= kPollLag) {
-                        const uint32_t old = uint32_t((k - kPollLag) % kPollRing);
-                        e = check_hip(ctx, hipEventSynchronize(rs->poll_ev[old]), "drain poll");
+                        const uint32_t ring = uint32_t(k % kPollRing);
+                        e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount, rs->b.counters,
+                                                          kCntCount * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+                                      "drain poll");
+                        if (e == FND_OK) e = check_hip(ctx, hipEventRecord(rs->poll_ev[ring], ctx->stream), "drain poll");
                         if (e != FND_OK) return e;
-                        // queue `old_next` is what iteration k - kPollLag left to shade
-                        const uint32_t* c = rs->poll_host + old * kCntCount;
-                        const uint32_t old_next = uint32_t(((k - kPollLag) & 1u) ? kCntExtA : kCntExtB);
-                        drained = c[old_next] == 0u && c[kCntNext] >= W;
+                        cur = nxt;
+                        if (k >= kPollLag) {
+                            const uint32_t old = uint32_t((k - kPollLag) % kPollRing);
+                            e = check_hip(ctx, hipEventSynchronize(rs->poll_ev[old]), "drain poll");
+                            if (e != FND_OK) return e;
+                            // queue `old_next` is what iteration k - kPollLag left to shade
+                            const uint32_t* c = rs->poll_host + old * kCntCount;
+                            const uint32_t old_next = uint32_t(((k - kPollLag) & 1u) ? kCntExtA : kCntExtB);
+                            drained = c[old_next] == 0u && c[kCntNext] >= W;
+                        }
                     }
                 }
-            }
-            if (apply_cache) {
-                // the queries not answered early: [split, count), then wait for the early ones
-                launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
-                                     cg, ctx->stream, early ? &rs->b.counters[kCntQuerySplit] : nullptr);
-                if (early) {
-                    e = check_hip(ctx, hipStreamWaitEvent(ctx->stream, rs->side_ev[1], 0), "side stream");
-                    if (e != FND_OK) return e;
+                if (apply_cache) {
+                    // the queries not answered early: [split, count), then wait for the early ones
+                    launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
+                                         cg, ctx->stream, early ? &rs->b.counters[kCntQuerySplit] : nullptr);
+                    if (early) {
+                        e = check_hip(ctx, hipStreamWaitEvent(ctx->stream, rs->side_ev[1], 0), "side stream");
+                        if (e != FND_OK) return e;
+                    }
+                    launch_cache_apply(wa, wgrid, ctx->stream);
                 }
-                launch_cache_apply(wa, wgrid, ctx->stream);
+                if (collect)
+                    launch_cache_collect(wa, g0, W, std::max(1u, std::min(wgrid, (wa.b.t_slots + 255u) / 256u)), ctx->stream);
+                launch_accumulate(wa, g0, W, wgrid, ctx->stream);
+                e = check_hip(ctx, hipGetLastError(), "render launch");
+                if (e != FND_OK) return e;
             }
-            if (collect)
-                launch_cache_collect(wa, g0, W, std::max(1u, std::min(wgrid, (wa.b.t_slots + 255u) / 256u)), ctx->stream);
-            launch_accumulate(wa, g0, W, wgrid, ctx->stream);
-            e = check_hip(ctx, hipGetLastError(), "render launch");
-            if (e != FND_OK) return e;
-        }
-        return FND_OK;
-    };
+            return FND_OK;
+        };
 
-    st = check_hip(ctx, hipMemsetAsync(rs->b.rays, 0, sizeof(unsigned long long), ctx->stream), "clear");
-    if (st != FND_OK) return st;
-    (void)hipEventRecord(ctx->ev[2], ctx->stream);
-
-    // ---- the frame; training paths among its own paths record their cache vertices
-    if (any_train) {
-        wa.b.t_slots = 0;
-        for (uint32_t i = 0; i < num_jobs; ++i) wa.b.t_slots += hj[i].train_count;
-        launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
-        launch_fill_u64(wa.b.t_list, wa.b.t_slots, ~0ull, ctx->stream);
-    }
-    st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice,
-                                       ctx->stream), "upload jobs");
-    if (st != FND_OK) return st;
-    st = run(num_jobs, total, cap, max_depth, any_use, any_train);
-    if (st != FND_OK) return st;
-
-    // ---- optimise on the samples the frame gathered (the next frame reads the result)
-    if (any_train) {
-        uint32_t nsamp = 0;
-        st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[kCntTrain], 4, hipMemcpyDeviceToHost, ctx->stream),
-                       "read sample count");
-        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "training paths");
+        st = check_hip(ctx, hipMemsetAsync(rs->b.rays, 0, sizeof(unsigned long long), ctx->stream), "clear");
         if (st != FND_OK) return st;
-        nsamp = std::min(nsamp, cs->sample_cap);
-        if (nsamp) {
-            // step k takes samples k, k + S, k + 2S, ... (per of them): every
-            // step spans the whole frame, and the steps are disjoint
-            const uint32_t steps = std::min(cs->desc.train_steps, nsamp);
-            const uint32_t per = std::max(1u, std::min(cs->desc.train_batch, nsamp / steps));
-            const uint32_t S = nsamp / per;
-            for (uint32_t k = 0; k < steps; ++k) {
-                float* lp = (k + 1 == steps) ? &cs->train_loss_host : nullptr;
-                st = cache_step(ctx, cs, cs->s_query + k, cs->s_target + k, per, S, true, nullptr, nullptr, lp);
-                if (st != FND_OK) return st;
-            }
-            cs->last_samples = nsamp;
+        (void)hipEventRecord(ctx->ev[2], ctx->stream);
+
+        // ---- the frame; training paths among its own paths record their cache vertices
+        if (any_train) {
+            wa.b.t_slots = 0;
+            for (uint32_t i = 0; i < num_jobs; ++i) wa.b.t_slots += hj[i].train_count;
+            launch_zero_slots(wa, 1u << kCntTrain, ctx->stream);
+            launch_fill_u64(wa.b.t_list, wa.b.t_slots, ~0ull, ctx->stream);
         }
-        cs->frames += 1;
+        st = check_hip(ctx, hipMemcpyAsync(rs->d_jobs, hj.data(), hj.size() * sizeof(JobDev), hipMemcpyHostToDevice,
+                                           ctx->stream), "upload jobs");
+        if (st != FND_OK) return st;
+        st = run(num_jobs, total, cap, max_depth, any_use, any_train);
+        if (st != FND_OK) return st;
+
+        // ---- optimise on the samples the frame gathered (the next frame reads the result)
+        if (any_train) {
+            uint32_t nsamp = 0;
+            st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[kCntTrain], 4, hipMemcpyDeviceToHost, ctx->stream),
+                           "read sample count");
+            if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "training paths");
+            if (st != FND_OK) return st;
+            nsamp = std::min(nsamp, cs->sample_cap);
+            if (nsamp) {
+                // step k takes samples k, k + S, k + 2S, ... (per of them): every
+                // step spans the whole frame, and the steps are disjoint
+                const uint32_t steps = std::min(cs->desc.train_steps, nsamp);
+                const uint32_t per = std::max(1u, std::min(cs->desc.train_batch, nsamp / steps));
+                const uint32_t S = nsamp / per;
+                for (uint32_t k = 0; k < steps; ++k) {
+                    float* lp = (k + 1 == steps) ? &cs->train_loss_host : nullptr;
+                    st = cache_step(ctx, cs, cs->s_query + k, cs->s_target + k, per, S, true, nullptr, nullptr, lp);
+                    if (st != FND_OK) return st;
+                }
+                cs->last_samples = nsamp;
+            }
+            cs->frames += 1;
+        }
+        (void)hipEventRecord(ctx->ev[3], ctx->stream);
+        st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "render");
+        if (st != FND_OK) return st;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+        ctx->stats.last_call_ms = ms;
+        unsigned long long rays = 0;
+        st = check_hip(ctx, hipMemcpyAsync(&rays, rs->b.rays, sizeof rays, hipMemcpyDeviceToHost, ctx->stream),
+                       "read ray count");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read ray count");
+        if (st != FND_OK) return st;
+        ctx->stats.render_rays += rays;
+        ctx->stats.paths += total;
+        for (uint32_t i = 0; i < num_jobs; ++i) fbs[i]->samples += jobs[i].spp;
+        return FND_OK;
+    });
+}
+
+fnd_status fnd_camera_basis(const fnd_camera* camera, uint32_t width, uint32_t height, float* out14)
+{
+    if (!camera || !out14 || width == 0 || height == 0) return FND_E_INVALID_ARG;
+    if (camera_problem(*camera)) return FND_E_INVALID_ARG;
+    JobDev j{};
+    camera_basis(*camera, width, height, j);
+    for (int i = 0; i < 3; ++i) {
+        out14[i] = j.eye[i];
+        out14[3 + i] = j.fwd[i];
+        out14[6 + i] = j.right[i];
+        out14[9 + i] = j.up[i];
     }
-    (void)hipEventRecord(ctx->ev[3], ctx->stream);
-    st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "render");
-    if (st != FND_OK) return st;
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-    ctx->stats.last_call_ms = ms;
-    unsigned long long rays = 0;
-    st = check_hip(ctx, hipMemcpyAsync(&rays, rs->b.rays, sizeof rays, hipMemcpyDeviceToHost, ctx->stream),
-                   "read ray count");
-    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "read ray count");
-    if (st != FND_OK) return st;
-    ctx->stats.render_rays += rays;
-    ctx->stats.paths += total;
-    for (uint32_t i = 0; i < num_jobs; ++i) fbs[i]->samples += jobs[i].spp;
+    out14[12] = j.tan_half;
+    out14[13] = j.aspect;
     return FND_OK;
+}
+
+fnd_status fnd_pt_camera_rays(fnd_pt_ctx* ctx, const fnd_camera* camera, uint32_t width, uint32_t height,
+                              uint64_t seed, uint32_t sample, fnd_ray* out)
+{
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !camera || !out) return FND_E_INVALID_ARG;
+        if (width == 0 || height == 0 || width > 16384 || height > 16384)
+            return set_error(ctx, FND_E_INVALID_ARG, "image size %ux%u out of range", width, height);
+        if (const char* why = camera_problem(*camera))
+            return set_error(ctx, FND_E_INVALID_ARG, "degenerate camera (%s)", why);
+        JobDev j{};
+        camera_basis(*camera, width, height, j);
+        j.seed = seed;
+        const uint64_t bytes = uint64_t(width) * height * sizeof(Ray);
+        void* p = nullptr;
+        fnd_status st = dev_alloc(ctx, &p, bytes);
+        if (st != FND_OK) return st;
+        launch_camera_rays(j, sample, static_cast<Ray*>(p), ctx->stream);
+        st = check_hip(ctx, hipGetLastError(), "camera rays");
+        if (st == FND_OK) st = copy_sync(ctx, out, p, bytes, hipMemcpyDeviceToHost, "camera rays");
+        dev_free(ctx, p, bytes);
+        return st;
+    });
+}
+
+fnd_status fnd_pt_rng_draws(fnd_pt_ctx* ctx, uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t count,
+                            uint32_t bounce, uint32_t draws, uint32_t* out)
+{
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || (!out && count && draws)) return FND_E_INVALID_ARG;
+        if (uint64_t(count) * draws > (1ull << 28))
+            return set_error(ctx, FND_E_INVALID_ARG, "at most 2^28 draws per call");
+        if (uint64_t(pixel0) + count > (1ull << 32))
+            return set_error(ctx, FND_E_INVALID_ARG, "pixel index overflows 32 bits");
+        if (count == 0 || draws == 0) return FND_OK;
+        const uint64_t bytes = uint64_t(count) * draws * sizeof(uint32_t);
+        void* p = nullptr;
+        fnd_status st = dev_alloc(ctx, &p, bytes);
+        if (st != FND_OK) return st;
+        launch_rng_draws(seed, sample, pixel0, count, bounce, draws, static_cast<uint32_t*>(p), ctx->stream);
+        st = check_hip(ctx, hipGetLastError(), "generator draws");
+        if (st == FND_OK) st = copy_sync(ctx, out, p, bytes, hipMemcpyDeviceToHost, "generator draws");
+        dev_free(ctx, p, bytes);
+        return st;
+    });
 }
 
 fnd_status fnd_pt_queue_bytes(uint32_t capacity, uint64_t* bytes)
@@ -791,43 +900,47 @@ fnd_status fnd_pt_queue_bytes(uint32_t capacity, uint64_t* bytes)
 
 fnd_status fnd_pt_set_frame_arena(fnd_pt_ctx* ctx, void* memory, uint64_t bytes)
 {
-    if (!ctx) return FND_E_INVALID_ARG;
-    if (memory && reinterpret_cast<uintptr_t>(memory) % kQueueAlign)
-        return set_error(ctx, FND_E_INVALID_ARG, "frame arena must be %llu-byte aligned",
-                         (unsigned long long)kQueueAlign);
-    if (memory && bytes == 0) return set_error(ctx, FND_E_INVALID_ARG, "frame arena of 0 bytes");
-    fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "set frame arena");
-    if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->side), "set frame arena");
-    if (st != FND_OK) return st;
-    if (memory) {  // the library's block is no longer used: give it back
-        if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
-            dev_free(ctx, rs->owned, rs->owned_bytes);
-            rs->owned = nullptr;
-            rs->owned_bytes = 0;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx) return FND_E_INVALID_ARG;
+        if (memory && reinterpret_cast<uintptr_t>(memory) % kQueueAlign)
+            return set_error(ctx, FND_E_INVALID_ARG, "frame arena must be %llu-byte aligned",
+                             (unsigned long long)kQueueAlign);
+        if (memory && bytes == 0) return set_error(ctx, FND_E_INVALID_ARG, "frame arena of 0 bytes");
+        fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "set frame arena");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->side), "set frame arena");
+        if (st != FND_OK) return st;
+        if (memory) {  // the library's block is no longer used: give it back
+            if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
+                dev_free(ctx, rs->owned, rs->owned_bytes);
+                rs->owned = nullptr;
+                rs->owned_bytes = 0;
+            }
         }
-    }
-    if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
-        fnd_arena_reset(&rs->frame, nullptr, 0);
-        rs->b = WfBuffers{};
-        rs->batch = 0;
-    }
-    ctx->frame_memory = memory;
-    ctx->frame_bytes = memory ? bytes : 0;
-    return FND_OK;
+        if (auto* rs = static_cast<RenderState*>(ctx->render_state)) {
+            fnd_arena_reset(&rs->frame, nullptr, 0);
+            rs->b = WfBuffers{};
+            rs->batch = 0;
+        }
+        ctx->frame_memory = memory;
+        ctx->frame_bytes = memory ? bytes : 0;
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_pt_frame_arena_info(fnd_pt_ctx* ctx, fnd_frame_arena_info* out)
 {
-    if (!ctx || !out) return FND_E_INVALID_ARG;
-    std::memset(out, 0, sizeof(*out));
-    const auto* rs = static_cast<const RenderState*>(ctx->render_state);
-    out->caller_owned = ctx->frame_memory ? 1u : 0u;
-    out->size = ctx->frame_memory ? ctx->frame_bytes : (rs ? rs->owned_bytes : 0);
-    if (rs) {
-        out->used = fnd_arena_used(&rs->frame);
-        out->batch_paths = rs->batch;
-    }
-    return FND_OK;
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        std::memset(out, 0, sizeof(*out));
+        const auto* rs = static_cast<const RenderState*>(ctx->render_state);
+        out->caller_owned = ctx->frame_memory ? 1u : 0u;
+        out->size = ctx->frame_memory ? ctx->frame_bytes : (rs ? rs->owned_bytes : 0);
+        if (rs) {
+            out->used = fnd_arena_used(&rs->frame);
+            out->batch_paths = rs->batch;
+        }
+        return FND_OK;
+    });
 }
 
 }  // extern "C"

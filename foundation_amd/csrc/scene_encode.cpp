@@ -12,10 +12,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <thread>
 
 #include "scene_encode.hpp"
+#include "host_pool.hpp"
 
 namespace fnd {
 
@@ -112,17 +114,13 @@ fnd_status encode_scene(const fnd_scene_desc* d, HostScene& out, std::string& er
             build_one(0, hw);
         } else {
             std::atomic<uint32_t> next{0};
-            std::vector<std::thread> pool;
-            auto worker = [&]() {
+            run_pool(std::min<unsigned>(hw, d->num_meshes), [&]() {
                 for (;;) {
                     const uint32_t m = next.fetch_add(1);
                     if (m >= d->num_meshes) break;
                     build_one(m, 1);
                 }
-            };
-            for (unsigned i = 1; i < std::min<unsigned>(hw, d->num_meshes); ++i) pool.emplace_back(worker);
-            worker();
-            for (auto& th : pool) th.join();
+            });
         }
     } catch (const std::bad_alloc&) {
         return fail(err, FND_E_OOM_HOST, "out of host memory building the BVH");
@@ -307,17 +305,17 @@ fnd_status fnd_scene_encode(const fnd_scene_desc* desc, fnd_host_scene** out, ch
 {
     if (!out) return FND_E_INVALID_ARG;
     *out = nullptr;
-    auto* h = new (std::nothrow) fnd_host_scene();
-    if (!h) return FND_E_OOM_HOST;
-    std::string e;
-    const fnd_status st = fnd::encode_scene(desc, h->s, e);
-    if (st != FND_OK) {
-        if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
-        delete h;
-        return st;
-    }
-    *out = h;
-    return FND_OK;
+    return fnd::guard_host(err, err_len, [&]() -> fnd_status {
+        std::unique_ptr<fnd_host_scene> h(new fnd_host_scene());
+        std::string e;
+        const fnd_status st = fnd::encode_scene(desc, h->s, e);
+        if (st != FND_OK) {
+            if (err && err_len) snprintf(err, err_len, "%s", e.c_str());
+            return st;
+        }
+        *out = h.release();
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_host_scene_get(const fnd_host_scene* h, fnd_host_scene_view* v)

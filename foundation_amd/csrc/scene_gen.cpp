@@ -18,12 +18,14 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/fnd_pt.h"
+#include "host_pool.hpp"
 
 namespace {
 
@@ -367,18 +369,15 @@ void gen_instanced(fnd_generated_scene& s, const fnd_gen_params& p)
     s.meshes.resize(ninst + 1);
     // distinct meshes, one seed each, built in parallel
     {
-        std::vector<std::thread> pool;
         const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::atomic<uint32_t> next{0};
-        for (unsigned t = 0; t < nt; ++t)
-            pool.emplace_back([&]() {
-                for (;;) {
-                    const uint32_t i = next.fetch_add(1);
-                    if (i >= ninst) break;
-                    gen_sphere(s.meshes[i], p.subdivision, p.amplitude, p.octaves, uint32_t(p.seed + i), false);
-                }
-            });
-        for (auto& th : pool) th.join();
+        fnd::run_pool(nt, [&]() {
+            for (;;) {
+                const uint32_t i = next.fetch_add(1);
+                if (i >= ninst) break;
+                gen_sphere(s.meshes[i], p.subdivision, p.amplitude, p.octaves, uint32_t(p.seed + i), false);
+            }
+        });
     }
     Pcg rng(p.seed, 0x1257u);
     const double R = 4.5;
@@ -508,33 +507,29 @@ fnd_status fnd_gen_scene(const fnd_gen_params* p, fnd_generated_scene** out)
         (p->subdivision == 0 || p->subdivision > 4096))
         return FND_E_INVALID_ARG;
     if (p->kind == FND_GEN_INSTANCED_SPHERES && (p->instances == 0 || p->instances > 100000)) return FND_E_INVALID_ARG;
-    auto* s = new (std::nothrow) fnd_generated_scene();
-    if (!s) return FND_E_OOM_HOST;
-    try {
+    return fnd::guard_host(nullptr, 0, [&]() -> fnd_status {
+        std::unique_ptr<fnd_generated_scene> s(new fnd_generated_scene());
         switch (p->kind) {
         case FND_GEN_CORNELL_SOUP: gen_cornell(*s, *p); break;
         case FND_GEN_DISPLACED_SPHERE: gen_displaced_sphere(*s, *p); break;
         case FND_GEN_INSTANCED_SPHERES: gen_instanced(*s, *p); break;
         case FND_GEN_FURNACE: gen_furnace(*s, *p); break;
-        default: delete s; return FND_E_INVALID_ARG;
+        default: return FND_E_INVALID_ARG;
         }
-    } catch (const std::bad_alloc&) {
-        delete s;
-        return FND_E_OOM_HOST;
-    }
-    s->mesh_descs.resize(s->meshes.size());
-    for (size_t i = 0; i < s->meshes.size(); ++i) {
-        MeshData& m = s->meshes[i];
-        fnd_mesh_desc& d = s->mesh_descs[i];
-        d.positions = m.pos.data();
-        d.num_vertices = uint32_t(m.pos.size() / 3);
-        d.indices = m.idx.data();
-        d.num_triangles = uint32_t(m.idx.size() / 3);
-        d.material_ids = m.mat.empty() ? nullptr : m.mat.data();
-        d.light_ids = m.light.empty() ? nullptr : m.light.data();
-    }
-    *out = s;
-    return FND_OK;
+        s->mesh_descs.resize(s->meshes.size());
+        for (size_t i = 0; i < s->meshes.size(); ++i) {
+            MeshData& m = s->meshes[i];
+            fnd_mesh_desc& d = s->mesh_descs[i];
+            d.positions = m.pos.data();
+            d.num_vertices = uint32_t(m.pos.size() / 3);
+            d.indices = m.idx.data();
+            d.num_triangles = uint32_t(m.idx.size() / 3);
+            d.material_ids = m.mat.empty() ? nullptr : m.mat.data();
+            d.light_ids = m.light.empty() ? nullptr : m.light.data();
+        }
+        *out = s.release();
+        return FND_OK;
+    });
 }
 
 fnd_status fnd_gen_scene_get(const fnd_generated_scene* s, fnd_scene_desc* d, fnd_camera* cam)

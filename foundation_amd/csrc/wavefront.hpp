@@ -153,5 +153,9 @@ void launch_cache_collect(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t gri
 void launch_fill_u64(unsigned long long* p, uint32_t n, unsigned long long v, hipStream_t s);
 // counters[dst] = counters[src]
 void launch_snapshot(const WfArgs& a, uint32_t src, uint32_t dst, hipStream_t s);
+// Diagnostics: every pixel's camera ray of one sample; keyed generator outputs.
+void launch_camera_rays(const JobDev& job, uint32_t sample, Ray* out, hipStream_t s);
+void launch_rng_draws(uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t npix, uint32_t bounce, uint32_t ndraws,
+                      uint32_t* out, hipStream_t s);
 
 }  // namespace fnd

@@ -492,6 +492,20 @@ class Context:
         arr = (A.RenderJob * len(jobs))(*jobs)
         self._ck(self.lib.fnd_pt_render(self.h, arr, len(jobs)))
 
+    def camera_rays(self, camera, width, height, seed=1, sample=0):
+        """Every pixel's camera ray for `sample` as the renderer draws it
+        (fnd_pt_camera_rays): (H*W, 8) float32."""
+        out = np.zeros((height * width, 8), np.float32)
+        self._ck(self.lib.fnd_pt_camera_rays(self.h, C.byref(camera), width, height, seed, sample, out.ctypes.data))
+        return out
+
+    def rng_draws(self, seed, sample, pixel0, count, bounce, draws):
+        """The first `draws` outputs of the renderer's generator of (seed,
+        sample, pixel, bounce) for `count` pixels from pixel0: (count, draws) u32."""
+        out = np.zeros((count, draws), np.uint32)
+        self._ck(self.lib.fnd_pt_rng_draws(self.h, seed, sample, pixel0, count, bounce, draws, out.ctypes.data))
+        return out
+
     def set_frame_arena(self, device_ptr, nbytes):
         """Per-frame ray queues from the caller's device memory (0: the
         library's own block); the caller keeps the memory alive."""
@@ -511,6 +525,18 @@ class Context:
         self._ck(self.lib.fnd_pt_autotune(self.h, arr, len(jobs), path, A.AUTOTUNE_FORCE if force else 0,
                                           C.byref(res)))
         return res
+
+
+def camera_basis(camera, width, height):
+    """The renderer's camera basis (fnd_camera_basis, host only): dict of
+    eye, forward, right, up (float32 arrays), tan_half and aspect.  Raises
+    FndError for a camera without a basis."""
+    out = np.zeros(14, np.float32)
+    st = A.load().fnd_camera_basis(C.byref(camera), width, height, out.ctypes.data)
+    if st != A.OK:
+        raise FndError(st, "camera has no basis")
+    return dict(eye=out[0:3].copy(), forward=out[3:6].copy(), right=out[6:9].copy(), up=out[9:12].copy(),
+                tan_half=float(out[12]), aspect=float(out[13]))
 
 
 def batch_scene(index):

@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define FND_PT_ABI_VERSION 5u
+#define FND_PT_ABI_VERSION 6u
 
 typedef uint64_t fnd_handle;
 #define FND_INVALID_HANDLE (~(uint64_t)0)
@@ -309,6 +309,32 @@ fnd_status fnd_pt_framebuffer_present(fnd_pt_ctx* ctx, fnd_handle fb, float expo
 /* Renders all jobs in one wavefront launch sequence: their paths form one path
  * space, processed in windows (see fnd_tuning.queue_capacity). */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);
+
+/* ---------------------------------------------------------------- camera and generator checks */
+
+/* The camera basis the renderer forms for a width x height image (host only,
+ * no device): out14 = eye[3], forward[3], right[3], up'[3], tan(fov_y / 2),
+ * aspect, from the glm::lookAt convention of fnd_camera, formed in double and
+ * rounded to float once.  FND_E_INVALID_ARG (also from fnd_pt_render and
+ * fnd_pt_camera_rays) for a camera without a basis: non-finite values,
+ * eye == at, fov_y outside (0, 180) degrees, or an up vector of zero length or
+ * within 1e-6 radians of the view direction. */
+fnd_status fnd_camera_basis(const fnd_camera* camera, uint32_t width, uint32_t height, float* out14);
+
+/* Device building blocks exposed for host-side checks (no counterpart in the
+ * reference; the parity tests compare them with the host oracle bit for bit).
+ * fnd_pt_camera_rays: the camera ray each pixel of a width x height image
+ * starts sample `sample` with (jitter from the generator of (seed, sample,
+ * pixel, bounce 0), as fnd_pt_render draws it): width * height fnd_ray,
+ * row-major, row 0 at the top, tmin 0, tmax 3e38, unit directions.
+ * fnd_pt_rng_draws: the first `draws` 32-bit outputs of the PCG32 generator
+ * the renderer keys by (seed, sample, pixel, bounce), for pixels pixel0 ..
+ * pixel0 + count - 1: out[(p - pixel0) * draws + k]; at most 2^28 outputs.
+ * Both return after the device has finished. */
+fnd_status fnd_pt_camera_rays(fnd_pt_ctx* ctx, const fnd_camera* camera, uint32_t width, uint32_t height,
+                              uint64_t seed, uint32_t sample, fnd_ray* out);
+fnd_status fnd_pt_rng_draws(fnd_pt_ctx* ctx, uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t count,
+                            uint32_t bounce, uint32_t draws, uint32_t* out);
 
 /* ---------------------------------------------------------------- frame arena */
 

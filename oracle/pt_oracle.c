@@ -646,6 +646,11 @@ static void trace_mesh(const omesh* m, const oray* r, uint32_t id_base, obest* b
 static void trace_one(const orc_scene* s, const float* ray8, int anyhit, int brute, float* hit4, uint32_t* cnt2,
                       uint32_t* stack_n, float* stack_t, uint32_t* tstack_n, float* tstack_t)
 {
+    if (orc_ray_rejected(ray8)) {
+        orc_miss_record(ray8, hit4);
+        if (cnt2) cnt2[0] = cnt2[1] = 0;
+        return;
+    }
     oray r;
     oray_setup(&r, ray8, ray8 + 4, ray8[3], ray8[7]);
     obest b = {r.tmax, 0.0f, 0.0f, ORC_NO_HIT, 0, 0, 0};

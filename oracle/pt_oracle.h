@@ -24,9 +24,14 @@
  *   orc_trace (BVH)     brute force over every triangle (orc_trace with
  *                       brute = 1) on scenes up to 50k triangles.
  *   orc_trace (2-level) brute force over every instance and triangle.
- *   orc_render          closed forms (furnace), unbiasedness of NEE vs BSDF
- *                       sampling, pdf normalisation (see the header of
- *                       pt_oracle_render.c).
+ *   orc_render          closed forms (furnace, with and without Russian
+ *                       roulette), unbiasedness of NEE vs BSDF sampling, pdf
+ *                       normalisation (see the header of pt_oracle_render.c).
+ *   orc_camera_rays     closed forms of the glm::lookAt view: the centre ray
+ *                       is normalize(at - eye), the top and bottom edges lie
+ *                       fov/2 above and below it, the side edges at
+ *                       atan(tan(fov/2) * aspect), row 0 at the top.
+ *   orc_sincos_2pi      libm sin/cos within a few ulp, exact at quarter turns.
  */
 #ifndef PT_ORACLE_H
 #define PT_ORACLE_H
@@ -98,6 +103,20 @@ void orc_scene_set_background(orc_scene* s, const float* rgb);
 uint64_t orc_render(const orc_scene* s, const float* cam10, uint32_t width, uint32_t height, uint32_t spp,
                     uint32_t sample_offset, uint32_t max_depth, uint32_t rr_depth, uint64_t seed, uint32_t mode,
                     int nthreads, float* accum, double* accum64);
+
+/* Camera rays as orc_render starts its paths (out8 per ray: origin, 0,
+ * direction, 3e38).  xs/ys/jxy given: n rays through (xs[i] + jxy[2i],
+ * ys[i] + jxy[2i+1]); xs NULL: every pixel in order with the jitter of
+ * `sample` (PCG32 stream of (seed, sample, pixel, bounce 0)). */
+void orc_camera_rays(const float* cam10, uint32_t width, uint32_t height, const uint32_t* xs, const uint32_t* ys,
+                     const float* jxy, uint32_t n, uint64_t seed, uint32_t sample, float* out8);
+/* The first ndraws outputs of the keyed PCG32 stream of (seed, sample,
+ * pixel, bounce) for npix consecutive pixels from pixel0. */
+void orc_rng_draws(uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t npix, uint32_t bounce, uint32_t ndraws,
+                   uint32_t* out);
+/* sin and cos of 2 pi u (out2 = sin, cos per input), the sampler's own
+ * evaluation (exact quarter-turn reduction + Taylor polynomials). */
+void orc_sincos_2pi(const float* u, uint32_t n, float* out2);
 
 /* Local-frame BSDF sampling / evaluation (type 0 Lambert, 1 GGX), for the
  * sampling tests.  out7 = wi[3], weight[3], pdf; out4 = f*cos[3], pdf. */

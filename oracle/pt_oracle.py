@@ -66,6 +66,12 @@ def load():
     lib.orc_cache_loss_grad.restype = C.c_double
     lib.orc_adam_step.argtypes = [P, P, P, P, C.c_uint64, C.c_double, C.c_double, C.c_double, C.c_double, C.c_uint32]
     lib.orc_adam_step.restype = None
+    lib.orc_camera_rays.argtypes = [P, C.c_uint32, C.c_uint32, P, P, P, C.c_uint32, C.c_uint64, C.c_uint32, P]
+    lib.orc_camera_rays.restype = None
+    lib.orc_rng_draws.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P]
+    lib.orc_rng_draws.restype = None
+    lib.orc_sincos_2pi.argtypes = [P, C.c_uint32, P]
+    lib.orc_sincos_2pi.restype = None
     lib.orc_wide_max_stack.argtypes = [C.c_int]
     lib.orc_wide_max_stack.restype = C.c_uint32
     _lib = lib
@@ -217,6 +223,37 @@ def scene_from_generated(gen, leaf_max=4, bins=16, threads=0, shading=False):
         s.set_lights(lights)
         s.set_background(np.array(d.background[:], np.float32))
     return s.build(leaf_max=leaf_max, bins=bins, threads=threads)
+
+
+def camera_rays(cam10, width, height, seed=1, sample=0, points=None):
+    """Camera rays as orc_render starts its paths, (N, 8) float32.  points =
+    (xs, ys, jxy) for explicit image points (x + jx, y + jy); otherwise every
+    pixel with the jitter of `sample`."""
+    cam = np.ascontiguousarray(cam10, np.float32)
+    if points is None:
+        out = np.zeros((width * height, 8), np.float32)
+        load().orc_camera_rays(cam.ctypes.data, width, height, None, None, None, 0, seed, sample, out.ctypes.data)
+        return out
+    xs = np.ascontiguousarray(points[0], np.uint32)
+    ys = np.ascontiguousarray(points[1], np.uint32)
+    jxy = np.ascontiguousarray(points[2], np.float32).reshape(-1, 2)
+    out = np.zeros((xs.size, 8), np.float32)
+    load().orc_camera_rays(cam.ctypes.data, width, height, xs.ctypes.data, ys.ctypes.data, jxy.ctypes.data, xs.size,
+                           seed, sample, out.ctypes.data)
+    return out
+
+
+def rng_draws(seed, sample, pixel0, count, bounce, draws):
+    out = np.zeros((count, draws), np.uint32)
+    load().orc_rng_draws(seed, sample, pixel0, count, bounce, draws, out.ctypes.data)
+    return out
+
+
+def sincos_2pi(u):
+    u = np.ascontiguousarray(u, np.float32).reshape(-1)
+    out = np.zeros((u.size, 2), np.float32)
+    load().orc_sincos_2pi(u.ctypes.data, u.size, out.ctypes.data)
+    return out[:, 0].copy(), out[:, 1].copy()
 
 
 def camera10(cam):

@@ -78,6 +78,31 @@ typedef struct orc_shade_hit {
 
 typedef struct v3f_in { float x, y, z; } v3f_in;
 
+/* A ray the traversal refuses: a non-finite origin or direction component,
+ * or a NaN range bound.  It is answered as a miss (t = tmax as given, no
+ * primitive, u = v = 0) without visiting the tree, as the device does. */
+static inline int orc_ray_rejected(const float* ray8)
+{
+    for (int k = 0; k < 8; ++k) {
+        const float x = ray8[k];
+        if (k == 3 || k == 7) {
+            if (x != x) return 1;
+        } else if (!(x - x == 0.0f)) {
+            return 1;
+        }
+    }
+    return 0;
+}
+
+static inline void orc_miss_record(const float* ray8, float* hit4)
+{
+    const uint32_t none = 0xFFFFFFFFu;
+    hit4[0] = ray8[7];
+    __builtin_memcpy(&hit4[1], &none, 4);
+    hit4[2] = 0.0f;
+    hit4[3] = 0.0f;
+}
+
 /* Closest hit along o + t d, 0 < t < 3e38 (same walk as orc_trace). */
 int orc_closest_shade(const orc_scene* s, v3f_in o, v3f_in d, orc_shade_hit* out);
 /* Any hit with 0 < t < tmax. */

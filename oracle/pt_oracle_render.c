@@ -49,12 +49,48 @@ static inline v3f add(v3f a, v3f b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z)
 static inline v3f sub(v3f a, v3f b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 static inline v3f scl(v3f a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 static inline v3f mul(v3f a, v3f b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-static inline float dot3(v3f a, v3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+/* fma nesting of the device's dot(): the oracle evaluates the device's
+ * expressions in the same order, so a path's radiance has the same bits. */
+static inline float dot3(v3f a, v3f b) { return fmaf(a.x, b.x, fmaf(a.y, b.y, a.z * b.z)); }
 static inline v3f cross3(v3f a, v3f b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 static inline v3f nrm(v3f a) { return scl(a, 1.0f / sqrtf(dot3(a, a))); }
 static inline float maxc(v3f a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 
 #define PI_F 3.14159265358979323846f
+#define INV_PI_F 0.31830988618379067154f
+
+/* sin and cos of 2 pi u, u in [0, 1) a multiple of 2^-24: exact reduction to
+ * the nearest quarter turn (q = round(4u), r = u - q/4 exact, |r| <= 1/8),
+ * x = 2 pi r, Taylor series of sin through x^11 and of cos through x^12 in
+ * Horner form (first omitted terms below 2.2e-9 and 1.1e-10 relative), then
+ * a rotation by q quarter turns. */
+static void sincos_2pi(float u, float* s, float* c)
+{
+    const float qf = floorf(fmaf(u, 4.0f, 0.5f));
+    const float r = fmaf(qf, -0.25f, u);
+    const uint32_t q = (uint32_t)qf & 3u;
+    const float x = r * 6.28318530717958647692f;
+    const float x2 = x * x;
+    float ps = fmaf(x2, 2.7557319223985893e-06f, -1.9841269841269841e-04f);
+    ps = fmaf(x2, ps, 8.3333333333333333e-03f);
+    ps = fmaf(x2, ps, -1.6666666666666667e-01f);
+    const float sx = fmaf(x * x2, ps, x);
+    float pc = fmaf(x2, 2.0876756987868099e-09f, -2.7557319223985891e-07f);
+    pc = fmaf(x2, pc, 2.4801587301587302e-05f);
+    pc = fmaf(x2, pc, -1.3888888888888889e-03f);
+    pc = fmaf(x2, pc, 4.1666666666666667e-02f);
+    pc = fmaf(x2, pc, -0.5f);
+    const float cx = fmaf(x2, pc, 1.0f);
+    float ss = (q & 1u) ? cx : sx, cc = (q & 1u) ? -sx : cx;
+    if (q & 2u) { ss = -ss; cc = -cc; }
+    *s = ss;
+    *c = cc;
+}
+
+void orc_sincos_2pi(const float* u, uint32_t n, float* out2)
+{
+    for (uint32_t i = 0; i < n; ++i) sincos_2pi(u[i], &out2[2 * (size_t)i], &out2[2 * (size_t)i + 1]);
+}
 
 /* SplitMix64 output mixer (Steele, Lea, Flood, OOPSLA 2014). */
 static uint64_t mix64(uint64_t z)
@@ -86,7 +122,7 @@ static void basis(v3f n, v3f* t, v3f* b)
 
 static float ggx_D(float nh, float a2)
 {
-    const float d = nh * nh * (a2 - 1.0f) + 1.0f;
+    const float d = fmaf(nh * nh, a2 - 1.0f, 1.0f);
     return a2 / (PI_F * d * d);
 }
 
@@ -94,14 +130,13 @@ static float ggx_Lambda(float c, float a2)
 {
     const float c2 = c * c;
     const float t2 = fmaxf(0.0f, 1.0f - c2) / fmaxf(c2, 1e-20f);
-    return 0.5f * (-1.0f + sqrtf(1.0f + a2 * t2));
+    return 0.5f * (-1.0f + sqrtf(fmaf(a2, t2, 1.0f)));
 }
 
 static v3f schlick(v3f f0, float c)
 {
-    c = fminf(fmaxf(c, 0.0f), 1.0f);
-    const float m = 1.0f - c;
-    const float m5 = m * m * m * m * m;
+    const float m = 1.0f - fminf(fmaxf(c, 0.0f), 1.0f);
+    const float m5 = (m * m) * (m * m) * m;
     return add(f0, scl(sub(mk(1, 1, 1), f0), m5));
 }
 
@@ -112,17 +147,18 @@ static void bsdf_eval(int type, v3f albedo, float rough, v3f wo, v3f wi, v3f* fc
     *pdf = 0.0f;
     if (wo.z <= 0.0f || wi.z <= 0.0f) return;
     if (type == ORC_MAT_LAMBERT) {
-        *fcos = scl(albedo, wi.z / PI_F);
-        *pdf = wi.z / PI_F;
+        *fcos = scl(albedo, wi.z * INV_PI_F);
+        *pdf = wi.z * INV_PI_F;
         return;
     }
     const float a = rough * rough, a2 = a * a;
     const v3f h = nrm(add(wo, wi));
     const float D = ggx_D(h.z, a2);
     const float Lo = ggx_Lambda(wo.z, a2), Li = ggx_Lambda(wi.z, a2);
+    const float G2 = 1.0f / (1.0f + Lo + Li), G1o = 1.0f / (1.0f + Lo);
     const v3f F = schlick(albedo, dot3(wi, h));
-    *fcos = scl(F, D / (1.0f + Lo + Li) / (4.0f * wo.z));
-    *pdf = D / (1.0f + Lo) / (4.0f * wo.z);
+    *fcos = scl(F, D * G2 / (4.0f * wo.z));
+    *pdf = G1o * D / (4.0f * wo.z);
 }
 
 /* Heitz 2018 visible-normal sample for view wo. */
@@ -132,9 +168,11 @@ static v3f ggx_vndf(v3f wo, float a, float u1, float u2)
     const float l2 = vh.x * vh.x + vh.y * vh.y;
     const v3f T1 = l2 > 0.0f ? scl(mk(-vh.y, vh.x, 0.0f), 1.0f / sqrtf(l2)) : mk(1, 0, 0);
     const v3f T2 = cross3(vh, T1);
-    const float r = sqrtf(u1), phi = 2.0f * PI_F * u2;
-    const float t1 = r * cosf(phi);
-    float t2 = r * sinf(phi);
+    const float r = sqrtf(u1);
+    float sn, cs;
+    sincos_2pi(u2, &sn, &cs);
+    const float t1 = r * cs;
+    float t2 = r * sn;
     const float s = 0.5f * (1.0f + vh.z);
     t2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1)) + s * t2;
     const v3f nh = add(add(scl(T1, t1), scl(T2, t2)), scl(vh, sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2))));
@@ -146,9 +184,11 @@ static int bsdf_sample(int type, v3f albedo, float rough, v3f wo, float u1, floa
 {
     if (wo.z <= 0.0f) return 0;
     if (type == ORC_MAT_LAMBERT) {
-        const float r = sqrtf(u1), phi = 2.0f * PI_F * u2;
-        *wi = mk(r * cosf(phi), r * sinf(phi), sqrtf(fmaxf(0.0f, 1.0f - u1)));
-        *pdf = wi->z / PI_F;
+        const float r = sqrtf(u1);
+        float sn, cs;
+        sincos_2pi(u2, &sn, &cs);
+        *wi = mk(r * cs, r * sn, sqrtf(fmaxf(0.0f, 1.0f - u1)));
+        *pdf = wi->z * INV_PI_F;
         *w = albedo;
         return wi->z > 0.0f;
     }
@@ -211,11 +251,17 @@ static v3f oct_decode(uint32_t c)
     return nrm(mk(u, v, z));
 }
 
+/* Unit normal of a light, formed in FP32 from its edges as the device forms it. */
+static v3f light_n(const orc_light* lt)
+{
+    return nrm(cross3(mk(lt->e0[0], lt->e0[1], lt->e0[2]), mk(lt->e1[0], lt->e1[1], lt->e1[2])));
+}
+
 static float light_weight(const orc_light* lt, v3f p, v3f n)
 {
     const v3f c0 = mk(lt->corner[0], lt->corner[1], lt->corner[2]);
     const v3f e0 = mk(lt->e0[0], lt->e0[1], lt->e0[2]), e1 = mk(lt->e1[0], lt->e1[1], lt->e1[2]);
-    const float side = dot3(mk(lt->n[0], lt->n[1], lt->n[2]), sub(p, c0));
+    const float side = dot3(light_n(lt), sub(p, c0));
     if (!(side > 0.0f)) return 0.0f;
     float cx = 0.0f;
     const v3f cs[4] = {c0, add(c0, e0), add(c0, e1), add(add(c0, e0), e1)};
@@ -264,6 +310,83 @@ static float light_select(const orc_scene* s, v3f p, v3f n, uint32_t li, float u
     return w[li] * inv + floor_share * s->lights[li].pdf_select;
 }
 
+/* Camera of the reference's only view (src/Renderer/Renderer.cpp:373-380):
+ * glm::lookAt with a Z-up world, forward = normalize(at - eye), right =
+ * normalize(cross(forward, up)), up' = cross(right, forward), a vertical
+ * field of view and aspect = width / height.  The basis is formed in double
+ * and rounded once; tan(fov / 2) likewise. */
+static void camera_init(rjob* j, const float* cam10, uint32_t width, uint32_t height)
+{
+    d3 e = {cam10[0], cam10[1], cam10[2]}, a = {cam10[3], cam10[4], cam10[5]}, u = {cam10[6], cam10[7], cam10[8]};
+    d3 f = {a.x - e.x, a.y - e.y, a.z - e.z};
+    double l = sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
+    f.x /= l; f.y /= l; f.z /= l;
+    d3 r = {f.y * u.z - f.z * u.y, f.z * u.x - f.x * u.z, f.x * u.y - f.y * u.x};
+    l = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
+    r.x /= l; r.y /= l; r.z /= l;
+    d3 up = {r.y * f.z - r.z * f.y, r.z * f.x - r.x * f.z, r.x * f.y - r.y * f.x};
+    j->width = width;
+    j->height = height;
+    j->eye = mk((float)e.x, (float)e.y, (float)e.z);
+    j->fwd = mk((float)f.x, (float)f.y, (float)f.z);
+    j->right = mk((float)r.x, (float)r.y, (float)r.z);
+    j->up = mk((float)up.x, (float)up.y, (float)up.z);
+    j->tan_half = (float)tan((double)cam10[9] * M_PI / 360.0);
+    j->aspect = (float)((double)width / (double)height);
+}
+
+/* Unit direction through image point (x + jx, y + jy), row 0 at the top:
+ * screen x in [-1, 1] scaled by tan(fov / 2) * aspect, screen y by tan(fov / 2). */
+static v3f camera_dir(const rjob* j, uint32_t x, uint32_t y, float jx, float jy)
+{
+    const float sx = (2.0f * ((float)x + jx) / (float)j->width - 1.0f) * j->tan_half * j->aspect;
+    const float sy = (1.0f - 2.0f * ((float)y + jy) / (float)j->height) * j->tan_half;
+    return nrm(add(add(j->fwd, scl(j->right, sx)), scl(j->up, sy)));
+}
+
+/* Camera rays as the renderer starts its paths: out8 per point (origin, 0,
+ * direction, 0).  With jitter NULL, point i is (xs[i] + jx[i], ys[i] + jy[i])
+ * given by the caller (closed-form checks); otherwise every pixel of the
+ * image with the jitter of sample `sample` drawn from rng_for(seed, sample,
+ * pixel, 0), in pixel order. */
+void orc_camera_rays(const float* cam10, uint32_t width, uint32_t height, const uint32_t* xs, const uint32_t* ys,
+                     const float* jxy, uint32_t n, uint64_t seed, uint32_t sample, float* out8)
+{
+    rjob j;
+    memset(&j, 0, sizeof j);
+    camera_init(&j, cam10, width, height);
+    const uint32_t count = xs ? n : width * height;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t x, y;
+        float jx, jy;
+        if (xs) {
+            x = xs[i]; y = ys[i]; jx = jxy[2 * (size_t)i]; jy = jxy[2 * (size_t)i + 1];
+        } else {
+            x = i % width; y = i / width;
+            orc_pcg32 rng;
+            rng_for(&rng, seed, sample, i, 0);
+            jx = orc_pcg32_float(&rng);
+            jy = orc_pcg32_float(&rng);
+        }
+        const v3f d = camera_dir(&j, x, y, jx, jy);
+        float* o = out8 + 8 * (size_t)i;
+        o[0] = j.eye.x; o[1] = j.eye.y; o[2] = j.eye.z; o[3] = 0.0f;
+        o[4] = d.x; o[5] = d.y; o[6] = d.z; o[7] = 3.0e38f;
+    }
+}
+
+/* `ndraws` outputs of the PCG32 stream of (seed, sample, pixel, bounce) for
+ * pixels pixel0 .. pixel0 + npix - 1: out[(p - pixel0) * ndraws + k]. */
+void orc_rng_draws(uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t npix, uint32_t bounce, uint32_t ndraws,
+                   uint32_t* out)
+{
+    for (uint32_t p = 0; p < npix; ++p) {
+        orc_pcg32 r;
+        rng_for(&r, seed, sample, pixel0 + p, bounce);
+        for (uint32_t k = 0; k < ndraws; ++k) out[(size_t)p * ndraws + k] = orc_pcg32_next(&r);
+    }
+}
+
 /* mode: 0 = MIS (NEE + BSDF), 1 = BSDF sampling only, 2 = NEE only for direct light */
 static void render_pixel(rjob* j, uint32_t pixel)
 {
@@ -278,10 +401,8 @@ static void render_pixel(rjob* j, uint32_t pixel)
         orc_pcg32 rng;
         rng_for(&rng, j->seed, sample, pixel, 0);
         const float jx = orc_pcg32_float(&rng), jy = orc_pcg32_float(&rng);
-        const float sx = (2.0f * ((float)x + jx) / (float)j->width - 1.0f) * j->tan_half * j->aspect;
-        const float sy = (1.0f - 2.0f * ((float)y + jy) / (float)j->height) * j->tan_half;
         v3f o = j->eye;
-        v3f d = nrm(add(add(j->fwd, scl(j->right, sx)), scl(j->up, sy)));
+        v3f d = camera_dir(j, x, y, jx, jy);
         v3f thr = mk(1, 1, 1), L = mk(0, 0, 0);
         float pdf_prev = 0.0f;
         uint32_t ncode_prev = 0;  /* quantised normal of the previous vertex (MIS of light hits) */
@@ -304,7 +425,7 @@ static void render_pixel(rjob* j, uint32_t pixel)
                 float w = 1.0f;
                 if (bounce > 0 && pdf_prev > 0.0f && h.light >= 0) {
                     const orc_light* lt = &s->lights[h.light];
-                    const float cl = fabsf(dot3(mk(lt->n[0], lt->n[1], lt->n[2]), d));
+                    const float cl = fabsf(dot3(light_n(lt), d));
                     const float ps = light_select(s, o, oct_decode(ncode_prev), (uint32_t)h.light, 0.0f, NULL);
                     const float pl = cl > 0.0f ? ps * h.t * h.t / (lt->area * cl) : 0.0f;
                     w = j->mode == 1 ? 1.0f : (j->mode == 2 ? 0.0f : power_h(pdf_prev, pl));
@@ -332,7 +453,7 @@ static void render_pixel(rjob* j, uint32_t pixel)
                 v3f wi = sub(xl, p);
                 const float d2 = dot3(wi, wi);
                 wi = scl(wi, 1.0f / sqrtf(d2));
-                const float cl = -dot3(mk(lt->n[0], lt->n[1], lt->n[2]), wi);
+                const float cl = -dot3(light_n(lt), wi);
                 if (cl > 0.0f && d2 > 0.0f) {
                     const float pl = psel * d2 / (lt->area * cl);
                     v3f fc;
@@ -398,21 +519,7 @@ uint64_t orc_render(const orc_scene* s, const float* cam10, uint32_t width, uint
     j.s = s; j.width = width; j.height = height; j.spp = spp; j.sample_offset = sample_offset;
     j.max_depth = max_depth; j.rr_depth = rr_depth; j.mode = mode; j.seed = seed;
     j.accum = accum; j.accum64 = accum64;
-    /* glm::lookAt basis in double, rounded once */
-    d3 e = {cam10[0], cam10[1], cam10[2]}, a = {cam10[3], cam10[4], cam10[5]}, u = {cam10[6], cam10[7], cam10[8]};
-    d3 f = {a.x - e.x, a.y - e.y, a.z - e.z};
-    double l = sqrt(f.x * f.x + f.y * f.y + f.z * f.z);
-    f.x /= l; f.y /= l; f.z /= l;
-    d3 r = {f.y * u.z - f.z * u.y, f.z * u.x - f.x * u.z, f.x * u.y - f.y * u.x};
-    l = sqrt(r.x * r.x + r.y * r.y + r.z * r.z);
-    r.x /= l; r.y /= l; r.z /= l;
-    d3 up = {r.y * f.z - r.z * f.y, r.z * f.x - r.x * f.z, r.x * f.y - r.y * f.x};
-    j.eye = mk((float)e.x, (float)e.y, (float)e.z);
-    j.fwd = mk((float)f.x, (float)f.y, (float)f.z);
-    j.right = mk((float)r.x, (float)r.y, (float)r.z);
-    j.up = mk((float)up.x, (float)up.y, (float)up.z);
-    j.tan_half = (float)tan((double)cam10[9] * M_PI / 360.0);
-    j.aspect = (float)((double)width / (double)height);
+    camera_init(&j, cam10, width, height);
     atomic_init(&j.next, 0u);
     atomic_init(&j.rays, 0ull);
     if (nthreads <= 0) {

@@ -25,6 +25,7 @@
 #include <unistd.h>
 
 #include "pt_oracle.h"
+#include "pt_oracle_internal.h"
 
 #define W_SLACK (1.0f + 0x1p-16f)
 #define W_TINY 0x1p-80f
@@ -137,6 +138,11 @@ static void note_stack(uint32_t sp)
 static void walk(const wscene* s, const float* ray8, int anyhit, uint32_t* stk_e, float* stk_t, float* hit4,
                  uint32_t* cnt3, uint32_t stack_cap)
 {
+    if (orc_ray_rejected(ray8)) {
+        orc_miss_record(ray8, hit4);
+        if (cnt3) cnt3[0] = cnt3[1] = cnt3[2] = 0;
+        return;
+    }
     uint32_t max_sp = 0;
     wray r;
     const float* wo = ray8;

@@ -129,6 +129,34 @@ def test_degenerate_rays(ctx, cornell):
     assert ctx.trace(scene, rays[:0]).shape == (0, 4)
 
 
+@pytest.mark.parametrize("which", ["cornell", "instanced"])
+def test_refused_rays_are_misses(ctx, request, which):
+    """A ray with a non-finite origin or direction component, or a NaN range
+    bound, is answered as a miss (t = tmax as given, no primitive) without a
+    walk, as in the oracle; whole waves of such rays (the first 300) leave the
+    other rays' records untouched."""
+    gen, scene, ref = request.getfixturevalue(which)
+    rays = R.primary_rays(gen.camera, 32, 32)
+    bad = rays.copy()
+    bad[:300, 4] = np.nan
+    bad[300:310, 0] = np.inf
+    bad[310:320, 7] = np.nan
+    bad[320:330, 5] = -np.inf
+    bad[330:340, 3] = np.nan
+    bad[340:350, 2] = np.nan
+    refused = np.zeros(len(bad), bool)
+    refused[:350] = True
+    g = ctx.trace(scene, bad)
+    _assert_same(g, ref.trace(bad), f"{which} refused rays")
+    assert np.all(_words(g)[refused, 1] == NO_HIT)
+    assert np.array_equal(_words(g)[refused, 0], bad[refused, 7].view(np.uint32))
+    clean = ctx.trace(scene, rays)
+    assert np.array_equal(_words(g)[~refused], _words(clean)[~refused])
+    occ = _occluded(ctx.trace(scene, bad, anyhit=True))
+    assert not occ[refused].any()
+    assert np.array_equal(occ, _occluded(ref.trace(bad, anyhit=True)))
+
+
 def test_repeat_is_bitwise_identical(ctx, instanced):
     gen, scene, ref = instanced
     _, bounce, _ = _ray_sets(gen, ref, 96, 64, [0, 0, 9])

@@ -336,6 +336,33 @@ def test_wide_walk_equals_binary_walk(oracle):
                 assert np.array_equal(w.view(np.uint32), b.view(np.uint32))
 
 
+def test_refused_rays_are_misses_without_a_walk(oracle):
+    """Rays with a non-finite origin or direction component, or a NaN range
+    bound, get the miss record (t = tmax as given, no primitive, u = v = 0)
+    from the binary and the wide walks, with no node visited; the same rays
+    made finite are ordinary rays that do visit nodes."""
+    gen = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    enc = _encode(gen)
+    binary = oracle.scene_from_generated(gen)
+    rays = R.primary_rays(gen.camera, 8, 8)[:6].copy()
+    rays[0, 4] = np.nan
+    rays[1, 0] = np.inf
+    rays[2, 6] = -np.inf
+    rays[3, 3] = np.nan
+    rays[4, 7] = np.nan
+    rays[5, 1] = np.nan
+    for anyhit in (False, True):
+        w, cnt = oracle.trace_wide(enc, rays, anyhit=anyhit)
+        b = binary.trace(rays, anyhit=anyhit)
+        for h in (w, b):
+            assert np.all(h[:, 1].view(np.uint32) == 0xFFFFFFFF)
+            assert np.array_equal(h[:, 0].view(np.uint32), rays[:, 7].view(np.uint32))
+            assert np.all(h[:, 2:] == 0)
+        assert np.all(cnt == 0)
+    _, cnt = oracle.trace_wide(enc, R.primary_rays(gen.camera, 8, 8)[:6])
+    assert np.all(cnt[:, 0] > 0)
+
+
 def _scene_hash(gen):
     h = hashlib.sha256()
     for m in gen.meshes():

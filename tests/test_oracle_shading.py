@@ -136,6 +136,27 @@ def test_furnace_closed_form(oracle, furnace_scene, depth):
     np.testing.assert_allclose(img, expect, rtol=1e-5)
 
 
+def test_furnace_with_russian_roulette_is_unbiased(oracle, furnace_scene):
+    """Russian roulette from bounce 1 of 8 keeps the furnace expectation
+    L = sum_{k<=8} 0.5^k: survival q = min(max throughput, 0.95) and the
+    survivor's throughput divided by q.  Roulette must also end paths early
+    (fewer rays than the 9 per path without it).  A survivor not divided by
+    q, or roulette applied to the emission of the current vertex, is biased
+    by many standard errors here."""
+    g, s = furnace_scene
+    cam = oracle.camera10(g.camera)
+    w, h, spp, depth = 48, 36, 32, 8
+    acc, _, rays_rr = s.render(cam, w, h, spp, depth, rr_depth=1, seed=5)
+    _, _, rays_plain = s.render(cam, w, h, 1, depth, seed=5)
+    assert rays_plain == w * h * (depth + 1)
+    assert rays_rr < 0.6 * spp * rays_plain
+    px = acc.astype(np.float64) / spp  # per-pixel means of spp independent samples
+    expect = sum(0.5 ** k for k in range(depth + 1))
+    se = px[..., 0].std(ddof=1) / np.sqrt(px[..., 0].size)
+    assert abs(px[..., 0].mean() - expect) < 4.5 * se, (px.mean(), expect, se)
+    assert px[..., 0].std() > 0.01  # the roulette actually varies the estimate
+
+
 def test_estimators_agree_in_expectation(oracle):
     """MIS, BSDF-only and light-sampling-only estimators of the Cornell box
     converge to the same image: their means over 8 independent runs agree
