@@ -21,6 +21,8 @@ GEN_CORNELL_SOUP, GEN_DISPLACED_SPHERE, GEN_INSTANCED_SPHERES, GEN_FURNACE = 1, 
 TRACE_CLOSEST, TRACE_ANYHIT, TRACE_COUNT, ASYNC = 0, 1, 2, 0x100
 JOB_USE_CACHE, JOB_TRAIN_CACHE = 1, 2
 FB_FP64 = 1
+PIPELINE_PATH_TRACE, PIPELINE_TRACE_CLOSEST, PIPELINE_TRACE_ANYHIT, PIPELINE_PRESENT, PIPELINE_CLEAR = 1, 2, 3, 4, 5
+DESCRIPTOR_ACCELERATION_STRUCTURE, DESCRIPTOR_STORAGE_BUFFER, DESCRIPTOR_STORAGE_IMAGE = 1, 2, 3
 
 
 class HostAllocator(C.Structure):
@@ -276,10 +278,33 @@ class HostSceneView(C.Structure):
     ]
 
 
+class Descriptor(C.Structure):
+    _fields_ = [
+        ("binding", C.c_uint32),
+        ("type", C.c_uint32),
+        ("resource", fnd_handle),
+        ("offset", C.c_uint64),
+        ("range", C.c_uint64),
+        ("device_ptr", C.c_void_p),
+    ]
+
+
+class PathTraceConstants(C.Structure):
+    _fields_ = [
+        ("camera", Camera),
+        ("max_depth", C.c_uint32),
+        ("rr_depth", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("sample_offset", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
 def _check_sizes():
     assert C.sizeof(Ray) == 32 and C.sizeof(Hit) == 16
     assert C.sizeof(Material) == 32 and C.sizeof(QuadLight) == 48
     assert C.sizeof(InstanceDesc) == 104
+    assert C.sizeof(Descriptor) == 40 and C.sizeof(PathTraceConstants) == 64
 
 
 _check_sizes()
@@ -320,6 +345,19 @@ PROTOTYPES = {
     "fnd_camera_basis": (_S, [C.POINTER(Camera), C.c_uint32, C.c_uint32, _P]),
     "fnd_pt_camera_rays": (_S, [_P, C.POINTER(Camera), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, _P]),
     "fnd_pt_rng_draws": (_S, [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "fnd_pt_cmd_create": (_S, [_P, C.POINTER(_P)]),
+    "fnd_pt_cmd_destroy": (None, [_P]),
+    "fnd_pt_cmd_begin": (_S, [_P]),
+    "fnd_pt_cmd_set_pipeline": (_S, [_P, C.c_uint32]),
+    "fnd_pt_cmd_bind_descriptors": (_S, [_P, C.POINTER(Descriptor), C.c_uint32]),
+    "fnd_pt_cmd_push_constants": (_S, [_P, _P, C.c_uint32]),
+    "fnd_pt_cmd_dispatch": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "fnd_pt_cmd_end": (_S, [_P]),
+    "fnd_pt_cmd_size": (C.c_uint32, [_P]),
+    "fnd_pt_fence_create": (_S, [_P, C.POINTER(_P)]),
+    "fnd_pt_fence_destroy": (None, [_P]),
+    "fnd_pt_submit": (_S, [_P, C.POINTER(_P), C.c_uint32, _P]),
+    "fnd_pt_fence_wait": (_S, [_P, _P]),
     "fnd_arena_reset": (None, [C.POINTER(Arena), _P, C.c_uint64]),
     "fnd_arena_alloc": (_P, [C.POINTER(Arena), C.c_uint64]),
     "fnd_arena_alloc_aligned": (_P, [C.POINTER(Arena), C.c_uint64, C.c_uint64]),

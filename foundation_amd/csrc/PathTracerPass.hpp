@@ -8,9 +8,11 @@
 //    type with the Allocate(size, alignment) / Reallocate(ptr, size,
 //    alignment) / Deallocate(ptr) members of Core::Allocator
 //    (src/Core/Allocator/Allocator.hpp:27-31) plugs in;
-//  * Execute() records and submits one frame and waits for it, in place of
+//  * Record(cmd) records one frame's compute commands into a command list
+//    (ComputeCommandList, the compute half of the reference's RHICommandList),
+//    and Execute() records, submits and waits on the frame's fence, in place of
 //    Record + Submit + the fence wait in Renderer::Draw
-//    (src/Renderer/Renderer.cpp:382-394);
+//    (src/Renderer/Renderer.cpp:312-394);
 //  * the destructor waits for the device to go idle first, as
 //    Renderer::~Renderer does (src/Renderer/Renderer.cpp:402-406);
 //  * a failed call is fatal, as CHECK -> BugCheck -> std::terminate is
@@ -25,6 +27,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <exception>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -93,6 +96,87 @@ private:
     fnd_arena arena_{};
 };
 
+// A compute command list in the chaining style of the reference's
+// RHICommandList (src/Platform/RHI/Command.hpp:32-116), over fnd_pt_cmd_*:
+// Begin, SetPipeline, BindDescriptorSet, PushConstants, Dispatch, End.  The
+// list belongs to the context that made it (the reference's command pool);
+// failures go to failure_handler(), as every call of the pass does.
+class ComputeCommandList {
+public:
+    explicit ComputeCommandList(fnd_pt_ctx* ctx) : ctx_(ctx) { check(fnd_pt_cmd_create(ctx, &cmd_), "fnd_pt_cmd_create"); }
+    ComputeCommandList(const ComputeCommandList&) = delete;
+    ComputeCommandList& operator=(const ComputeCommandList&) = delete;
+    ~ComputeCommandList() { fnd_pt_cmd_destroy(cmd_); }
+
+    ComputeCommandList& Begin() { return check(fnd_pt_cmd_begin(cmd_), "fnd_pt_cmd_begin"); }
+    ComputeCommandList& SetPipeline(fnd_pipeline pipeline)
+    {
+        return check(fnd_pt_cmd_set_pipeline(cmd_, pipeline), "fnd_pt_cmd_set_pipeline");
+    }
+    ComputeCommandList& BindDescriptorSet(const fnd_descriptor* set, uint32_t count)
+    {
+        return check(fnd_pt_cmd_bind_descriptors(cmd_, set, count), "fnd_pt_cmd_bind_descriptors");
+    }
+    template <class T>
+    ComputeCommandList& PushConstants(const T& constants)
+    {
+        return check(fnd_pt_cmd_push_constants(cmd_, &constants, uint32_t(sizeof(T))), "fnd_pt_cmd_push_constants");
+    }
+    ComputeCommandList& Dispatch(uint32_t x, uint32_t y = 1, uint32_t z = 1)
+    {
+        return check(fnd_pt_cmd_dispatch(cmd_, x, y, z), "fnd_pt_cmd_dispatch");
+    }
+    void End() { check(fnd_pt_cmd_end(cmd_), "fnd_pt_cmd_end"); }
+    uint32_t Size() const { return fnd_pt_cmd_size(cmd_); }
+    fnd_pt_cmd* Handle() const { return cmd_; }
+
+    static fnd_descriptor Scene(uint32_t binding, fnd_handle scene)
+    {
+        fnd_descriptor d{};
+        d.binding = binding;
+        d.type = FND_DESCRIPTOR_ACCELERATION_STRUCTURE;
+        d.resource = scene;
+        return d;
+    }
+    static fnd_descriptor Image(uint32_t binding, fnd_handle framebuffer)
+    {
+        fnd_descriptor d{};
+        d.binding = binding;
+        d.type = FND_DESCRIPTOR_STORAGE_IMAGE;
+        d.resource = framebuffer;
+        return d;
+    }
+    static fnd_descriptor DeviceImage(uint32_t binding, void* device_rgba, uint64_t row_pitch)
+    {
+        fnd_descriptor d{};
+        d.binding = binding;
+        d.type = FND_DESCRIPTOR_STORAGE_IMAGE;
+        d.resource = FND_INVALID_HANDLE;
+        d.range = row_pitch;
+        d.device_ptr = device_rgba;
+        return d;
+    }
+    static fnd_descriptor Buffer(uint32_t binding, fnd_handle buffer, uint64_t offset = 0, uint64_t range = 0)
+    {
+        fnd_descriptor d{};
+        d.binding = binding;
+        d.type = FND_DESCRIPTOR_STORAGE_BUFFER;
+        d.resource = buffer;
+        d.offset = offset;
+        d.range = range;
+        return d;
+    }
+
+private:
+    ComputeCommandList& check(fnd_status st, const char* call)
+    {
+        if (st != FND_OK) failure_handler()(st, call, fnd_pt_last_error(ctx_));
+        return *this;
+    }
+    fnd_pt_ctx* ctx_;
+    fnd_pt_cmd* cmd_ = nullptr;
+};
+
 class PathTracerPass {
 public:
     struct Desc {
@@ -124,7 +208,8 @@ public:
     {
         if (!ctx_) return;
         (void)fnd_pt_wait(ctx_);
-        fnd_pt_destroy(ctx_);  // also releases scenes, buffers and framebuffers
+        cmd_.reset();
+        fnd_pt_destroy(ctx_);  // also releases scenes, buffers, framebuffers and the fence
     }
 
     // ---- scene and view
@@ -165,21 +250,57 @@ public:
     }
 
     // ---- one frame
+    // The pass's commands of one frame: a path-trace dispatch that adds
+    // desc.spp samples per pixel to the pass's image, then, when a
+    // presentation target is set, the R8G8B8A8 conversion into it.  The
+    // reference records its raster pass the same way (Renderer::Record,
+    // src/Renderer/Renderer.cpp:312-366).
+    void Record(ComputeCommandList& cmd)
+    {
+        if (scene_ == FND_INVALID_HANDLE) check(FND_E_INVALID_ARG, "PathTracerPass::Record (no scene set)");
+        fnd_path_trace_constants k{};
+        k.camera = camera_;
+        k.max_depth = desc_.max_depth;
+        k.rr_depth = desc_.rr_depth;
+        k.flags = (desc_.use_cache ? FND_JOB_USE_CACHE : 0u) | (desc_.train_cache ? FND_JOB_TRAIN_CACHE : 0u);
+        k.sample_offset = samples_ + recorded_;
+        k.seed = desc_.seed;
+        const fnd_descriptor set[2] = {ComputeCommandList::Scene(0, scene_), ComputeCommandList::Image(1, fb_)};
+        cmd.SetPipeline(FND_PIPELINE_PATH_TRACE).BindDescriptorSet(set, 2).PushConstants(k).Dispatch(desc_.width,
+                                                                                                      desc_.height,
+                                                                                                      desc_.spp);
+        recorded_ += desc_.spp;
+        if (present_) {
+            const fnd_descriptor out[2] = {ComputeCommandList::Image(0, fb_),
+                                           ComputeCommandList::DeviceImage(1, present_, present_pitch_)};
+            cmd.SetPipeline(FND_PIPELINE_PRESENT).BindDescriptorSet(out, 2).PushConstants(exposure_).Dispatch(
+                desc_.width, desc_.height, 1);
+        }
+    }
+
+    // Records, submits and waits for one frame, in place of Record + Submit +
+    // the fence wait of Renderer::Draw (src/Renderer/Renderer.cpp:382-394).
     void Execute()
     {
-        if (scene_ == FND_INVALID_HANDLE) check(FND_E_INVALID_ARG, "PathTracerPass::Execute (no scene set)");
-        fnd_render_job job{};
-        job.scene = scene_;
-        job.framebuffer = fb_;
-        job.camera = camera_;
-        job.spp = desc_.spp;
-        job.max_depth = desc_.max_depth;
-        job.rr_depth = desc_.rr_depth;
-        job.flags = (desc_.use_cache ? FND_JOB_USE_CACHE : 0u) | (desc_.train_cache ? FND_JOB_TRAIN_CACHE : 0u);
-        job.seed = desc_.seed;
-        job.sample_offset = samples_;
-        check(fnd_pt_render(ctx_, &job, 1), "fnd_pt_render");
-        samples_ += desc_.spp;
+        cmd_->Begin();
+        recorded_ = 0;
+        Record(*cmd_);
+        cmd_->End();
+        fnd_pt_cmd* list = cmd_->Handle();
+        check(fnd_pt_submit(ctx_, &list, 1, fence_), "fnd_pt_submit");
+        check(fnd_pt_fence_wait(ctx_, fence_), "fnd_pt_fence_wait");
+        samples_ += recorded_;
+        recorded_ = 0;
+    }
+
+    // Every Execute() also converts the frame to R8G8B8A8 into this device
+    // memory (the swapchain image's memory imported into HIP, for instance),
+    // rows row_pitch bytes apart; nullptr stops it.
+    void SetPresentTarget(void* device_rgba, uint64_t row_pitch, float exposure = 1.0f)
+    {
+        present_ = device_rgba;
+        present_pitch_ = row_pitch;
+        exposure_ = exposure;
     }
 
     void Reset()
@@ -262,6 +383,8 @@ private:
         check(fnd_pt_create(&cd, alloc, &ctx_), "fnd_pt_create");
         check(fnd_pt_framebuffer_create(ctx_, desc_.width, desc_.height, desc_.fp64 ? FND_FB_FP64 : 0u, &fb_),
               "fnd_pt_framebuffer_create");
+        cmd_.reset(new ComputeCommandList(ctx_));
+        check(fnd_pt_fence_create(ctx_, &fence_), "fnd_pt_fence_create");
     }
 
     void check(fnd_status st, const char* call)
@@ -276,6 +399,12 @@ private:
     fnd_handle scene_ = FND_INVALID_HANDLE;
     fnd_camera camera_{};
     uint32_t samples_ = 0;
+    uint32_t recorded_ = 0;  // samples recorded into the list being built
+    std::unique_ptr<ComputeCommandList> cmd_;
+    fnd_pt_fence* fence_ = nullptr;
+    void* present_ = nullptr;
+    uint64_t present_pitch_ = 0;
+    float exposure_ = 1.0f;
 };
 
 }  // namespace fnd

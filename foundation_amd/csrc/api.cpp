@@ -309,6 +309,7 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
         ctx_delete(ctx, f);
     }
     ctx->framebuffers.clear();
+    commands_free(ctx);
     render_state_free(ctx);
     cache_free(ctx);
     dev_free(ctx, ctx->d_counters, 64 * sizeof(uint32_t));
@@ -569,12 +570,16 @@ fnd_status fnd_pt_buffer_destroy(fnd_pt_ctx* ctx, fnd_handle h)
     });
 }
 
-static Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h)
+}  // extern "C"
+
+Buffer* fnd::find_buffer(fnd_pt_ctx* ctx, fnd_handle h)
 {
     if (handle_kind(h) != kHandleBuffer) return nullptr;
     auto it = ctx->buffers.find(h);
     return it == ctx->buffers.end() ? nullptr : it->second;
 }
+
+extern "C" {
 
 fnd_status fnd_pt_buffer_write(fnd_pt_ctx* ctx, fnd_handle h, uint64_t offset, const void* src, uint64_t bytes)
 {
@@ -602,7 +607,9 @@ fnd_status fnd_pt_buffer_read(fnd_pt_ctx* ctx, fnd_handle h, uint64_t offset, vo
 
 // ---------------------------------------------------------------- ray queries
 
-static fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags)
+}  // extern "C"
+
+fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags)
 {
     const bool any = (flags & FND_TRACE_ANYHIT) != 0;
     const bool count = (flags & FND_TRACE_COUNT) != 0;
@@ -658,6 +665,8 @@ static fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float
     }
     return FND_OK;
 }
+
+extern "C" {
 
 fnd_status fnd_pt_trace_buffers(fnd_pt_ctx* ctx, fnd_handle scene, fnd_handle rays, fnd_handle hits, uint32_t n,
                                 uint32_t flags)

@@ -130,6 +130,7 @@ struct fnd_pt_ctx {
     uint64_t frame_bytes = 0;
     void* render_state = nullptr;              // wavefront buffers (render.cpp)
     void* cache = nullptr;                     // radiance cache (cache_api.cpp)
+    void* commands = nullptr;                  // command lists and fences made by this context (command.cpp)
 };
 
 namespace fnd {
@@ -143,6 +144,19 @@ void dev_free(fnd_pt_ctx* ctx, void* ptr, uint64_t bytes);
 fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_lds, uint32_t threads);
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
 Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
+Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h);
+// One trace launch over device-resident rays and hits (FND_TRACE_* flags;
+// with FND_ASYNC it returns once the launch is queued).
+fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags);
+// Queues the R8G8B8A8 conversion of a framebuffer's mean into device memory
+// (`pitch` in pixels) on the context's stream; no wait.
+fnd_status present_async(fnd_pt_ctx* ctx, Framebuffer* f, float exposure, uint32_t* out, uint32_t pitch);
+// Queues the clearing of a framebuffer's sums and zeroes its sample count; no wait.
+fnd_status clear_async(fnd_pt_ctx* ctx, Framebuffer* f);
+// Why a camera has no basis (null when it has one).
+const char* camera_problem(const fnd_camera& c);
+// Command lists and fences still alive when the context is destroyed.
+void commands_free(fnd_pt_ctx* ctx);
 uint32_t trace_grid(const fnd_pt_ctx* ctx, const TraceLaunch& l);
 void resolve_timings(fnd_pt_ctx* ctx);
 fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);

@@ -226,6 +226,8 @@ void select_training(JobDev& j, uint32_t spp, uint32_t base, uint32_t count, uin
     j.train_count = uint32_t(std::min<uint64_t>(count, std::max<uint64_t>(1, j.train_mod / kTrainShare)));
 }
 
+}  // namespace
+
 // Why a camera cannot form a basis, or null: non-finite values, eye == at, a
 // field of view outside (0, 180) degrees, or an up vector of no length or
 // (nearly) parallel to the view direction (the right vector would be the
@@ -249,6 +251,8 @@ const char* camera_problem(const fnd_camera& c)
     if (!(lr > 1e-6 * lf * lu)) return "up vector parallel to the view direction";
     return nullptr;
 }
+
+namespace {
 
 void camera_basis(const fnd_camera& c, uint32_t w, uint32_t h, JobDev& j)
 {
@@ -287,6 +291,27 @@ fnd_status present_table(fnd_pt_ctx* ctx)
 }
 
 }  // namespace
+
+fnd_status present_async(fnd_pt_ctx* ctx, Framebuffer* f, float exposure, uint32_t* out, uint32_t pitch)
+{
+    fnd_status st = present_table(ctx);
+    if (st != FND_OK) return st;
+    const uint64_t pixels = uint64_t(f->width) * f->height;
+    const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
+    launch_present(f->accum, pixels, inv, exposure, ctx->d_present, out, f->width, pitch, ctx->stream);
+    return check_hip(ctx, hipGetLastError(), "present");
+}
+
+fnd_status clear_async(fnd_pt_ctx* ctx, Framebuffer* f)
+{
+    const uint64_t n = uint64_t(f->width) * f->height * 3;
+    fnd_status st = check_hip(ctx, hipMemsetAsync(f->accum, 0, n * sizeof(float), ctx->stream), "clear");
+    if (st == FND_OK && f->accum64)
+        st = check_hip(ctx, hipMemsetAsync(f->accum64, 0, n * sizeof(double), ctx->stream), "clear");
+    if (st == FND_OK) f->samples = 0;
+    return st;
+}
+
 }  // namespace fnd
 
 extern "C" {
@@ -358,12 +383,8 @@ fnd_status fnd_pt_framebuffer_clear(fnd_pt_ctx* ctx, fnd_handle h)
         if (!ctx) return FND_E_INVALID_ARG;
         Framebuffer* f = find_framebuffer(ctx, h);
         if (!f) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
-        const uint64_t n = uint64_t(f->width) * f->height * 3;
-        fnd_status st = check_hip(ctx, hipMemsetAsync(f->accum, 0, n * sizeof(float), ctx->stream), "clear");
-        if (st == FND_OK && f->accum64)
-            st = check_hip(ctx, hipMemsetAsync(f->accum64, 0, n * sizeof(double), ctx->stream), "clear");
+        fnd_status st = clear_async(ctx, f);
         if (st != FND_OK) return st;
-        f->samples = 0;
         return check_hip(ctx, hipStreamSynchronize(ctx->stream), "clear");
     });
 }
@@ -436,13 +457,7 @@ fnd_status fnd_pt_framebuffer_present(fnd_pt_ctx* ctx, fnd_handle h, float expos
             return set_error(ctx, FND_E_INVALID_ARG,
                              "present target must be 4-byte aligned with a row pitch of at least %u bytes, a multiple of 4",
                              4 * f->width);
-        fnd_status st = present_table(ctx);
-        if (st != FND_OK) return st;
-        const uint64_t pixels = uint64_t(f->width) * f->height;
-        const float inv = f->samples ? 1.0f / float(f->samples) : 1.0f;
-        launch_present(f->accum, pixels, inv, exposure, ctx->d_present, static_cast<uint32_t*>(device_rgba), f->width,
-                       uint32_t(row_pitch / 4), ctx->stream);
-        st = check_hip(ctx, hipGetLastError(), "present");
+        fnd_status st = present_async(ctx, f, exposure, static_cast<uint32_t*>(device_rgba), uint32_t(row_pitch / 4));
         if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "present");
         if (st != FND_OK) return st;
         if (out_samples) *out_samples = f->samples;

@@ -506,6 +506,24 @@ class Context:
         self._ck(self.lib.fnd_pt_rng_draws(self.h, seed, sample, pixel0, count, bounce, draws, out.ctypes.data))
         return out
 
+    def command_list(self):
+        return CommandList(self)
+
+    def fence_create(self):
+        h = C.c_void_p()
+        self._ck(self.lib.fnd_pt_fence_create(self.h, C.byref(h)))
+        return h
+
+    def fence_destroy(self, fence):
+        self.lib.fnd_pt_fence_destroy(fence)
+
+    def submit(self, lists, fence=None):
+        arr = (C.c_void_p * max(1, len(lists)))(*[cl.h for cl in lists])
+        self._ck(self.lib.fnd_pt_submit(self.h, arr, len(lists), fence))
+
+    def fence_wait(self, fence):
+        self._ck(self.lib.fnd_pt_fence_wait(self.h, fence))
+
     def set_frame_arena(self, device_ptr, nbytes):
         """Per-frame ray queues from the caller's device memory (0: the
         library's own block); the caller keeps the memory alive."""
@@ -525,6 +543,76 @@ class Context:
         self._ck(self.lib.fnd_pt_autotune(self.h, arr, len(jobs), path, A.AUTOTUNE_FORCE if force else 0,
                                           C.byref(res)))
         return res
+
+
+class CommandList:
+    """A compute command list of a context (fnd_pt_cmd_*), chained like the
+    reference's RHICommandList (src/Platform/RHI/Command.hpp:32-116):
+    ``cl.begin().set_pipeline(p).bind(...).push_constants(c).dispatch(x, y, z).end()``.
+    A refused call raises FndError at once."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._ck(ctx.lib.fnd_pt_cmd_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def destroy(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.fnd_pt_cmd_destroy(self.h)
+        self.h = None
+
+    def begin(self):
+        self.ctx._ck(self.ctx.lib.fnd_pt_cmd_begin(self.h))
+        return self
+
+    def set_pipeline(self, pipeline):
+        self.ctx._ck(self.ctx.lib.fnd_pt_cmd_set_pipeline(self.h, pipeline))
+        return self
+
+    def bind(self, *descriptors):
+        """Descriptors as (binding, type, resource[, offset, range, device_ptr])."""
+        arr = (A.Descriptor * max(1, len(descriptors)))()
+        for i, d in enumerate(descriptors):
+            b, t, r = d[:3]
+            arr[i].binding, arr[i].type, arr[i].resource = b, t, r
+            if len(d) > 3:
+                arr[i].offset = d[3]
+            if len(d) > 4:
+                arr[i].range = d[4]
+            if len(d) > 5:
+                arr[i].device_ptr = d[5]
+        self.ctx._ck(self.ctx.lib.fnd_pt_cmd_bind_descriptors(self.h, arr, len(descriptors)))
+        return self
+
+    def push_constants(self, data):
+        """A ctypes structure or a bytes-like object."""
+        buf = bytes(data) if not isinstance(data, (bytes, bytearray)) else bytes(data)
+        cbuf = C.create_string_buffer(buf, len(buf))
+        self.ctx._ck(self.ctx.lib.fnd_pt_cmd_push_constants(self.h, cbuf, len(buf)))
+        return self
+
+    def dispatch(self, x, y=1, z=1):
+        self.ctx._ck(self.ctx.lib.fnd_pt_cmd_dispatch(self.h, x, y, z))
+        return self
+
+    def end(self):
+        self.ctx._ck(self.ctx.lib.fnd_pt_cmd_end(self.h))
+        return self
+
+    def size(self):
+        return int(self.ctx.lib.fnd_pt_cmd_size(self.h))
+
+
+def path_trace_constants(camera, max_depth, rr_depth=None, seed=1, sample_offset=0, flags=0):
+    k = A.PathTraceConstants()
+    k.camera = camera
+    k.max_depth = max_depth
+    k.rr_depth = max_depth if rr_depth is None else rr_depth
+    k.flags = flags
+    k.sample_offset = sample_offset
+    k.seed = seed
+    return k
 
 
 def camera_basis(camera, width, height):

@@ -336,6 +336,102 @@ fnd_status fnd_pt_camera_rays(fnd_pt_ctx* ctx, const fnd_camera* camera, uint32_
 fnd_status fnd_pt_rng_draws(fnd_pt_ctx* ctx, uint64_t seed, uint32_t sample, uint32_t pixel0, uint32_t count,
                             uint32_t bounce, uint32_t draws, uint32_t* out);
 
+/* ---------------------------------------------------------------- compute command lists */
+
+/* The compute half the reference's RHI lacks.  Its RHICommandList records
+ * transitions, pipeline and descriptor binds and draws, and a queue submits
+ * lists with a fence the host waits on (src/Platform/RHI/Command.hpp:32-116,
+ * src/Platform/RHI/Device.hpp:15-37, src/Renderer/Renderer.cpp:312-394), but
+ * there is no Dispatch, the pipeline state is graphics-only
+ * (src/Platform/RHI/Vulkan/PipelineState.cpp:103-110) and no descriptor type
+ * names a storage image (src/Platform/RHI/Common.hpp:67-74).  Here a command
+ * list records, in the same order of calls, compute pipelines, descriptor
+ * sets (scenes as acceleration structures, storage buffers, storage images),
+ * push constants and dispatches; fnd_pt_submit runs the recorded lists in
+ * order on the context's stream and signals a fence, and fnd_pt_fence_wait is
+ * the host's fence wait.  Lists and fences belong to the context that made
+ * them (destroying it destroys them, the factory rule of
+ * src/Platform/RHI/Details/Details.hpp:39-46).
+ *
+ * Pipelines and what a dispatch of each needs (bindings by number):
+ *   FND_PIPELINE_PATH_TRACE    0 scene, 1 accumulation image (framebuffer);
+ *                              constants fnd_path_trace_constants; dispatch
+ *                              (width, height, spp) of that framebuffer; adds
+ *                              spp samples per pixel, as one fnd_pt_render job.
+ *                              Consecutive path-trace dispatches into
+ *                              different images run as one batched launch.
+ *   FND_PIPELINE_TRACE_CLOSEST 0 scene, 1 rays (storage buffer of fnd_ray),
+ *   FND_PIPELINE_TRACE_ANYHIT  2 hits (storage buffer of fnd_hit); dispatch
+ *                              (rays, 1, 1), both buffers large enough.
+ *   FND_PIPELINE_PRESENT       0 accumulation image, 1 R8G8B8A8 storage image
+ *                              in device memory (device_ptr, row pitch in
+ *                              `range`); constants: float exposure; dispatch
+ *                              (width, height, 1); the bytes of
+ *                              fnd_pt_framebuffer_present.
+ *   FND_PIPELINE_CLEAR         0 accumulation image; dispatch (width, height,
+ *                              1); the image's sums and sample count to zero.
+ * Recording checks each call against the bound pipeline and returns
+ * FND_E_INVALID_ARG for a mismatch; a list with a refused call cannot be
+ * submitted until the next fnd_pt_cmd_begin.  Resources must stay alive until
+ * the fence of their submission has signalled. */
+typedef struct fnd_pt_cmd fnd_pt_cmd;
+typedef struct fnd_pt_fence fnd_pt_fence;
+
+typedef enum fnd_pipeline {
+    FND_PIPELINE_PATH_TRACE = 1,
+    FND_PIPELINE_TRACE_CLOSEST = 2,
+    FND_PIPELINE_TRACE_ANYHIT = 3,
+    FND_PIPELINE_PRESENT = 4,
+    FND_PIPELINE_CLEAR = 5
+} fnd_pipeline;
+
+typedef enum fnd_descriptor_type {
+    FND_DESCRIPTOR_ACCELERATION_STRUCTURE = 1,  /* resource: a scene handle */
+    FND_DESCRIPTOR_STORAGE_BUFFER = 2,          /* resource: a buffer handle; offset, range (0: to the end) */
+    FND_DESCRIPTOR_STORAGE_IMAGE = 3            /* resource: a framebuffer handle, or FND_INVALID_HANDLE with
+                                                   device_ptr = R8G8B8A8 device memory and range = row pitch */
+} fnd_descriptor_type;
+
+typedef struct fnd_descriptor {
+    uint32_t binding;
+    uint32_t type;                 /* fnd_descriptor_type */
+    fnd_handle resource;
+    uint64_t offset;
+    uint64_t range;
+    void* device_ptr;
+} fnd_descriptor;
+
+typedef struct fnd_path_trace_constants {
+    fnd_camera camera;
+    uint32_t max_depth;
+    uint32_t rr_depth;
+    uint32_t flags;                /* FND_JOB_* */
+    uint32_t sample_offset;
+    uint64_t seed;
+} fnd_path_trace_constants;
+
+fnd_status fnd_pt_cmd_create(fnd_pt_ctx* ctx, fnd_pt_cmd** out);
+void fnd_pt_cmd_destroy(fnd_pt_cmd* cmd);
+/* Begin() of RHICommandList: forgets what was recorded (and a refused call). */
+fnd_status fnd_pt_cmd_begin(fnd_pt_cmd* cmd);
+fnd_status fnd_pt_cmd_set_pipeline(fnd_pt_cmd* cmd, uint32_t pipeline);
+/* Replaces the bound set; bindings must be distinct. */
+fnd_status fnd_pt_cmd_bind_descriptors(fnd_pt_cmd* cmd, const fnd_descriptor* set, uint32_t count);
+fnd_status fnd_pt_cmd_push_constants(fnd_pt_cmd* cmd, const void* data, uint32_t bytes);
+fnd_status fnd_pt_cmd_dispatch(fnd_pt_cmd* cmd, uint32_t x, uint32_t y, uint32_t z);
+fnd_status fnd_pt_cmd_end(fnd_pt_cmd* cmd);
+/* Commands recorded since the last begin (for tools and tests). */
+uint32_t fnd_pt_cmd_size(const fnd_pt_cmd* cmd);
+
+fnd_status fnd_pt_fence_create(fnd_pt_ctx* ctx, fnd_pt_fence** out);
+void fnd_pt_fence_destroy(fnd_pt_fence* fence);
+/* Runs `count` ended lists in order; `fence` (may be NULL) signals when the
+ * device has finished them. */
+fnd_status fnd_pt_submit(fnd_pt_ctx* ctx, fnd_pt_cmd* const* cmds, uint32_t count, fnd_pt_fence* fence);
+/* The host waits until the fence has signalled (a fence never submitted
+ * counts as signalled). */
+fnd_status fnd_pt_fence_wait(fnd_pt_ctx* ctx, fnd_pt_fence* fence);
+
 /* ---------------------------------------------------------------- frame arena */
 
 /* Bump allocation over one block of memory, with the semantics of the

@@ -53,6 +53,20 @@ def test_null_context_calls_fail_cleanly(lib):
     assert lib.fnd_pt_trace(None, 0, None, 0, None, 0) == A.E_INVALID_ARG
     assert lib.fnd_pt_render(None, None, 0) == A.E_INVALID_ARG
     lib.fnd_pt_destroy(None)
+    h = C.c_void_p()
+    assert lib.fnd_pt_cmd_create(None, C.byref(h)) == A.E_INVALID_ARG and not h.value
+    for fn in (lib.fnd_pt_cmd_begin, lib.fnd_pt_cmd_end):
+        assert fn(None) == A.E_INVALID_ARG
+    assert lib.fnd_pt_cmd_set_pipeline(None, A.PIPELINE_PATH_TRACE) == A.E_INVALID_ARG
+    assert lib.fnd_pt_cmd_dispatch(None, 1, 1, 1) == A.E_INVALID_ARG
+    assert lib.fnd_pt_cmd_size(None) == 0
+    assert lib.fnd_pt_fence_create(None, C.byref(h)) == A.E_INVALID_ARG
+    assert lib.fnd_pt_submit(None, None, 0, None) == A.E_INVALID_ARG
+    assert lib.fnd_pt_fence_wait(None, None) == A.E_INVALID_ARG
+    lib.fnd_pt_cmd_destroy(None)
+    lib.fnd_pt_fence_destroy(None)
+    assert lib.fnd_pt_camera_rays(None, None, 1, 1, 0, 0, None) == A.E_INVALID_ARG
+    assert lib.fnd_pt_rng_draws(None, 0, 0, 0, 1, 0, 1, None) == A.E_INVALID_ARG
 
 
 def test_allocator_oom_is_reported(lib):
