@@ -225,6 +225,22 @@ def batch64(ctx, count, spp, depth, cache_path):
         ctx.render(jobs)
         st = ctx.stats()
         ms = st.last_call_ms
+        # Held-out check of the tuner: the full workload (every scene at spp
+        # samples) on samples the search never rendered, under the tuned
+        # configuration, the library default and the first round's median.
+        held = {}
+        for name, t in (("tuned", tun), ("default", None), ("median", res.median)):
+            if t is None:
+                ctx.set_tuning(block_size=0, stack_lds=0, waves_per_cu=0, refill=0, queue_capacity=0, tile=0)
+            else:
+                ctx.set_tuning(**{k: getattr(t, k) for k, _ in t._fields_})
+            for j in jobs:
+                j.sample_offset = spp
+            ctx.render(jobs)
+            held[f"{name}_ms"] = round(ctx.stats().last_call_ms, 2)
+        held["tuned_vs_default"] = round(held["default_ms"] / held["tuned_ms"], 3)
+        held["tuned_vs_median"] = round(held["median_ms"] / held["tuned_ms"], 3)
+        log(f"batch held-out: {held}")
         return {"scenes": count, "triangles": int(tris), "pixels": int(pixels), "spp": spp, "max_depth": depth,
                 "paths": int(pixels * spp), "build_s": round(build_s, 1), "seconds": round(ms / 1e3, 3),
                 "msamples_s": round(pixels * spp / ms / 1e3, 1), "render_mrays_s": round(st.render_rays / ms / 1e3, 1),
@@ -232,6 +248,7 @@ def batch64(ctx, count, spp, depth, cache_path):
                              "seconds": round(tune_s, 1), "from_cache": bool(res.from_cache),
                              "best_ms_1spp": round(res.best_ms, 2), "default_ms_1spp": round(res.default_ms, 2),
                              "median_ms_1spp": round(res.median_ms, 2),
+                             "held_out_full_workload": held,
                              "tuning": {k: getattr(tun, k) for k, _ in tun._fields_}}}
     finally:
         for f in fbs:
@@ -303,26 +320,129 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
                 flags = A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE
             fb = ctx.framebuffer_create(width, height)
             gpu_ms = 0.0
+            wall = 0.0  # host wall clock around each render call (relMSE evaluation excluded)
             e = float("inf")
             f = 0
             while f < max_frames and e > target:
+                t0 = time.perf_counter()
                 ctx.render([pt.make_job(scene, fb, gen.camera, 1, depth, rr_depth=rr, seed=5, sample_offset=f,
                                         flags=flags)])
+                wall += time.perf_counter() - t0
                 gpu_ms += ctx.stats().last_call_ms
                 f += 1
                 e = rel_mse(ctx.framebuffer_read(fb, width, height)[0], ref)
             ctx.framebuffer_destroy(fb)
             out[mode] = {"frames": f, "seconds": round(gpu_ms / 1e3, 3), "ms_per_frame": round(gpu_ms / f, 3),
-                         "rel_mse": round(e, 5), "reached": bool(e <= target)}
+                         "wall_seconds": round(wall, 3), "rel_mse": round(e, 5), "reached": bool(e <= target)}
             log(f"relMSE {mode}: {e:.4g} after {f} frames, {gpu_ms / 1e3:.2f} s")
         if out["plain"]["reached"] and out["cache"]["reached"]:
             out["speedup_cache_vs_plain"] = round(out["plain"]["seconds"] / out["cache"]["seconds"], 3)
+            out["speedup_cache_vs_plain_wall"] = round(out["plain"]["wall_seconds"] / out["cache"]["wall_seconds"], 3)
     finally:
         d = A.CacheDesc()
         d.enable = 0
         ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d))
         ctx.scene_destroy(scene)
     return out
+
+
+CPU_SHARE = 16  # host threads a one-GPU job may use on the GPU box (its CPU share)
+
+
+def kernel_source_sha():
+    """SHA-256 (16 hex digits) of the traversal kernel's sources: a PMC summary
+    made from other sources no longer describes the kernel that runs."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in ("trace_kernel.hpp", "pt_device.hpp", "pt_layout.hpp", "trace_launch.hip"):
+        with open(os.path.join(ROOT, "foundation_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def host_facts(threads):
+    """What the CPU baseline ran on: threads used, CPUs online and in this
+    process's affinity mask, and the processor model from /proc/cpuinfo."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"threads": threads, "cpus_online": os.cpu_count(), "cpus_in_affinity": affinity, "model": model,
+            "note": f"threads = min({CPU_SHARE}, CPUs in the affinity mask): the CPU share of one GPU on this box"}
+
+
+def c1_leg(ctx, threads, with_cpu):
+    """The CPU configuration C1 in full on both sides: Cornell box + 10,000
+    soup triangles, 256x256, 16 spp, depth 4, PCG seed 1.  GPU device time
+    and oracle wall time (1 warm-up + median of 3), samples/s of each, and
+    whether the two images are equal bit for bit (relMSE between them)."""
+    import numpy as np
+    from foundation_amd import _abi as A
+    from foundation_amd import pt
+    from foundation_amd.metrics import rel_mse
+    gen = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    scene = ctx.scene_create(gen.desc)
+    w = h = 256
+    spp, depth = 16, 4
+    try:
+        fb = ctx.framebuffer_create(w, h)
+        times = []
+        for _ in range(4):
+            ctx.framebuffer_clear(fb)
+            ctx.render([pt.make_job(scene, fb, gen.camera, spp, depth, seed=1)])
+            times.append(ctx.stats().last_call_ms)
+        gpu_img = ctx.framebuffer_read(fb, w, h)[0]
+        ctx.framebuffer_destroy(fb)
+        gms = statistics.median(times[1:])
+        out = {"workload": "Cornell box + 10k soup, 256x256, 16 spp, depth 4, seed 1",
+               "gpu_ms": round(gms, 3), "gpu_msamples_s": round(w * h * spp / gms / 1e3, 1)}
+        if with_cpu:
+            from oracle import pt_oracle as O
+            osc = O.scene_from_generated(gen, shading=True, threads=threads)
+            cam = O.camera10(gen.camera)
+            ctimes = []
+            for k in range(4):
+                t = time.perf_counter()
+                acc, _, rays = osc.render(cam, w, h, spp, depth, seed=1, threads=threads)
+                if k:
+                    ctimes.append(time.perf_counter() - t)
+            cs = statistics.median(ctimes)
+            ref = acc * (np.float32(1.0) / np.float32(spp))
+            out.update({"cpu_s": round(cs, 3), "cpu_msamples_s": round(w * h * spp / cs / 1e6, 3),
+                        "cpu_mrays_s": round(rays / cs / 1e6, 2), "cpu_threads": threads,
+                        "gpu_vs_cpu": round((w * h * spp / gms / 1e3) / (w * h * spp / cs / 1e6), 1),
+                        "images_bit_exact": bool(np.array_equal(gpu_img.view(np.uint32), ref.view(np.uint32))),
+                        "rel_mse_gpu_vs_cpu": float(rel_mse(gpu_img, ref.astype(np.float64)))})
+            osc.close()
+        return out
+    finally:
+        ctx.scene_destroy(scene)
+
+
+def c3_cpu_subset(gen, threads):
+    """The oracle's samples/s on a bounded sample of C3: the 10.08M-triangle
+    scene at 1 spp, 480x270 (the 1080p view at a quarter of the width and
+    height), depth 8, roulette from 3, NEE + MIS on the 4 quad lights."""
+    from oracle import pt_oracle as O
+    osc = O.scene_from_generated(gen, shading=True, threads=threads)
+    cam = O.camera10(gen.camera)
+    w, h = 480, 270
+    osc.render(cam, 96, 54, 1, 8, rr_depth=3, seed=1, threads=threads)  # warm-up
+    t = time.perf_counter()
+    _, _, rays = osc.render(cam, w, h, 1, 8, rr_depth=3, seed=1, threads=threads)
+    s = time.perf_counter() - t
+    osc.close()
+    return {"sample": "10.08M-triangle scene, 480x270, 1 spp, depth 8, RR from 3", "seconds": round(s, 3),
+            "msamples_s": round(w * h / s / 1e6, 4), "mrays_s": round(rays / s / 1e6, 3), "threads": threads}
 
 
 def main():
@@ -334,7 +454,8 @@ def main():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--instances", type=int, default=100)
     ap.add_argument("--subdivision", type=int, default=71)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, online CPUs)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0: min(16, CPUs in the affinity mask), the CPU share of one GPU on the box")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity check")
     ap.add_argument("--no-render", action="store_true",
                     help="skip the path-tracing legs (samples/s and time to relMSE 0.01, plain vs cache)")
@@ -367,6 +488,11 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    try:
+        cpu_threads = min(CPU_SHARE, len(os.sched_getaffinity(0)))
+    except (AttributeError, OSError):
+        cpu_threads = min(CPU_SHARE, os.cpu_count() or 1)
 
     if args.frames_only:
         frames_only(args.frames_only, args.relmse_width, args.relmse_height, 8, 3)
@@ -426,14 +552,27 @@ def main():
         shadow_rate = shadow.shape[0] * csteps / swall / 1e6
         log(f"coherent: {coherent:.1f} Mrays/s, shadow any-hit: {shadow_rate:.1f} Mrays/s")
 
+    # Counter figures come from the committed rocprof PMC summary of this
+    # kernel; the summary records the hash of the kernel sources it was
+    # measured on, so a summary of an older kernel is reported as stale.
     traffic = None
     valu = None
+    pmc_source = None
     if os.path.exists(args.pmc_summary):
         try:
             pmc = json.load(open(args.pmc_summary))
             traffic = pmc.get("hbm_bytes_per_launch")
             v = pmc.get("valu") or {}
             valu = {k: v[k] for k in ("valu_busy", "lane_utilization") if k in v} or None
+            if valu and "valu_busy" in valu and "lane_utilization" in valu:
+                valu["useful_lane_frac"] = round(valu["valu_busy"] * valu["lane_utilization"], 4)
+            cur = kernel_source_sha()
+            pmc_source = {"file": os.path.relpath(args.pmc_summary, ROOT),
+                          "kernel_source_sha": pmc.get("kernel_source_sha"), "current_kernel_source_sha": cur,
+                          "stale": pmc.get("kernel_source_sha") != cur}
+            if pmc_source["stale"]:
+                log(f"PMC summary {pmc_source['file']} was measured on other kernel sources: traffic and VALU "
+                    "figures are reported as stale")
         except (OSError, ValueError):
             traffic = None
 
@@ -441,7 +580,7 @@ def main():
     parity = None
     if not args.no_cpu and not args.only_incoherent:
         from oracle import pt_oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or cpu_threads
         t = time.perf_counter()
         oscene = O.scene_from_generated(gen, threads=threads)
         log(f"oracle built in {time.perf_counter() - t:.1f}s")
@@ -456,7 +595,8 @@ def main():
             times.append(time.perf_counter() - t)
         med = statistics.median(times)
         cpu = {"value": round(nrays / med / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"all {nrays} incoherent rays, binary-BVH oracle, 1 warm-up + median of {reps}"}
+               "sample": f"all {nrays} incoherent rays, binary-BVH oracle, 1 warm-up + median of {reps}",
+               "host": host_facts(threads)}
         g = gpu_hits.view(np.uint32)
         r = ref.view(np.uint32)
         mism = int(np.any(g != r, axis=1).sum())
@@ -469,12 +609,20 @@ def main():
     micro = None
     if not args.only_incoherent:
         micro = traversal_1_5m(ctx, max(2, args.steps // 2), args.warmup,
-                               args.cpu_threads or min(16, os.cpu_count() or 1), not args.no_cpu)
+                               args.cpu_threads or cpu_threads, not args.no_cpu)
         log(f"traversal 1.5M tris: {micro}")
+
+    c1 = None
+    if not args.only_incoherent:
+        c1 = c1_leg(ctx, args.cpu_threads or cpu_threads, not args.no_cpu)
+        log(f"C1: {c1}")
 
     render = relmse = batch = arena = None
     if not args.no_render and not args.only_incoherent:
         render = render_throughput(ctx, scene, gen.camera, args.width, args.height, 64, 8, 3)
+        if not args.no_cpu:
+            render["cpu_subset"] = c3_cpu_subset(gen, args.cpu_threads or cpu_threads)
+            render["gpu_vs_cpu_samples"] = round(render["msamples_s"] / render["cpu_subset"]["msamples_s"], 1)
         log(f"path tracing @10M tris: {render}")
         arena = frame_arena_sweep(ctx, scene, gen.camera, args.width, args.height, 8, 3)
         log(f"frame arena: {arena}")
@@ -504,8 +652,14 @@ def main():
         "config": {"workload": "incoherent closest-hit rays, 100 instanced displaced spheres (10.08M tris), "
                                "two-level 64-byte wide BVH", "width": args.width, "height": args.height,
                    "rays_per_step": nrays, "triangles": int(nprims)},
+        # frac prices the algorithmic bytes against the HBM peak (the bench
+        # contract); what binds the kernel is VALU issue with few useful lanes
+        # and dependent-fetch latency (the PMC summary), named in `limiter`
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "limiter": "VALU issue (few useful lanes per instruction) and dependent-fetch latency",
+                     "useful_lane_frac": valu.get("useful_lane_frac") if valu else None,
+                     "pmc_source": pmc_source,
                      "algorithmic_bytes_per_launch": int(algo_bytes), "kernel_ms": round(kernel_ms, 4)},
         "cpu_baseline": cpu,
         "extra": {"coherent_mrays_s": coherent and round(coherent, 2),
@@ -517,11 +671,24 @@ def main():
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
                   "trace_valu_issue": valu,
                   "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
-                  "traversal_1_5M": micro, "path_tracing_10M": render, "frame_arena_10M": arena, "time_to_rel_mse_0_01": relmse,
-                  "batch_64_scenes": batch},
+                  "traversal_1_5M": micro, "c1_cornell_256": c1, "path_tracing_10M": render,
+                  "frame_arena_10M": arena, "time_to_rel_mse_0_01": relmse, "batch_64_scenes": batch},
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+    # every parity check of the run must hold; a mismatch fails the run after the line is printed
+    bad = []
+    if parity and (parity["closest_hit_record_mismatches"] or parity["anyhit_mismatches"]):
+        bad.append(f"10M-triangle sets {parity}")
+    if micro and micro.get("parity") and any(micro["parity"].values()):
+        bad.append(f"traversal microbenchmark {micro['parity']}")
+    if c1 and c1.get("images_bit_exact") is False:
+        bad.append("C1 image differs from the oracle's")
+    if arena and not all(r["same_image"] for r in arena["rows"]):
+        bad.append("frame-arena images differ")
+    if bad:
+        log("PARITY FAILED: " + "; ".join(bad))
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -871,6 +871,8 @@ fnd_status fnd_pt_camera_rays(fnd_pt_ctx* ctx, const fnd_camera* camera, uint32_
             return set_error(ctx, FND_E_INVALID_ARG, "degenerate camera (%s)", why);
         JobDev j{};
         camera_basis(*camera, width, height, j);
+        j.width = width;
+        j.height = height;
         j.seed = seed;
         const uint64_t bytes = uint64_t(width) * height * sizeof(Ray);
         void* p = nullptr;
