@@ -334,6 +334,7 @@ PROTOTYPES = {
     "fnd_pt_buffer_read": (_S, [_P, _H, C.c_uint64, _P, C.c_uint64]),
     "fnd_pt_trace": (_S, [_P, _H, _P, C.c_uint32, _P, C.c_uint32]),
     "fnd_pt_trace_buffers": (_S, [_P, _H, _H, _H, C.c_uint32, C.c_uint32]),
+    "fnd_pt_trace_noalloc": (_S, [_P, _H, _P, _P, C.c_uint32, C.c_uint32]),
     "fnd_pt_framebuffer_create": (_S, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(_H)]),
     "fnd_pt_framebuffer_destroy": (_S, [_P, _H]),
     "fnd_pt_framebuffer_clear": (_S, [_P, _H]),

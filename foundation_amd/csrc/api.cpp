@@ -268,6 +268,9 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
     hipDeviceProp_t prop;
     st = check_hip(ctx, hipGetDeviceProperties(&prop, ctx->device), "hipGetDeviceProperties");
     if (st != FND_OK) return fail(st);
+    // the allocation-free trace's scratch array, resolved once with this device current
+    st = check_hip(ctx, noalloc_scratch_address(&ctx->scratch), "hipGetSymbolAddress(scratch)");
+    if (st != FND_OK) return fail(st);
     ctx->num_cus = uint32_t(prop.multiProcessorCount);
     snprintf(ctx->arch, sizeof ctx->arch, "%s", prop.gcnArchName);
     ctx->stats.num_cus = ctx->num_cus;
@@ -684,6 +687,62 @@ fnd_status fnd_pt_trace_buffers(fnd_pt_ctx* ctx, fnd_handle scene, fnd_handle ra
             return set_error(ctx, FND_E_INVALID_ARG, "buffers too small for %u rays", n);
         if (n == 0) return FND_OK;
         return trace_device(ctx, s, static_cast<const Ray*>(rb->ptr), static_cast<float4*>(hb->ptr), n, flags);
+    });
+}
+
+fnd_status fnd_pt_trace_noalloc(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, fnd_hit* hits, uint32_t n,
+                                uint32_t flags)
+{
+    if (!ctx) return FND_E_INVALID_ARG;
+    return guard(ctx, [&]() -> fnd_status {
+        Scene* s = find_scene(ctx, scene);
+        if (!s) return set_error(ctx, FND_E_NOT_FOUND, "unknown scene handle");
+        if (flags & ~(FND_TRACE_ANYHIT | FND_ASYNC))
+            return set_error(ctx, FND_E_INVALID_ARG, "unknown trace flags 0x%x (no FND_TRACE_COUNT here)", flags);
+        if (n == 0) return FND_OK;
+        if (!rays || !hits || reinterpret_cast<uintptr_t>(rays) % 16 || reinterpret_cast<uintptr_t>(hits) % 16)
+            return set_error(ctx, FND_E_INVALID_ARG, "rays and hits must be 16-byte aligned device arrays");
+        if (n > kMaxRaysPerTrace) return set_error(ctx, FND_E_INVALID_ARG, "at most %u rays per call", kMaxRaysPerTrace);
+        if (!ctx->scratch) return set_error(ctx, FND_E_DEVICE, "the scratch array has no device address");
+        TraceLaunch l{};
+        l.mode = (flags & FND_TRACE_ANYHIT) ? kModeAny : kModeClosest;
+        l.two_level = s->two_level;
+        l.multi = false;
+        l.count = false;
+        l.block = ctx->tuning.block_size;
+        l.stack_lds = ctx->tuning.stack_lds;
+        l.grid = trace_grid(ctx, l);
+        // the spill area holds (stack entries - LDS entries) 8-byte entries per
+        // thread of the grid: fewer persistent blocks when the full grid's do
+        // not fit (each thread still finishes every ray it takes)
+        const uint64_t per_thread =
+            uint64_t(s->info.stack_entries > l.stack_lds ? s->info.stack_entries - l.stack_lds : 0) * sizeof(uint2);
+        if (per_thread) {
+            const uint64_t fit = (kNoAllocScratchBytes - kNoAllocSpillOffset) / per_thread / l.block;
+            if (fit == 0)
+                return set_error(ctx, FND_E_INVALID_ARG,
+                                 "the scene's traversal stack (%u entries) needs %llu spill bytes per block; the fixed "
+                                 "scratch holds %llu",
+                                 s->info.stack_entries, (unsigned long long)(per_thread * l.block),
+                                 (unsigned long long)(kNoAllocScratchBytes - kNoAllocSpillOffset));
+            l.grid = uint32_t(std::min<uint64_t>(l.grid, fit));
+        }
+        TraceArgs a{};
+        a.scene = SceneDev{s->nodes, s->tris, s->insts, s->root, s->two_level ? 1u : 0u};
+        a.rays = reinterpret_cast<const Ray*>(rays);
+        a.hits = reinterpret_cast<float4*>(hits);
+        a.nrays = n;
+        a.counter = static_cast<uint32_t*>(ctx->scratch);
+        a.spill = reinterpret_cast<uint2*>(static_cast<char*>(ctx->scratch) + kNoAllocSpillOffset);
+        a.spill_stride = l.grid * l.block;
+        a.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
+        launch_zero_counter(a.counter, ctx->stream);
+        fnd_status st = check_hip(ctx, launch_trace(l, a, ctx->stream), "trace launch");
+        if (st != FND_OK) return st;
+        ctx->stats.last_grid = l.grid;
+        ctx->stats.rays_traced += n;
+        if (flags & FND_ASYNC) return FND_OK;
+        return check_hip(ctx, hipStreamSynchronize(ctx->stream), "trace");
     });
 }
 

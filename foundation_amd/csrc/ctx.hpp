@@ -131,6 +131,7 @@ struct fnd_pt_ctx {
     void* render_state = nullptr;              // wavefront buffers (render.cpp)
     void* cache = nullptr;                     // radiance cache (cache_api.cpp)
     void* commands = nullptr;                  // command lists and fences made by this context (command.cpp)
+    void* scratch = nullptr;                   // this device's fnd_pt_trace_noalloc scratch (noalloc.hip)
 };
 
 namespace fnd {
@@ -157,6 +158,16 @@ fnd_status clear_async(fnd_pt_ctx* ctx, Framebuffer* f);
 const char* camera_problem(const fnd_camera& c);
 // Command lists and fences still alive when the context is destroyed.
 void commands_free(fnd_pt_ctx* ctx);
+
+// Fixed device scratch of fnd_pt_trace_noalloc (noalloc.hip): 256 bytes of
+// work counter, then the traversal's stack spill area.
+constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
+constexpr uint64_t kNoAllocSpillOffset = 256;
+// Device address of the scratch array on the current device.
+hipError_t noalloc_scratch_address(void** out);
+// Zeroes the 32-bit counter with a one-thread kernel on `s` (a kernel, not a
+// memset: callers may capture the stream into a graph).
+void launch_zero_counter(uint32_t* counter, hipStream_t s);
 uint32_t trace_grid(const fnd_pt_ctx* ctx, const TraceLaunch& l);
 void resolve_timings(fnd_pt_ctx* ctx);
 fnd_status timing_pair(fnd_pt_ctx* ctx, hipEvent_t* start, hipEvent_t* end);

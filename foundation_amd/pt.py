@@ -433,6 +433,10 @@ class Context:
     def trace_buffers(self, scene, ray_buf, hit_buf, n, flags=0):
         self._ck(self.lib.fnd_pt_trace_buffers(self.h, scene, ray_buf, hit_buf, n, flags))
 
+    def trace_noalloc(self, scene, rays_ptr, hits_ptr, n, flags=0):
+        """fnd_pt_trace_noalloc over caller device memory (device pointers)."""
+        self._ck(self.lib.fnd_pt_trace_noalloc(self.h, scene, rays_ptr, hits_ptr, n, flags))
+
     def wait(self):
         self._ck(self.lib.fnd_pt_wait(self.h))
 

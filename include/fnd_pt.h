@@ -280,6 +280,22 @@ fnd_status fnd_pt_trace(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, 
 fnd_status fnd_pt_trace_buffers(fnd_pt_ctx* ctx, fnd_handle scene, fnd_handle rays, fnd_handle hits,
                                 uint32_t num_rays, uint32_t flags);
 
+/* Allocation-free variant of fnd_pt_trace_buffers.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * rays and hits are device pointers of the caller's (16-byte aligned, n
+ * fnd_ray / fnd_hit).  The call allocates nothing: the traversal's scratch
+ * (its work counter and stack spill area) is a fixed 16 MiB device array of
+ * the library, one per GPU, held until the process ends.  Every call on a
+ * GPU shares that array, so calls that overlap on one GPU (on different
+ * streams or contexts) race; calls on one context's stream are ordered.  When
+ * the scene's stack spill for the full persistent grid does not fit, fewer
+ * persistent blocks run (same records); a scene whose spill for one block
+ * does not fit is refused with FND_E_INVALID_ARG.  Flags: FND_TRACE_ANYHIT,
+ * FND_ASYNC (no FND_TRACE_COUNT, no per-launch timing). */
+fnd_status fnd_pt_trace_noalloc(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, fnd_hit* hits,
+                                uint32_t num_rays, uint32_t flags);
+
 /* ---------------------------------------------------------------- rendering */
 
 fnd_status fnd_pt_framebuffer_create(fnd_pt_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags,
