@@ -258,7 +258,7 @@ def batch64(ctx, count, spp, depth, cache_path):
         ctx.set_tuning(block_size=0, stack_lds=0, waves_per_cu=0, refill=0, queue_capacity=0, tile=0)
 
 
-def frames_only(frames, width, height, depth, rr):
+def frames_only(frames, width, height, depth, rr, grid_gradient=0):
     """The frames of the time-to-relMSE leg without its reference: `frames`
     plain frames, then `frames` frames with the radiance cache, 1 spp each;
     prints the mean device ms per frame of each (for kernel-trace profiles)."""
@@ -274,6 +274,7 @@ def frames_only(frames, width, height, depth, rr):
         if mode == "cache":
             d = A.CacheDesc()
             ctx.lib.fnd_pt_cache_default_desc(C.byref(d))
+            d.grid_gradient = grid_gradient
             ctx._ck(ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d)))
             flags = A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE
         ctx.framebuffer_clear(fb)
@@ -287,7 +288,7 @@ def frames_only(frames, width, height, depth, rr):
     ctx.close()
 
 
-def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
+def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, grid_gradient=0):
     """Progressive 1 spp/frame rendering of the displaced-sphere scene, plain and
     with the radiance cache (trained every frame on the frame's own paths),
     until relMSE <= target against an FP64 GPU reference of ref_spp samples.
@@ -299,7 +300,8 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
     gen = pt.GeneratedScene(A.GEN_DISPLACED_SPHERE)
     scene = ctx.scene_create(gen.desc)
     out = {"scene": "displaced icosphere (1.31M tris) + 200k soup, 4 quad lights, 50/50 Lambert/GGX",
-           "width": width, "height": height, "max_depth": depth, "rr_depth": rr, "target": target}
+           "width": width, "height": height, "max_depth": depth, "rr_depth": rr, "target": target,
+           "cache_grid_gradient": "bf16" if grid_gradient == 0 else "fp32"}
     try:
         ref_fb = ctx.framebuffer_create(width, height, fp64=True)
         t = time.perf_counter()
@@ -316,6 +318,7 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames):
             if mode == "cache":
                 d = A.CacheDesc()
                 ctx.lib.fnd_pt_cache_default_desc(C.byref(d))
+                d.grid_gradient = grid_gradient
                 ctx._ck(ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d)))
                 flags = A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE
             fb = ctx.framebuffer_create(width, height)
@@ -466,6 +469,10 @@ def main():
     ap.add_argument("--relmse-width", type=int, default=1920)
     ap.add_argument("--relmse-height", type=int, default=1080)
     ap.add_argument("--relmse-ref-spp", type=int, default=4096)
+    ap.add_argument("--cache-grid-gradient", choices=("bf16", "fp32"), default="bf16",
+                    help="how the radiance cache sums its hash-grid gradient (FP32 or packed BF16 atomics)")
+    ap.add_argument("--relmse-only", action="store_true",
+                    help="run only the time-to-relMSE leg (plain vs cache) and print its result")
     ap.add_argument("--pmc-summary", default=os.path.join(ROOT, "profiles", "trace_incoherent_10M_pmc.json"))
     ap.add_argument("--only-incoherent", action="store_true",
                     help="profiling runs: time only the incoherent set (no coherent/shadow sets, no CPU leg)")
@@ -494,8 +501,17 @@ def main():
     except (AttributeError, OSError):
         cpu_threads = min(CPU_SHARE, os.cpu_count() or 1)
 
+    grid_gradient = 0 if args.cache_grid_gradient == "bf16" else 1
     if args.frames_only:
-        frames_only(args.frames_only, args.relmse_width, args.relmse_height, 8, 3)
+        frames_only(args.frames_only, args.relmse_width, args.relmse_height, 8, 3, grid_gradient)
+        return
+    if args.relmse_only:
+        from foundation_amd import pt
+        ctx = pt.Context(0)
+        r = time_to_rel_mse(ctx, args.relmse_width, args.relmse_height, 8, 3, 0.01, args.relmse_ref_spp, 1500,
+                            grid_gradient)
+        print(json.dumps(r), flush=True)
+        ctx.close()
         return
 
     if args.calibrate_copy_mib:
@@ -628,7 +644,8 @@ def main():
         log(f"frame arena: {arena}")
         ctx.scene_destroy(scene)  # free the 10M-triangle scene before the next one
         scene = None
-        relmse = time_to_rel_mse(ctx, args.relmse_width, args.relmse_height, 8, 3, 0.01, args.relmse_ref_spp, 1500)
+        relmse = time_to_rel_mse(ctx, args.relmse_width, args.relmse_height, 8, 3, 0.01, args.relmse_ref_spp, 1500,
+                                 grid_gradient)
     if not args.no_batch and not args.only_incoherent:
         if scene is not None:
             ctx.scene_destroy(scene)

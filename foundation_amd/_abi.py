@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-ABI_VERSION = 6  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
+ABI_VERSION = 7  # FND_PT_ABI_VERSION of include/fnd_pt.h these declarations follow
 LIB_PATH = os.path.join(_HERE, "libfnd_pt.so")
 
 fnd_handle = C.c_uint64
@@ -260,7 +260,11 @@ class CacheDesc(C.Structure):
         ("seed", C.c_uint64),
         ("bounds_lo", C.c_float * 3),
         ("bounds_hi", C.c_float * 3),
+        ("grid_gradient", C.c_uint32),
     ]
+
+
+CACHE_GRAD_BF16, CACHE_GRAD_FP32 = 0, 1
 
 
 class HostSceneView(C.Structure):

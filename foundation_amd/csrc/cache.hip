@@ -270,6 +270,9 @@ __global__ __launch_bounds__(256) void k_cache_loss(const float4* pred, const fl
     if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
+// F32: the grid gradient is summed with FP32 atomics (two per corner),
+// otherwise with one packed bf16 atomic per corner.
+template <bool F32>
 __global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const CacheQuery* q, uint32_t qstride, uint32_t n,
                                                         const uint16_t* act, uint16_t* dz, uint32_t stride,
                                                         uint32_t* grad_grid)
@@ -370,16 +373,26 @@ __global__ __launch_bounds__(256) void k_cache_backward(CacheView v, const Cache
                     uint32_t idx[8];
                     float wc[8];
                     level_corners(x, y, z, v.res[l], (v.dense_mask >> l) & 1u, idx, wc);
-                    // one packed bf16 atomic per corner carries both features
-                    // (global_atomic_pk_add_bf16): half the atomics and half the
-                    // table bytes of two FP32 adds
-                    auto* gl = (__attribute__((address_space(1))) s2*)(grad_grid + size_t(l) * kCacheT);
+                    if (F32) {
+                        // FP32 sums: two global_atomic_add_f32 per corner
+                        auto* gf = (__attribute__((address_space(1))) float*)(grad_grid) + size_t(l) * kCacheT * 2;
 #pragma unroll
-                    for (int c = 0; c < 8; ++c) {
-                        s2 val;
-                        val[0] = short(bf16_bits(wc[c] * g0));
-                        val[1] = short(bf16_bits(wc[c] * g1));
-                        (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(gl + idx[c], val);
+                        for (int c = 0; c < 8; ++c) {
+                            (void)__builtin_amdgcn_global_atomic_fadd_f32(gf + 2 * size_t(idx[c]), wc[c] * g0);
+                            (void)__builtin_amdgcn_global_atomic_fadd_f32(gf + 2 * size_t(idx[c]) + 1, wc[c] * g1);
+                        }
+                    } else {
+                        // one packed bf16 atomic per corner carries both features
+                        // (global_atomic_pk_add_bf16): half the atomics and half the
+                        // table bytes of two FP32 adds
+                        auto* gl = (__attribute__((address_space(1))) s2*)(grad_grid + size_t(l) * kCacheT);
+#pragma unroll
+                        for (int c = 0; c < 8; ++c) {
+                            s2 val;
+                            val[0] = short(bf16_bits(wc[c] * g0));
+                            val[1] = short(bf16_bits(wc[c] * g1));
+                            (void)__builtin_amdgcn_global_atomic_fadd_v2bf16(gl + idx[c], val);
+                        }
                     }
                 }
             }
@@ -423,26 +436,32 @@ __global__ __launch_bounds__(256) void k_cache_dw(const uint16_t* act, const uin
 }
 
 // Adam (Kingma & Ba 2015) with bias-correction factors bc1 = 1 - b1^t,
-// bc2 = 1 - b2^t.
-// GRID (the hash grid): gradients arrive as one packed bf16 pair per table
-// entry; an entry whose two gradients are both 0 is left alone, moments
-// included (Mueller et al. 2022, section 4: updates of zero-gradient table
-// entries are skipped); every gradient read is cleared, so the next step
-// starts from zero without a separate pass; the FP16 copy the forward pass
-// reads is refreshed.
-template <bool GRID>
+// bc2 = 1 - b2^t.  MODE 0: plain FP32 gradients (the MLP).  MODE 1 and 2
+// (the hash grid): gradients arrive per table entry as one packed bf16 pair
+// (1) or two FP32 values (2); an entry whose two gradients are both 0 is left
+// alone, moments included (Mueller et al. 2022, section 4: updates of
+// zero-gradient table entries are skipped); every gradient read is cleared,
+// so the next step starts from zero without a separate pass; the FP16 copy
+// the forward pass reads is refreshed.
+template <int MODE>
 __global__ __launch_bounds__(256) void k_adam(float* p, void* gr, float* m, float* v, uint64_t n, float lr,
                                               float b1, float b2, float eps, float bc1, float bc2, uint32_t* out_half2)
 {
+    constexpr bool GRID = MODE != 0;
     const uint64_t pairs = n / 2;
     for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < pairs; i += uint64_t(gridDim.x) * 256u) {
         float2 gg;
-        if (GRID) {
+        if (MODE == 1) {
             uint32_t* gb = static_cast<uint32_t*>(gr);
             const uint32_t w = gb[i];
             if ((w & 0x7FFF7FFFu) == 0u) continue;  // both +-0: untouched this step
             gb[i] = 0u;
             gg = make_float2(__builtin_bit_cast(float, w << 16), __builtin_bit_cast(float, w & 0xFFFF0000u));
+        } else if (MODE == 2) {
+            float2* gf = static_cast<float2*>(gr);
+            gg = gf[i];
+            if (gg.x == 0.0f && gg.y == 0.0f) continue;  // untouched this step
+            gf[i] = make_float2(0.0f, 0.0f);
         } else {
             gg = static_cast<const float2*>(gr)[i];
         }
@@ -533,11 +552,16 @@ void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstrid
 }
 
 void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint16_t* act,
-                           uint16_t* dz, uint32_t stride, uint32_t* grad_grid, hipStream_t s)
+                           uint16_t* dz, uint32_t stride, uint32_t* grad_grid, bool fp32, hipStream_t s)
 {
     // one 16-sample tile per wave: the scatter's atomics want every CU
     const uint32_t grid = std::max(1u, std::min<uint32_t>(4096u, (n + 63u) / 64u));
-    hipLaunchKernelGGL(k_cache_backward, dim3(grid), dim3(256), 0, s, v, q, qstride, n, act, dz, stride, grad_grid);
+    if (fp32)
+        hipLaunchKernelGGL(k_cache_backward<true>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, act, dz, stride,
+                           grad_grid);
+    else
+        hipLaunchKernelGGL(k_cache_backward<false>, dim3(grid), dim3(256), 0, s, v, q, qstride, n, act, dz, stride,
+                           grad_grid);
 }
 
 void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32_t stride, float* grad_mlp,
@@ -557,15 +581,19 @@ static dim3 adam_grid(uint64_t n)
 void launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
                  float bc1, float bc2, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adam<false>, adam_grid(n), dim3(256), 0, s, p, const_cast<float*>(g), m, v, n, lr, b1, b2,
+    hipLaunchKernelGGL(k_adam<0>, adam_grid(n), dim3(256), 0, s, p, const_cast<float*>(g), m, v, n, lr, b1, b2,
                        eps, bc1, bc2, nullptr);
 }
 
-void launch_adam_grid(float* p, uint32_t* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
-                      float bc1, float bc2, uint32_t* out_half2, hipStream_t s)
+void launch_adam_grid(float* p, uint32_t* g, bool fp32, float* m, float* v, uint64_t n, float lr, float b1, float b2,
+                      float eps, float bc1, float bc2, uint32_t* out_half2, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adam<true>, adam_grid(n), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
-                       out_half2);
+    if (fp32)
+        hipLaunchKernelGGL(k_adam<2>, adam_grid(n), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
+                           out_half2);
+    else
+        hipLaunchKernelGGL(k_adam<1>, adam_grid(n), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, bc1, bc2,
+                           out_half2);
 }
 
 void launch_arrange(const float* w, _Float16* wfwd, uint16_t* wbwd, hipStream_t s)

@@ -73,7 +73,8 @@ struct CacheTrainBufs {
     uint16_t* dz;               // [kDzRows][cap] bf16
     float4* pred;               // [cap]
     float* grad_mlp;            // kMlpParams
-    uint32_t* grad_grid;        // kGridParams / 2: the two features of an entry as packed bf16
+    uint32_t* grad_grid;        // kGridParams words: per entry two FP32 features, or (bf16 form, first
+                                // half) the two features as packed bf16
     float* loss;                // 1
     uint32_t cap;               // samples the buffers hold (multiple of 32)
 };
@@ -86,7 +87,7 @@ void launch_cache_forward(const CacheView& v, const CacheQuery* q, uint32_t qstr
 void launch_cache_loss(const float4* pred, const float4* target, uint32_t tstride, uint32_t n, uint16_t* dz,
                        uint32_t stride, float* loss, hipStream_t s);
 void launch_cache_backward(const CacheView& v, const CacheQuery* q, uint32_t qstride, uint32_t n, const uint16_t* act,
-                           uint16_t* dz, uint32_t stride, uint32_t* grad_grid, hipStream_t s);
+                           uint16_t* dz, uint32_t stride, uint32_t* grad_grid, bool fp32, hipStream_t s);
 void launch_cache_dw(const uint16_t* act, const uint16_t* dz, uint32_t n, uint32_t stride, float* grad_mlp,
                      hipStream_t s);
 // lazy: skip pairs whose gradients are both 0 and clear the gradients read
@@ -96,8 +97,8 @@ void launch_adam(float* p, const float* g, float* m, float* v, uint64_t n, float
 // Lazy Adam over the hash grid: gradients are packed bf16 pairs; pairs with a
 // zero gradient keep their moments, the others are updated, their gradient
 // cleared and their FP16 copy refreshed into out_half2.
-void launch_adam_grid(float* p, uint32_t* g, float* m, float* v, uint64_t n, float lr, float b1, float b2, float eps,
-                      float bc1, float bc2, uint32_t* out_half2, hipStream_t s);
+void launch_adam_grid(float* p, uint32_t* g, bool fp32, float* m, float* v, uint64_t n, float lr, float b1, float b2,
+                      float eps, float bc1, float bc2, uint32_t* out_half2, hipStream_t s);
 void launch_arrange(const float* w, _Float16* wfwd, uint16_t* wbwd, hipStream_t s);
 void launch_zero(float* p, uint64_t n, hipStream_t s);
 

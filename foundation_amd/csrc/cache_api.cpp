@@ -45,9 +45,10 @@ fnd_status validate(fnd_pt_ctx* ctx, const fnd_cache_desc& d)
                          "5 x 64 hidden");
     if (!(d.learning_rate > 0.0f) || d.train_paths == 0 || d.train_paths > kCacheTrainMaxPaths || d.train_steps == 0 ||
         d.train_steps > 64 || d.train_depth == 0 || d.train_depth > kCacheMaxRecords || d.train_batch < 256 ||
-        d.train_batch > kCacheBatch || !(d.glossy_min_roughness >= 0.0f))
+        d.train_batch > kCacheBatch || !(d.glossy_min_roughness >= 0.0f) || d.grid_gradient > FND_CACHE_GRAD_FP32)
         return set_error(ctx, FND_E_INVALID_ARG, "cache: learning_rate > 0, train_paths in 1..65536, "
-                                                 "train_steps in 1..64, train_depth in 1..16, train_batch in 256..65536");
+                                                 "train_steps in 1..64, train_depth in 1..16, train_batch in 256..65536, "
+                                                 "grid_gradient FND_CACHE_GRAD_BF16 or FND_CACHE_GRAD_FP32");
     return FND_OK;
 }
 
@@ -89,13 +90,14 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
     launch_zero(cs->tb.grad_mlp, kMlpParams, s);
     // the lazy grid update clears what it reads; only a step without update
     // (gradients for the caller) leaves the grid gradient to be cleared here
-    if (cs->grid_grad_dirty) launch_zero(reinterpret_cast<float*>(cs->tb.grad_grid), kGridParams / 2, s);
+    const bool fp32 = cs->desc.grid_gradient == FND_CACHE_GRAD_FP32;
+    if (cs->grid_grad_dirty) launch_zero(reinterpret_cast<float*>(cs->tb.grad_grid), fp32 ? kGridParams : kGridParams / 2, s);
     cs->grid_grad_dirty = !apply;
     launch_zero(cs->tb.loss, 1, s);
     const uint32_t grid = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (n + 127u) / 128u));
     launch_cache_forward(cs->view, q, sample_stride, n, nullptr, cs->tb.pred, cs->tb.act, cs->tb.cap, grid, s);
     launch_cache_loss(cs->tb.pred, target, sample_stride, n, cs->tb.dz, cs->tb.cap, cs->tb.loss, s);
-    launch_cache_backward(cs->view, q, sample_stride, n, cs->tb.act, cs->tb.dz, cs->tb.cap, cs->tb.grad_grid, s);
+    launch_cache_backward(cs->view, q, sample_stride, n, cs->tb.act, cs->tb.dz, cs->tb.cap, cs->tb.grad_grid, fp32, s);
     launch_cache_dw(cs->tb.act, cs->tb.dz, n, cs->tb.cap, cs->tb.grad_mlp, s);
     fnd_status st = check_hip(ctx, hipGetLastError(), "cache step");
     if (st != FND_OK) return st;
@@ -105,8 +107,8 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
         const float bc1 = 1.0f - std::pow(b1, float(cs->step)), bc2 = 1.0f - std::pow(b2, float(cs->step));
         launch_adam(cs->w_master, cs->tb.grad_mlp, cs->w_m, cs->w_v, kMlpParams, cs->desc.learning_rate, b1, b2, eps,
                     bc1, bc2, s);
-        launch_adam_grid(cs->g_master, cs->tb.grad_grid, cs->g_m, cs->g_v, kGridParams, cs->desc.learning_rate, b1,
-                         b2, eps, bc1, bc2, cs->grid16, s);
+        launch_adam_grid(cs->g_master, cs->tb.grad_grid, fp32, cs->g_m, cs->g_v, kGridParams, cs->desc.learning_rate,
+                         b1, b2, eps, bc1, bc2, cs->grid16, s);
         launch_arrange(cs->w_master, cs->wfwd, cs->wbwd, s);
         st = check_hip(ctx, hipGetLastError(), "cache update");
         if (st != FND_OK) return st;
@@ -116,14 +118,16 @@ fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, cons
         if (st == FND_OK && dmlp_host)
             st = check_hip(ctx, hipMemcpyAsync(dmlp_host, cs->tb.grad_mlp, kMlpParams * 4, hipMemcpyDeviceToHost, s),
                            "read gradients");
-        // the grid gradient is bf16: copied into the front half of the caller's
-        // array, then widened in place from the back (element i's source bytes
-        // [2i, 2i+2) lie at or after its destination's start and are read first)
+        // an FP32 grid gradient is copied as it is; a bf16 one into the front
+        // half of the caller's array, then widened in place from the back
+        // (element i's source bytes [2i, 2i+2) lie at or after its
+        // destination's start and are read first)
         if (st == FND_OK && dgrid_host)
-            st = check_hip(ctx, hipMemcpyAsync(dgrid_host, cs->tb.grad_grid, kGridParams * 2, hipMemcpyDeviceToHost, s),
+            st = check_hip(ctx, hipMemcpyAsync(dgrid_host, cs->tb.grad_grid, kGridParams * (fp32 ? 4 : 2),
+                                               hipMemcpyDeviceToHost, s),
                            "read gradients");
         if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(s), "cache step");
-        if (st == FND_OK && dgrid_host) {
+        if (st == FND_OK && dgrid_host && !fp32) {
             auto* bytes = reinterpret_cast<unsigned char*>(dgrid_host);
             for (uint64_t i = kGridParams; i-- > 0;) {
                 uint16_t h;
@@ -234,7 +238,8 @@ fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* d)
         if ((st = calloc_dev(ctx, cs, &tb.dz, uint64_t(kDzRows) * tb.cap)) != FND_OK) return fail(st);
         if ((st = calloc_dev(ctx, cs, &tb.pred, tb.cap)) != FND_OK) return fail(st);
         if ((st = calloc_dev(ctx, cs, &tb.grad_mlp, kMlpParams)) != FND_OK) return fail(st);
-        if ((st = calloc_dev(ctx, cs, &tb.grad_grid, kGridParams / 2)) != FND_OK) return fail(st);
+        // room for the FP32 form (two floats per entry); the bf16 form uses the first half
+        if ((st = calloc_dev(ctx, cs, &tb.grad_grid, kGridParams)) != FND_OK) return fail(st);
         if ((st = calloc_dev(ctx, cs, &tb.loss, 1)) != FND_OK) return fail(st);
         if ((st = calloc_dev(ctx, cs, &cs->io_query, tb.cap)) != FND_OK) return fail(st);
         if ((st = calloc_dev(ctx, cs, &cs->io_target, tb.cap)) != FND_OK) return fail(st);

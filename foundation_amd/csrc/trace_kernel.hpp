@@ -148,51 +148,41 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs args)
                     const float4 ra = rp[0];
                     const float4 rb = rp[1];
                     // A non-finite origin or direction, or a NaN range bound, is
-                    // answered as a miss without a walk (NaN slab reciprocals
-                    // would pass every box test and visit the whole tree).  The
-                    // lane stays idle and takes a ray at the next refill.
-                    const bool refused = !(__builtin_isfinite(ra.x) && __builtin_isfinite(ra.y) &&
-                                           __builtin_isfinite(ra.z) && __builtin_isfinite(rb.x) &&
-                                           __builtin_isfinite(rb.y) && __builtin_isfinite(rb.z)) ||
-                                         __builtin_isnan(ra.w) || __builtin_isnan(rb.w);
-                    if (refused) {
-                        ((MIXED && second) ? args.hits2 : args.hits)[ray_idx] =
-                            make_float4(rb.w, __uint_as_float(0xFFFFFFFFu),
-                                        INTERNAL ? __uint_as_float(0xFFFFFFFFu) : 0.f, 0.f);
-                    } else {
-                        if (MULTI) sc = args.scenes[(second ? args.ray_scene2 : args.ray_scene)[ray_idx]];
-                        // canonical once here, so the slab tests' fmaxf/fminf need not
-                        // quiet them on every node
-                        tmin = __builtin_canonicalizef(ra.w);
-                        tmax = __builtin_canonicalizef(rb.w);
-                        ray_setup(r, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z);
-                        if (TWO) {
-                            wox = ra.x; woy = ra.y; woz = ra.z;
-                            wdx = rb.x; wdy = rb.y; wdz = rb.z;
-                            wix = r.ix; wiy = r.iy; wiz = r.iz;
-                        }
-                        tbest = tmax;
-                        best = 0xFFFFFFFFu;
-                        best_aux = 0xFFFFFFFFu;
-                        bV = bW = 0.f;
-                        bdet = 1.f;
-                        sp = 0;
-                        cur = sc.root;
-                        id_base = 0;
-                        inst_slot = 0xFFFFFFFFu;
-                        active = true;
+                    // refused: its range is made empty (tmin = +inf, best bound
+                    // -inf), so the root's slab tests fail (fmaxf/fminf drop the
+                    // NaN terms) and the ray finishes as a miss (t = tmax as
+                    // given) after one node step, without a walk (NaN slab
+                    // reciprocals alone would pass every box test).
+                    const float fin = ((ra.x - ra.x) + (ra.y - ra.y)) + ((ra.z - ra.z) + (rb.x - rb.x)) +
+                                      ((rb.y - rb.y) + (rb.z - rb.z));
+                    const bool refused = fin != fin || ra.w != ra.w || rb.w != rb.w;
+                    if (MULTI) sc = args.scenes[(second ? args.ray_scene2 : args.ray_scene)[ray_idx]];
+                    // canonical once here, so the slab tests' fmaxf/fminf need not
+                    // quiet them on every node
+                    tmin = refused ? __builtin_inff() : __builtin_canonicalizef(ra.w);
+                    tmax = refused ? rb.w : __builtin_canonicalizef(rb.w);
+                    ray_setup(r, ra.x, ra.y, ra.z, rb.x, rb.y, rb.z);
+                    if (TWO) {
+                        wox = ra.x; woy = ra.y; woz = ra.z;
+                        wdx = rb.x; wdy = rb.y; wdz = rb.z;
+                        wix = r.ix; wiy = r.iy; wiz = r.iz;
                     }
+                    tbest = refused ? -__builtin_inff() : tmax;
+                    best = 0xFFFFFFFFu;
+                    best_aux = 0xFFFFFFFFu;
+                    bV = bW = 0.f;
+                    bdet = 1.f;
+                    sp = 0;
+                    cur = sc.root;
+                    id_base = 0;
+                    inst_slot = 0xFFFFFFFFu;
+                    active = true;
                 } else {
                     retired = true;
                 }
             }
         }
-        if (__ballot(active) == 0) {
-            // every lane is idle: done once all are retired (a refused ray
-            // leaves its lane idle, not retired, to fetch again)
-            if (__ballot(!retired) == 0) break;
-            continue;
-        }
+        if (__ballot(active) == 0) break;
         if (!active) continue;
 
         // Work selection: a lane's next unit is a node, a triangle or an

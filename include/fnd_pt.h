@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define FND_PT_ABI_VERSION 6u
+#define FND_PT_ABI_VERSION 7u
 
 typedef uint64_t fnd_handle;
 #define FND_INVALID_HANDLE (~(uint64_t)0)
@@ -577,7 +577,15 @@ typedef struct fnd_cache_desc {
     uint64_t seed;              /* initialisation */
     float bounds_lo[3];         /* region the grid covers; lo == hi takes the bounds of the */
     float bounds_hi[3];         /* first scene rendered with the cache */
+    uint32_t grid_gradient;     /* how the hash grid's gradient is summed over a batch:
+                                   FND_CACHE_GRAD_BF16 (packed BF16 atomics, the default) or
+                                   FND_CACHE_GRAD_FP32 (FP32 atomics: twice the atomics, about
+                                   0.26 ms more per 1080p training frame on MI355X, and no
+                                   measurable change in the gradient's error or in convergence) */
 } fnd_cache_desc;
+
+#define FND_CACHE_GRAD_BF16 0u
+#define FND_CACHE_GRAD_FP32 1u
 
 fnd_status fnd_pt_cache_default_desc(fnd_cache_desc* out);
 fnd_status fnd_pt_cache_configure(fnd_pt_ctx* ctx, const fnd_cache_desc* desc);

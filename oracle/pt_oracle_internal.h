@@ -80,7 +80,8 @@ typedef struct v3f_in { float x, y, z; } v3f_in;
 
 /* A ray the traversal refuses: a non-finite origin or direction component,
  * or a NaN range bound.  It is answered as a miss (t = tmax as given, no
- * primitive, u = v = 0) without visiting the tree, as the device does. */
+ * primitive, u = v = 0) without a walk, as the device does (the device's one
+ * root-node test with an empty range is counted by the wide walk). */
 static inline int orc_ray_rejected(const float* ray8)
 {
     for (int k = 0; k < 8; ++k) {

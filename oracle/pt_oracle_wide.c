@@ -139,8 +139,9 @@ static void walk(const wscene* s, const float* ray8, int anyhit, uint32_t* stk_e
                  uint32_t* cnt3, uint32_t stack_cap)
 {
     if (orc_ray_rejected(ray8)) {
+        /* the device tests the root node's slots with an empty range, then stops */
         orc_miss_record(ray8, hit4);
-        if (cnt3) cnt3[0] = cnt3[1] = cnt3[2] = 0;
+        if (cnt3) { cnt3[0] = 1; cnt3[1] = 0; cnt3[2] = 0; }
         return;
     }
     uint32_t max_sp = 0;

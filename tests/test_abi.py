@@ -33,7 +33,7 @@ def test_bindings_cover_the_header():
 
 
 def test_abi_version_and_status_strings(lib):
-    assert lib.fnd_pt_abi_version() == A.ABI_VERSION == 6
+    assert lib.fnd_pt_abi_version() == A.ABI_VERSION == 7
     for s in range(8):
         assert lib.fnd_pt_status_string(s)
     assert lib.fnd_pt_status_string(99) == b"unknown status"

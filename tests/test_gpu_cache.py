@@ -1,10 +1,15 @@
 """Radiance cache on the GPU vs the FP64 CPU oracle, and what it gains.
 
-Tolerances: the GPU forward stores weights, grid values and activations in
-FP16 (the oracle rounds the same quantities with round_fp16=True); the
-backward runs in BF16 (8-bit mantissa) with FP32 accumulation, and the hash
-grid's gradient is accumulated by packed BF16 atomics, so gradients are
-compared as whole vectors per layer, |g_gpu - g_ref| / |g_ref| < 5e-2.
+Tolerances, about twice the agreement measured on the MI355X: the GPU
+forward stores weights, grid values and activations in FP16 (the oracle
+rounds the same quantities with round_fp16=True): |y_gpu - y_ref| <= 6e-4 +
+1e-3 |y_ref| (measured 3.1e-4 at most).  The backward multiplies in BF16
+(8-bit mantissa) with FP32 accumulation, so gradients are compared as whole
+vectors per layer, |g_gpu - g_ref| / |g_ref| < 1e-2 (measured <= 0.4%).  The
+hash grid's gradient is < 2.5e-2 off whether its sums use packed BF16
+atomics (the default, measured 1.85%) or FP32 atomics (FND_CACHE_GRAD_FP32,
+measured 1.84%): the summation is not what limits it, the BF16 operands of
+the backward are.
 """
 import ctypes as C
 
@@ -76,24 +81,26 @@ def test_forward_matches_oracle(ctx, oracle, params):
     _, ref, _, _ = oracle.cache_loss_grad(mlp, grid, q, t, [0, 0, 0], 1.0, round_fp16=True, want_grid=False)
     err = np.abs(got - ref)
     print(f"forward max err {err.max():.3e}, ref scale {np.abs(ref).mean():.3e}", flush=True)
-    assert np.all(err <= 2e-2 * (np.abs(ref) + 0.1))
+    assert np.all(err <= 6e-4 + 1e-3 * np.abs(ref)), err.max()
 
 
-def test_gradients_match_oracle(ctx, oracle, params):
+@pytest.mark.parametrize("mode,grid_tol", [(A.CACHE_GRAD_BF16, 2.5e-2), (A.CACHE_GRAD_FP32, 2.5e-2)])
+def test_gradients_match_oracle(ctx, oracle, params, mode, grid_tol):
     mlp, grid, q, t = params
-    configure(ctx)
+    configure(ctx, grid_gradient=mode)
     set_params(ctx, mlp, grid)
     loss, dm, dg = gradients(ctx, q, t)
     rloss, _, rdm, rdg = oracle.cache_loss_grad(mlp, grid, q, t, [0, 0, 0], 1.0, round_fp16=True)
     print(f"loss gpu {loss:.6g} oracle {rloss:.6g}", flush=True)
-    assert abs(loss - rloss) <= 1e-2 * abs(rloss)
+    assert abs(loss - rloss) <= 5e-3 * abs(rloss)
     for a, b in LAYERS:
         rel = np.linalg.norm(dm[a:b] - rdm[a:b]) / np.linalg.norm(rdm[a:b])
         print(f"layer [{a}:{b}) relative gradient error {rel:.3e}", flush=True)
-        assert rel < 5e-2, (a, b, rel)
+        assert rel < 1e-2, (a, b, rel)
     rel = np.linalg.norm(dg - rdg) / np.linalg.norm(rdg)
-    print(f"grid relative gradient error {rel:.3e}", flush=True)
-    assert rel < 5e-2
+    print(f"grid relative gradient error ({'fp32' if mode == A.CACHE_GRAD_FP32 else 'bf16'} sums) {rel:.3e}",
+          flush=True)
+    assert rel < grid_tol
     # no gradient outside the touched entries
     assert np.all(dg[rdg == 0] == 0)
 

@@ -339,8 +339,9 @@ def test_wide_walk_equals_binary_walk(oracle):
 def test_refused_rays_are_misses_without_a_walk(oracle):
     """Rays with a non-finite origin or direction component, or a NaN range
     bound, get the miss record (t = tmax as given, no primitive, u = v = 0)
-    from the binary and the wide walks, with no node visited; the same rays
-    made finite are ordinary rays that do visit nodes."""
+    from the binary and the wide walks; the wide walk counts the one root
+    test the device makes with an empty range, no triangle and no instance;
+    the same rays made finite are ordinary rays that visit more."""
     gen = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
     enc = _encode(gen)
     binary = oracle.scene_from_generated(gen)
@@ -358,9 +359,9 @@ def test_refused_rays_are_misses_without_a_walk(oracle):
             assert np.all(h[:, 1].view(np.uint32) == 0xFFFFFFFF)
             assert np.array_equal(h[:, 0].view(np.uint32), rays[:, 7].view(np.uint32))
             assert np.all(h[:, 2:] == 0)
-        assert np.all(cnt == 0)
-    _, cnt = oracle.trace_wide(enc, R.primary_rays(gen.camera, 8, 8)[:6])
-    assert np.all(cnt[:, 0] > 0)
+        assert np.all(cnt[:, 0] == 1) and np.all(cnt[:, 1:] == 0)
+    _, cnt = oracle.trace_wide(enc, R.primary_rays(gen.camera, 8, 8)[24:30])
+    assert np.all(cnt[:, 0] >= 1) and cnt[:, 0].sum() > 2 * len(cnt)
 
 
 def _scene_hash(gen):
