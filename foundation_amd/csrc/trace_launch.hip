@@ -27,9 +27,9 @@ template <int B, int S>
 static const void* kernel_cfg(const TraceLaunch& l)
 {
     if (l.multi) {
-        if (l.mode == kModeClosestInternal) return reinterpret_cast<const void*>(&k_trace<kModeClosestInternal, true, true, false, S, B>);
-        if (l.mode == kModeAnyInternal) return reinterpret_cast<const void*>(&k_trace<kModeAnyInternal, true, true, false, S, B>);
-        if (l.mode == kModeMixedInternal) return reinterpret_cast<const void*>(&k_trace<kModeMixedInternal, true, true, false, S, B>);
+        if (l.mode == kModeClosestInternal) return reinterpret_cast<const void*>(&k_trace_render<kModeClosestInternal, true, true, false, S, B>);
+        if (l.mode == kModeAnyInternal) return reinterpret_cast<const void*>(&k_trace_render<kModeAnyInternal, true, true, false, S, B>);
+        if (l.mode == kModeMixedInternal) return reinterpret_cast<const void*>(&k_trace_render<kModeMixedInternal, true, true, false, S, B>);
         return nullptr;
     }
     if (l.count) return nullptr;  // counting variants exist for the default configuration only
