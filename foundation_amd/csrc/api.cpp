@@ -188,9 +188,10 @@ void default_tuning(fnd_tuning& t)
     t.block_size = 128;
     t.stack_lds = 12;
     t.waves_per_cu = 24;
-    // refill 48: 2.5% more incoherent rays/s at 10.08M triangles than 40, with
-    // the fewer node visits per ray of tight instance boxes; renders unchanged
-    t.refill = 48;
+    // refill 16 with the sharded work counter: incoherent rays at 10.08M
+    // triangles 1875-1975 Mrays/s over refills 8-24 (16 best), against
+    // 1760-1795 at 48 with one counter address
+    t.refill = 16;
     t.queue_capacity = 1u << 21;
     t.tile = 8;
 }
@@ -283,7 +284,7 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
         if (st != FND_OK) return fail(st);
     }
     void* p = nullptr;
-    st = dev_alloc(ctx, &p, 64 * sizeof(uint32_t));
+    st = dev_alloc(ctx, &p, kCounterWords * sizeof(uint32_t));
     if (st != FND_OK) return fail(st);
     ctx->d_counters = static_cast<uint32_t*>(p);
     st = dev_alloc(ctx, &p, 8 * sizeof(unsigned long long));
@@ -315,7 +316,7 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
     commands_free(ctx);
     render_state_free(ctx);
     cache_free(ctx);
-    dev_free(ctx, ctx->d_counters, 64 * sizeof(uint32_t));
+    dev_free(ctx, ctx->d_counters, kCounterWords * sizeof(uint32_t));
     dev_free(ctx, ctx->d_present, 255 * sizeof(float));
     dev_free(ctx, ctx->d_stats, 8 * sizeof(unsigned long long));
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);
@@ -638,7 +639,8 @@ fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4*
     a.spill_stride = l.grid * l.block;
     a.refill = std::max(1u, std::min(64u, ctx->tuning.refill));
     a.stats = ctx->d_stats;
-    st = check_hip(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(uint32_t), ctx->stream), "reset counter");
+    st = check_hip(ctx, hipMemsetAsync(ctx->d_counters, 0, kCounterWords * sizeof(uint32_t), ctx->stream),
+                   "reset counter");
     if (st != FND_OK) return st;
     if (count) {
         st = check_hip(ctx, hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream), "reset stats");

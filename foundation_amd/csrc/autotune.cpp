@@ -34,7 +34,7 @@ struct Candidate {
 // Instantiated (block, LDS stack) pairs of the traversal kernel.
 const uint32_t kPairs[][2] = {{64, 16}, {64, 32}, {128, 8}, {128, 12}, {128, 16}, {128, 32}, {256, 8}, {256, 12}, {256, 16}};
 const uint32_t kWaves[] = {12, 16, 20, 24, 32};
-const uint32_t kRefill[] = {16, 32, 40, 48};
+const uint32_t kRefill[] = {8, 12, 16, 24, 32, 48};
 const uint32_t kQueue[] = {1u << 20, 1u << 21, 1u << 22};
 const uint32_t kTile[] = {4, 8, 16};
 

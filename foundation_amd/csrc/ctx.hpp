@@ -120,7 +120,7 @@ struct fnd_pt_ctx {
     fnd::CtxMap<uint64_t, fnd::Framebuffer*> framebuffers;
     // scratch owned by the context
     float* d_present = nullptr;                // 255 byte thresholds of the RGBA8 transfer
-    uint32_t* d_counters = nullptr;            // 64 work counters
+    uint32_t* d_counters = nullptr;            // traversal work counter (kCounterWords)
     unsigned long long* d_stats = nullptr;     // 8 counters
     uint2* d_spill = nullptr;
     uint64_t spill_bytes = 0;
@@ -162,7 +162,7 @@ void commands_free(fnd_pt_ctx* ctx);
 // Fixed device scratch of fnd_pt_trace_noalloc (noalloc.hip): 256 bytes of
 // work counter, then the traversal's stack spill area.
 constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
-constexpr uint64_t kNoAllocSpillOffset = 256;
+constexpr uint64_t kNoAllocSpillOffset = kCounterWords * sizeof(uint32_t);
 // Device address of the scratch array on the current device.
 hipError_t noalloc_scratch_address(void** out);
 // Zeroes the 32-bit counter with a one-thread kernel on `s` (a kernel, not a

@@ -7,19 +7,21 @@
 // context; this entry point uses one fixed array per GPU instead, resolved
 // to a device address once per context by fnd_pt_create.  Every call on a
 // GPU shares it, so calls that overlap on one GPU race; a call zeroes the
-// counter on its stream before the launch and needs nothing else cleared.
+// counter's shards on its stream before the launch (with a kernel, not a
+// memset) and needs nothing else cleared.
 #include "ctx.hpp"
 
 namespace fnd {
 
-// [0, 256): the work counter; [256, kNoAllocScratchBytes): the spill area.
+// [0, kNoAllocSpillOffset): the work counter's shards; [kNoAllocSpillOffset,
+// kNoAllocScratchBytes): the spill area.
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
 namespace {
 
 __global__ void k_zero_counter(uint32_t* counter)
 {
-    if (threadIdx.x == 0) counter[0] = 0u;
+    zero_trace_counter(counter, threadIdx.x);
 }
 
 }  // namespace

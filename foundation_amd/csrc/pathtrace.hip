@@ -375,17 +375,16 @@ __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
 }
 
 // Zeroes the counters one iteration appends to: the next extension queue,
-// the shadow queue, the traversal work counters and the free list this
-// iteration's retired slots go to.
-__global__ void k_prep(uint32_t* counters, uint32_t next, uint32_t free_zero)
+// the shadow queue, the traversal work counter's shards and the free list
+// this iteration's retired slots go to.
+__global__ void k_prep(uint32_t* counters, uint32_t* trace_ctr, uint32_t next, uint32_t free_zero)
 {
     if (threadIdx.x == 0) {
         counters[next] = 0u;
         counters[kCntShadow] = 0u;
-        counters[kCntWorkExt] = 0u;
-        counters[kCntWorkShadow] = 0u;
         counters[kCntFree0 + free_zero] = 0u;
     }
+    zero_trace_counter(trace_ctr, threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t fw)
@@ -824,7 +823,7 @@ void launch_present(const float* accum, uint64_t pixels, float inv, float exposu
 
 void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, next, free_zero);
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, a.b.trace_ctr, next, free_zero);
 }
 
 void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t grid, hipStream_t s)

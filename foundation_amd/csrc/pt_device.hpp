@@ -22,7 +22,6 @@ struct RayK {
     float dx, dy, dz;      // direction
     float ix, iy, iz;      // 1 / d with |d| clamped to >= 2^-80
     float Sx, Sy, Sz;      // shear constants
-    float okx, oky, okz;   // origin permuted to (kx, ky, kz)
     uint32_t perm;         // kx | ky << 2 | kz << 4
 };
 
@@ -57,9 +56,6 @@ __device__ __forceinline__ void ray_setup(RayK& r, float ox, float oy, float oz,
     r.Sz = sel3(kz, r.ix, r.iy, r.iz);
     r.Sx = sel3(kx, dx, dy, dz) * r.Sz;
     r.Sy = sel3(ky, dx, dy, dz) * r.Sz;
-    r.okx = sel3(kx, ox, oy, oz);
-    r.oky = sel3(ky, ox, oy, oz);
-    r.okz = sel3(kz, ox, oy, oz);
 }
 
 // Row-major 3x4 world -> object, o' = M o + m, d' = M d, fma nesting fixed.
@@ -83,15 +79,20 @@ __device__ __forceinline__ bool isect_tri(const RayK& r, const float4 a, const f
                                           float& Vn, float& Wn, float& detn)
 {
     const uint32_t kx = r.perm & 3u, ky = (r.perm >> 2) & 3u, kz = (r.perm >> 4) & 3u;
-    const float Akx = sel3(kx, a.x, a.y, a.z) - r.okx;
-    const float Aky = sel3(ky, a.x, a.y, a.z) - r.oky;
-    const float Akz = sel3(kz, a.x, a.y, a.z) - r.okz;
-    const float Bkx = sel3(kx, b.x, b.y, b.z) - r.okx;
-    const float Bky = sel3(ky, b.x, b.y, b.z) - r.oky;
-    const float Bkz = sel3(kz, b.x, b.y, b.z) - r.okz;
-    const float Ckx = sel3(kx, c.x, c.y, c.z) - r.okx;
-    const float Cky = sel3(ky, c.x, c.y, c.z) - r.oky;
-    const float Ckz = sel3(kz, c.x, c.y, c.z) - r.okz;
+    // vertex - origin per axis, then permuted (the same differences as
+    // permuting first; no permuted origin is kept per ray)
+    const float ax = a.x - r.ox, ay = a.y - r.oy, az = a.z - r.oz;
+    const float bx = b.x - r.ox, by = b.y - r.oy, bz = b.z - r.oz;
+    const float cx = c.x - r.ox, cy = c.y - r.oy, cz = c.z - r.oz;
+    const float Akx = sel3(kx, ax, ay, az);
+    const float Aky = sel3(ky, ax, ay, az);
+    const float Akz = sel3(kz, ax, ay, az);
+    const float Bkx = sel3(kx, bx, by, bz);
+    const float Bky = sel3(ky, bx, by, bz);
+    const float Bkz = sel3(kz, bx, by, bz);
+    const float Ckx = sel3(kx, cx, cy, cz);
+    const float Cky = sel3(ky, cx, cy, cz);
+    const float Ckz = sel3(kz, cx, cy, cz);
     const float Ax = fmaf(-r.Sx, Akz, Akx);
     const float Ay = fmaf(-r.Sy, Akz, Aky);
     const float Bx = fmaf(-r.Sx, Bkz, Bkx);
@@ -117,6 +118,13 @@ __device__ __forceinline__ bool isect_tri(const RayK& r, const float4 a, const f
     Wn = W;
     detn = det;
     return true;
+}
+
+// Zeroes the traversal work counter's shards; threads [0, kCounterShards) of
+// a kernel that runs before the traversal on the same stream each clear one.
+__device__ __forceinline__ void zero_trace_counter(uint32_t* counter, uint32_t t)
+{
+    if (t < kCounterShards) counter[t * kCounterStride] = 0u;
 }
 
 // x * 2^e, exact (v_ldexp_f32)

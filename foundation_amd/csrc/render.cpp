@@ -94,6 +94,7 @@ static bool carve_queues(fnd_arena* ar, uint64_t cap, WfBuffers& b)
     A(&b.ext_hits, cap);
     A(&b.sh_rays, cap); A(&b.sh_path, cap); A(&b.sh_scene, cap); A(&b.sh_contrib, cap); A(&b.sh_hits, cap);
     A(&b.counters, kCntCount);
+    A(&b.trace_ctr, kCounterWords);
     A(&b.rays, 1);
     A(&b.q_query, cap); A(&b.q_path, cap); A(&b.q_weight, cap); A(&b.q_pred, cap);
     return ok;
@@ -660,7 +661,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             TraceArgs ta{};
             ta.scenes = rs->d_scenes;
             ta.hits = rs->b.ext_hits;
-            ta.counter = &rs->b.counters[kCntWorkExt];
+            ta.counter = rs->b.trace_ctr;
             ta.spill = ctx->d_spill;
             ta.spill_stride = tl.grid * tl.block;
             ta.refill = std::max(1u, std::min(64u, ctx->tuning.refill));

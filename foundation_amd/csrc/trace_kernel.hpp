@@ -3,9 +3,10 @@
 // Design for CDNA4 (wave64, 160 KB LDS per CU, no ray-tracing units):
 //  * One ray per lane.  A wave keeps its 64 lanes busy by refetching rays:
 //    once `refill` lanes are idle, one lane takes a block of ray indices with a
-//    single atomicAdd and the idle lanes take theirs by popcount of the ballot
-//    below them (Aila & Laine, "Understanding the Efficiency of Ray Traversal on
-//    GPUs", HPG 2009, dynamic fetch; here done with wave ballots).
+//    single atomicAdd on the wave's shard of the work counter and the idle
+//    lanes take theirs by popcount of the ballot below them (Aila & Laine,
+//    "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009,
+//    dynamic fetch; here done with wave ballots).
 //  * The traversal stack is per lane in LDS, laid out [entry][thread] so any
 //    mix of stack depths across a wave hits distinct banks; entries past the
 //    LDS part go to a per-thread spill area in HBM sized from the tree depth.
@@ -24,9 +25,10 @@
 
 namespace fnd {
 
-// Rays per trace launch: ray indices are formed as (grid threads + a 32-bit
-// work counter) and must not wrap.
+// Rays per trace launch: ray indices are formed from grid threads and a
+// 32-bit shard counter and must not wrap.
 constexpr uint32_t kMaxRaysPerTrace = 0x7FFFFFFFu;
+constexpr uint32_t kWave = 64;
 
 struct SceneDev {
     const WideNode* nodes;
@@ -52,7 +54,7 @@ struct TraceArgs {
     float4* hits;
     uint32_t nrays;
     const uint32_t* nrays_dev; // if set, the ray count is read here (wavefront queues)
-    uint32_t* counter;         // work counter, zero at launch
+    uint32_t* counter;         // work counter: kCounterWords words, shards zero at launch
     uint2* spill;
     uint32_t spill_stride;     // threads in the grid
     uint32_t refill;
@@ -115,14 +117,20 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args)
 
     // Ray fetch.  The first fetch is static (lane gtid takes ray gtid), so a
     // launch whose queue is short or empty retires its surplus waves without
-    // touching the shared counter; later fetches take exactly the idle lanes'
-    // worth of rays past the grid's first gtid range from that counter.  The
-    // one shared counter keeps the rays in flight a sliding window over the
-    // ray array, and neighbouring rays share top-level nodes in L2: measured
-    // faster than fixed per-wave shares of the array, and than reserving the
-    // next block one refill ahead (the reply is waited for by the wave's next
-    // in-order load wait anyway); one counter per XCD measured no faster.
+    // an atomic; later fetches take exactly the idle lanes' worth of rays,
+    // past the grid's first gtid range, from the wave's shard of the work
+    // counter.  Shard s hands out the blocks of 64 rays s, s + kCounterShards,
+    // ... (shard-local index l is ray grid_threads + (l / 64 * kCounterShards
+    // + s) * 64 + l % 64), so the shards advance through the rays together and
+    // the rays in flight stay a sliding window over the ray array, whose
+    // neighbouring rays share nodes in L2 (measured faster than fixed
+    // per-wave shares of the array).  One address would serialise every
+    // wave's refill atomic: at a refill threshold of 48 the single counter was
+    // busy for about 40% of the incoherent 10M-triangle launch, and lower
+    // thresholds were slower still; with the shards, refills can be frequent.
     const uint32_t grid_threads = gridDim.x * BLOCK;
+    const uint32_t shard = (gtid / kWave) & (kCounterShards - 1u);
+    uint32_t* const shard_ctr = args.counter + shard * kCounterStride;
     bool first_fetch = true;
     for (;;) {
         const uint64_t idle = __ballot(!active && !retired);
@@ -130,15 +138,16 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args)
         if (idle != 0 && (act == 0 || uint32_t(__popcll(idle)) >= args.refill)) {
             const uint32_t nidle = uint32_t(__popcll(idle));
             const int leader = __builtin_ctzll(idle);
-            uint32_t base = 0;
+            uint32_t idx;
             if (first_fetch) {
-                base = gtid - __lane_id();  // every lane is idle: idx = gtid
+                idx = gtid;  // every lane is idle
                 first_fetch = false;
             } else {
-                if (int(__lane_id()) == leader) base = grid_threads + atomicAdd(args.counter, nidle);
-                base = uint32_t(__builtin_amdgcn_readlane(int(base), leader));
+                uint32_t base = 0;
+                if (int(__lane_id()) == leader) base = atomicAdd(shard_ctr, nidle);
+                const uint32_t l = uint32_t(__builtin_amdgcn_readlane(int(base), leader)) + lanes_below(idle);
+                idx = grid_threads + ((l / kWave) * kCounterShards + shard) * kWave + l % kWave;
             }
-            const uint32_t idx = base + lanes_below(idle);
             if (!active && !retired) {
                 if (idx < nrays) {
                     const bool second = MIXED && idx >= nrays1;

@@ -66,8 +66,7 @@ enum WfCounter : uint32_t {
     kCntExtA = 0,
     kCntExtB = 1,
     kCntShadow = 2,
-    kCntWorkExt = 3,
-    kCntWorkShadow = 4,
+    // 3, 4: unused (the traversal's work counter is WfBuffers::trace_ctr)
     kCntQuery = 5,
     kCntTrain = 6,
     kCntNext = 7,    // next window path to start (may pass the window's end)
@@ -112,6 +111,7 @@ struct WfBuffers {
     float4* sh_contrib;
     float4* sh_hits;
     uint32_t* counters;       // kCntCount
+    uint32_t* trace_ctr;      // the traversal launches' work counter (kCounterWords)
     unsigned long long* rays; // rays traced (extension + shadow), cumulative: added by the trace launches
     // radiance-cache queries (at most one per path per batch)
     CacheQuery* q_query;

@@ -199,7 +199,9 @@ def test_concurrent_allocations_are_disjoint():
 # 84; the window's radiance, 16 B for each of 8 window paths per slot, 128;
 # two free lists, 8.
 QUEUE_BYTES_PER_PATH = 56 + 80 + 16 + 72 + 84 + 128 + 8
-QUEUE_BYTES_FIXED = 256 + 256  # counters and the ray total, one 256-byte slot each
+# counters (one 256-byte slot), the traversal work counter (16 shards, one
+# 256-byte line each) and the ray total (one 256-byte slot)
+QUEUE_BYTES_FIXED = 256 + 16 * 256 + 256
 
 
 def test_queue_bytes_per_path():
@@ -207,9 +209,9 @@ def test_queue_bytes_per_path():
         cap = 64 * k  # every buffer a multiple of 256 bytes: no padding
         assert pt.queue_bytes(cap) == QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED, cap
     prev = 0
-    for cap in range(1, 600, 7):  # padding adds at most 255 bytes to each of the 29 buffers
+    for cap in range(1, 600, 7):  # padding adds at most 255 bytes to each of the 29 per-slot buffers
         q = pt.queue_bytes(cap)
-        assert QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED <= q <= QUEUE_BYTES_PER_PATH * cap + 29 * 256
+        assert QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED <= q <= QUEUE_BYTES_PER_PATH * cap + QUEUE_BYTES_FIXED + 29 * 256
         assert q % 256 == 0 and q >= prev
         prev = q
     with pytest.raises(pt.FndError):
