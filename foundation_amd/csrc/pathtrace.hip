@@ -303,7 +303,7 @@ __device__ __forceinline__ void retire(const WfArgs& a, uint32_t slot, float4 L)
 // queue A, padding slots (radiance 0) straight to free list 0.
 __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_t n)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.b.counters[kCntNext] = n;
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.b.counters[cnt(kCntNext)] = n;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
         const bool dead = i < n && !live;
         if (dead) retire(a, i, make_float4(0.f, 0.f, 0.f, 0.f));
         uint32_t slot[2];
-        block_append<2>({live, dead}, {&a.b.counters[kCntExtA], &a.b.counters[kCntFree0]}, slot);
+        block_append<2>({live, dead}, {&a.b.counters[cnt(kCntExtA)], &a.b.counters[cnt(kCntFree0)]}, slot);
         if (live) {
             a.b.ext_rays[0][slot[0]] = ray;
             a.b.ext_path[0][slot[0]] = i;
@@ -327,12 +327,12 @@ __global__ __launch_bounds__(256) void k_generate(WfArgs a, uint64_t g0, uint32_
 // (radiance 0) straight to the other free list.
 __global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t fidx, uint32_t next)
 {
-    const uint32_t nf = a.b.counters[kCntFree0 + fidx];
+    const uint32_t nf = a.b.counters[cnt(kCntFree0 + fidx)];
     // No path left: return, so the counter stops growing.  One read per block,
     // shared, so the whole block returns or none of it does (block_append
     // synchronises the block).
     __shared__ uint32_t s_next;
-    if (threadIdx.x == 0) s_next = a.b.counters[kCntNext];
+    if (threadIdx.x == 0) s_next = a.b.counters[cnt(kCntNext)];
     __syncthreads();
     if (s_next >= a.b.window) return;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t f
         const uint32_t i = base + threadIdx.x;
         const bool want = i < nf;
         uint32_t gs[1];
-        block_append<1>({want}, {&a.b.counters[kCntNext]}, gs);
+        block_append<1>({want}, {&a.b.counters[cnt(kCntNext)]}, gs);
         const bool has = want && gs[0] < a.b.window;
         const uint32_t slot = want ? a.b.free_list[fidx][i] : 0u;
         Ray ray{};
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(256) void k_regen(WfArgs a, uint64_t g0, uint32_t f
         const bool dead = has && !live;
         if (dead) retire(a, slot, make_float4(0.f, 0.f, 0.f, 0.f));
         uint32_t q[2];
-        block_append<2>({live, dead}, {&a.b.counters[next], &a.b.counters[kCntFree0 + (fidx ^ 1u)]}, q);
+        block_append<2>({live, dead}, {&a.b.counters[cnt(next)], &a.b.counters[cnt(kCntFree0 + (fidx ^ 1u))]}, q);
         if (live) {
             a.b.ext_rays[next][q[0]] = ray;
             a.b.ext_path[next][q[0]] = slot;
@@ -365,13 +365,13 @@ __global__ void k_fill_u64(unsigned long long* p, uint32_t n, unsigned long long
 
 __global__ void k_snapshot(uint32_t* counters, uint32_t src, uint32_t dst)
 {
-    if (threadIdx.x == 0) counters[dst] = counters[src];
+    if (threadIdx.x == 0) counters[cnt(dst)] = counters[cnt(src)];
 }
 
 // Zeroes the counter slots whose bit is set in mask.
 __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
 {
-    if (threadIdx.x < kCntCount && ((mask >> threadIdx.x) & 1u)) counters[threadIdx.x] = 0u;
+    if (threadIdx.x < kCntCount && ((mask >> threadIdx.x) & 1u)) counters[cnt(threadIdx.x)] = 0u;
 }
 
 // Zeroes the counters one iteration appends to: the next extension queue,
@@ -380,16 +380,16 @@ __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
 __global__ void k_prep(uint32_t* counters, uint32_t* trace_ctr, uint32_t next, uint32_t free_zero)
 {
     if (threadIdx.x == 0) {
-        counters[next] = 0u;
-        counters[kCntShadow] = 0u;
-        counters[kCntFree0 + free_zero] = 0u;
+        counters[cnt(next)] = 0u;
+        counters[cnt(kCntShadow)] = 0u;
+        counters[cnt(kCntFree0 + free_zero)] = 0u;
     }
     zero_trace_counter(trace_ctr, threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t fw)
 {
-    const uint32_t n = a.b.counters[cur];
+    const uint32_t n = a.b.counters[cnt(cur)];
     const uint32_t nxt = cur ^ 1u;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -570,8 +570,8 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
         const bool freed = a.b.retire && i < n && !cont && !shadow;
         uint32_t slots[4];
         block_append<4>({cont, query, shadow, freed},
-                        {&a.b.counters[nxt], &a.b.counters[kCntQuery], &a.b.counters[kCntShadow],
-                         &a.b.counters[kCntFree0 + fw]},
+                        {&a.b.counters[cnt(nxt)], &a.b.counters[cnt(kCntQuery)], &a.b.counters[cnt(kCntShadow)],
+                         &a.b.counters[cnt(kCntFree0 + fw)]},
                         slots);
         const uint32_t es = slots[0], qs = slots[1], ss = slots[2];
         if (freed) a.b.free_list[fw][slots[3]] = path;
@@ -600,7 +600,7 @@ __global__ __launch_bounds__(256) void k_shade(WfArgs a, uint32_t cur, uint32_t 
 // goes to free list fw.
 __global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a, uint32_t fw)
 {
-    const uint32_t n = a.b.counters[kCntShadow];
+    const uint32_t n = a.b.counters[cnt(kCntShadow)];
     if (!a.b.retire) {  // a window without regeneration: paths stay in their slots
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
             if (__float_as_uint(a.b.sh_hits[i].y) != 0xFFFFFFFFu) continue;
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(256) void k_shadow_resolve(WfArgs a, uint32_t fw)
             }
         }
         uint32_t q[1];
-        block_append<1>({freed}, {&a.b.counters[kCntFree0 + fw]}, q);
+        block_append<1>({freed}, {&a.b.counters[cnt(kCntFree0 + fw)]}, q);
         if (freed) a.b.free_list[fw][q[0]] = path;
     }
 }
@@ -670,7 +670,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WfArgs a, uint64_t g0, uint3
 // Cache predictions replace the rest of the path: L += throughput * albedo * max(pred, 0).
 __global__ __launch_bounds__(256) void k_cache_apply(WfArgs a)
 {
-    const uint32_t n = a.b.counters[kCntQuery];
+    const uint32_t n = a.b.counters[cnt(kCntQuery)];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t gid = a.b.q_path[i];
         const float4 w = a.b.q_weight[i], p = a.b.q_pred[i];
@@ -717,7 +717,7 @@ __global__ __launch_bounds__(256) void k_cache_collect(WfArgs a, uint64_t g0, ui
                     take = true;
                 }
             }
-            const uint32_t slot = wave_append(take, &a.b.counters[kCntTrain]);
+            const uint32_t slot = wave_append(take, &a.b.counters[cnt(kCntTrain)]);
             if (take && slot < a.b.t_cap) {
                 a.b.t_query[slot] = q;
                 a.b.t_target[slot] = target;

@@ -37,7 +37,7 @@ constexpr uint32_t kPollRing = kPollLag + 1;
 
 struct RenderState {
     WfBuffers b{};
-    uint32_t* poll_host = nullptr;  // kPollRing copies of the kCntCount counters, pinned host memory
+    uint32_t* poll_host = nullptr;  // kPollRing copies of the counter slots (cnt()), pinned host memory
     hipEvent_t poll_ev[kPollRing] = {};
     hipEvent_t side_ev[2] = {};     // early cache queries ready / answered
     fnd_arena frame{};        // reset at every render call, the queues are carved from it
@@ -93,7 +93,7 @@ static bool carve_queues(fnd_arena* ar, uint64_t cap, WfBuffers& b)
     }
     A(&b.ext_hits, cap);
     A(&b.sh_rays, cap); A(&b.sh_path, cap); A(&b.sh_scene, cap); A(&b.sh_contrib, cap); A(&b.sh_hits, cap);
-    A(&b.counters, kCntCount);
+    A(&b.counters, kCntCount * kCntStride);
     A(&b.trace_ctr, kCounterWords);
     A(&b.rays, 1);
     A(&b.q_query, cap); A(&b.q_path, cap); A(&b.q_weight, cap); A(&b.q_pred, cap);
@@ -633,7 +633,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         // Drain polling resources, made once per render state.
         if (!rs->poll_host) {
             void* h = nullptr;
-            st = check_hip(ctx, hipHostMalloc(&h, kPollRing * kCntCount * sizeof(uint32_t)), "hipHostMalloc");
+            st = check_hip(ctx, hipHostMalloc(&h, kPollRing * kCntCount * kCntStride * sizeof(uint32_t)), "hipHostMalloc");
             if (st != FND_OK) return st;
             rs->poll_host = static_cast<uint32_t*>(h);
             for (auto& e : rs->poll_ev) {
@@ -668,7 +668,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             ta.rays2 = rs->b.sh_rays;
             ta.hits2 = rs->b.sh_hits;
             ta.ray_scene2 = rs->b.sh_scene;
-            ta.nrays2_dev = &rs->b.counters[kCntShadow];
+            ta.nrays2_dev = &rs->b.counters[cnt(kCntShadow)];
             ta.ray_total = rs->b.rays;
             TraceLaunch l = tl;
             l.mode = kModeMixedInternal;
@@ -678,7 +678,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             auto trace = [&](uint32_t q, const char* what) -> fnd_status {
                 ta.rays = rs->b.ext_rays[q];
                 ta.ray_scene = rs->b.ext_scene[q];
-                ta.nrays_dev = &rs->b.counters[q];
+                ta.nrays_dev = &rs->b.counters[cnt(q)];
                 return check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
             };
             for (uint64_t g0 = 0; g0 < paths; g0 += wmax) {
@@ -717,7 +717,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                             e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
                             if (e == FND_OK) e = check_hip(ctx, hipStreamWaitEvent(ctx->side, rs->side_ev[0], 0), "side stream");
                             if (e != FND_OK) return e;
-                            launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuerySplit], rs->b.q_pred,
+                            launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[cnt(kCntQuerySplit)], rs->b.q_pred,
                                                  nullptr, 0, cg, ctx->side);
                             e = check_hip(ctx, hipEventRecord(rs->side_ev[1], ctx->side), "side stream");
                             if (e != FND_OK) return e;
@@ -750,8 +750,9 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                         if (e != FND_OK) return e;
                         launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
                         const uint32_t ring = uint32_t(k % kPollRing);
-                        e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount, rs->b.counters,
-                                                          kCntCount * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+                        e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount * kCntStride, rs->b.counters,
+                                                          kCntCount * kCntStride * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                                          ctx->stream),
                                       "drain poll");
                         if (e == FND_OK) e = check_hip(ctx, hipEventRecord(rs->poll_ev[ring], ctx->stream), "drain poll");
                         if (e != FND_OK) return e;
@@ -761,16 +762,16 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                             e = check_hip(ctx, hipEventSynchronize(rs->poll_ev[old]), "drain poll");
                             if (e != FND_OK) return e;
                             // queue `old_next` is what iteration k - kPollLag left to shade
-                            const uint32_t* c = rs->poll_host + old * kCntCount;
+                            const uint32_t* c = rs->poll_host + old * kCntCount * kCntStride;
                             const uint32_t old_next = uint32_t(((k - kPollLag) & 1u) ? kCntExtA : kCntExtB);
-                            drained = c[old_next] == 0u && c[kCntNext] >= W;
+                            drained = c[cnt(old_next)] == 0u && c[cnt(kCntNext)] >= W;
                         }
                     }
                 }
                 if (apply_cache) {
                     // the queries not answered early: [split, count), then wait for the early ones
-                    launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[kCntQuery], rs->b.q_pred, nullptr, 0,
-                                         cg, ctx->stream, early ? &rs->b.counters[kCntQuerySplit] : nullptr);
+                    launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[cnt(kCntQuery)], rs->b.q_pred, nullptr, 0,
+                                         cg, ctx->stream, early ? &rs->b.counters[cnt(kCntQuerySplit)] : nullptr);
                     if (early) {
                         e = check_hip(ctx, hipStreamWaitEvent(ctx->stream, rs->side_ev[1], 0), "side stream");
                         if (e != FND_OK) return e;
@@ -806,7 +807,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         // ---- optimise on the samples the frame gathered (the next frame reads the result)
         if (any_train) {
             uint32_t nsamp = 0;
-            st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[kCntTrain], 4, hipMemcpyDeviceToHost, ctx->stream),
+            st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[cnt(kCntTrain)], 4, hipMemcpyDeviceToHost, ctx->stream),
                            "read sample count");
             if (st == FND_OK) st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "training paths");
             if (st != FND_OK) return st;

@@ -75,6 +75,12 @@ enum WfCounter : uint32_t {
     kCntQuerySplit = 10,  // radiance-cache queries answered early, on the side stream
     kCntCount = 16
 };
+// Each counter slot has its own 256-byte line: atomics to one line serialise
+// (measured on MI355X: 88 M/s for one line, whether the atomics hit one word
+// or several words of it), and the shade and regeneration launches append to
+// several queues at once.
+constexpr uint32_t kCntStride = 64;
+__host__ __device__ constexpr uint32_t cnt(uint32_t slot) { return slot * kCntStride; }
 
 // Paths are rendered in windows of consecutive global path indices; a window
 // holds at most kWindowSlots times as many paths as there are slots, and a
@@ -110,7 +116,7 @@ struct WfBuffers {
     uint32_t* sh_scene;
     float4* sh_contrib;
     float4* sh_hits;
-    uint32_t* counters;       // kCntCount
+    uint32_t* counters;       // kCntCount slots, kCntStride words apart (index with cnt())
     uint32_t* trace_ctr;      // the traversal launches' work counter (kCounterWords)
     unsigned long long* rays; // rays traced (extension + shadow), cumulative: added by the trace launches
     // radiance-cache queries (at most one per path per batch)

@@ -199,9 +199,9 @@ def test_concurrent_allocations_are_disjoint():
 # 84; the window's radiance, 16 B for each of 8 window paths per slot, 128;
 # two free lists, 8.
 QUEUE_BYTES_PER_PATH = 56 + 80 + 16 + 72 + 84 + 128 + 8
-# counters (one 256-byte slot), the traversal work counter (16 shards, one
-# 256-byte line each) and the ray total (one 256-byte slot)
-QUEUE_BYTES_FIXED = 256 + 16 * 256 + 256
+# counters (16 slots, one 256-byte line each), the traversal work counter (16
+# shards, one 256-byte line each) and the ray total (one 256-byte slot)
+QUEUE_BYTES_FIXED = 16 * 256 + 16 * 256 + 256
 
 
 def test_queue_bytes_per_path():
