@@ -15,6 +15,11 @@ instance, counted by the kernel's counting variant over the same rays) over
 its mean launch time from HIP events; the CPU oracle timed on the same rays
 on the host cores (cpu_baseline); and hit-record parity GPU vs oracle.
 
+extra.launch_scaling: kernel ms of one launch over the incoherent set and
+over four copies of it in one array; the linear fit splits a launch into a
+fixed part (start, and the drain while the longest rays finish) and a
+steady per-ray rate.
+
 extra.traversal_1_5M: the traversal microbenchmark configuration (1.31M-
 triangle displaced icosphere + 200k soup; 1080p coherent, incoherent and
 shadow ray sets) with its own hit-record parity against the oracle.
@@ -81,6 +86,37 @@ def time_gpu_set(ctx, scene, rays, steps, warmup, flags):
     ctx.buffer_destroy(hb)
     counts = dict(nodes=cs.node_visits, tris=cs.tri_tests, instances=cs.instance_visits)
     return wall, kernel_ms, hits, counts
+
+
+def launch_scaling(ctx, scene, rays, copies=4, steps=5):
+    """Kernel ms of one launch over the ray set and over `copies` copies of it
+    back to back in one array: a linear fit splits a launch into a fixed part
+    (start, and the drain while its longest rays finish) and a per-ray part."""
+    import numpy as np
+    from foundation_amd import _abi as A
+    out = {}
+    for k in (1, copies):
+        rr = np.ascontiguousarray(np.tile(rays, (k, 1)))
+        n = rr.shape[0]
+        rb = ctx.buffer_create(n * 32)
+        hb = ctx.buffer_create(n * 16)
+        ctx.buffer_write(rb, rr)
+        ctx.trace_buffers(scene, rb, hb, n, A.ASYNC)
+        ctx.wait()
+        ctx.reset_stats()
+        for _ in range(steps):
+            ctx.trace_buffers(scene, rb, hb, n, A.ASYNC)
+        ctx.wait()
+        st = ctx.stats()
+        out[k] = st.trace_kernel_ms_total / max(1, st.trace_launches)
+        ctx.buffer_destroy(rb)
+        ctx.buffer_destroy(hb)
+    n1 = rays.shape[0]
+    per = (out[copies] - out[1]) / (copies - 1)
+    return {"rays": n1, "copies": copies, "ms_1x": round(out[1], 4), f"ms_{copies}x": round(out[copies], 4),
+            "mrays_s_1x": round(n1 / out[1] / 1e3, 1), f"mrays_s_{copies}x": round(copies * n1 / out[copies] / 1e3, 1),
+            "fixed_ms": round(out[1] - per, 4), "ms_per_ray_set": round(per, 4),
+            "steady_mrays_s": round(n1 / per / 1e3, 1)}
 
 
 def render_throughput(ctx, scene, camera, width, height, spp, depth, rr):
@@ -560,7 +596,10 @@ def main():
     coherent = shadow_rate = None
     ccounts = {"nodes": 0}
     gpu_shadow = None
+    scaling = None
     if not args.only_incoherent:
+        scaling = launch_scaling(ctx, scene, incoherent)
+        log(f"launch scaling: {scaling}")
         csteps = max(2, args.steps // 2)
         cwall, ckms, _, ccounts = time_gpu_set(ctx, scene, primary, csteps, 1, 0)
         swall, skms, gpu_shadow, _ = time_gpu_set(ctx, scene, shadow, csteps, 1, A.TRACE_ANYHIT)
@@ -686,6 +725,7 @@ def main():
                   "coherent_nodes_per_ray": round(ccounts["nodes"] / primary.shape[0], 3),
                   "scene_nodes": int(info.num_nodes), "scene_build_ms": round(info.build_ms, 1),
                   "scene_mib": round(info.device_bytes / 2**20, 1), "parity": parity,
+                  "launch_scaling": scaling,
                   "trace_valu_issue": valu,
                   "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
                   "traversal_1_5M": micro, "c1_cornell_256": c1, "path_tracing_10M": render,
