@@ -279,6 +279,10 @@ fnd_status fnd_pt_create(const fnd_pt_create_desc* desc, const fnd_host_allocato
     if (st != FND_OK) return fail(st);
     st = check_hip(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking), "hipStreamCreate");
     if (st != FND_OK) return fail(st);
+    st = check_hip(ctx, hipStreamCreateWithFlags(&ctx->train, hipStreamNonBlocking), "hipStreamCreate");
+    if (st != FND_OK) return fail(st);
+    st = check_hip(ctx, hipEventCreateWithFlags(&ctx->train_ev, hipEventDisableTiming), "hipEventCreate");
+    if (st != FND_OK) return fail(st);
     for (auto& ev : ctx->ev) {
         st = check_hip(ctx, hipEventCreate(&ev), "hipEventCreate");
         if (st != FND_OK) return fail(st);
@@ -329,6 +333,11 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
         (void)hipStreamSynchronize(ctx->side);
         (void)hipStreamDestroy(ctx->side);
     }
+    if (ctx->train) {
+        (void)hipStreamSynchronize(ctx->train);
+        (void)hipStreamDestroy(ctx->train);
+    }
+    if (ctx->train_ev) (void)hipEventDestroy(ctx->train_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     const fnd_host_allocator a = ctx->alloc;
     const bool has = ctx->has_alloc;
@@ -348,6 +357,7 @@ fnd_status fnd_pt_wait(fnd_pt_ctx* ctx)
         if (!ctx) return FND_E_INVALID_ARG;
         fnd_status st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         if (st == FND_OK) resolve_timings(ctx);
+        finish_training(ctx);
         return st;
     });
 }

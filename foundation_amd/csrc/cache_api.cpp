@@ -60,6 +60,7 @@ void cache_free(fnd_pt_ctx* ctx)
     if (!cs) return;
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);  // inference may read the weights there
+    finish_training(ctx);
     for (auto& a : cs->allocs) dev_free(ctx, a.first, a.second);
     ctx_delete(ctx, cs);
     ctx->cache = nullptr;
@@ -81,12 +82,24 @@ void cache_set_bounds(CacheState* cs, const float* lo, const float* hi, bool pad
 }
 
 // One optimiser step on n device-resident samples.
+fnd_status join_training(fnd_pt_ctx* ctx, hipStream_t s)
+{
+    if (!ctx->train_pending) return FND_OK;
+    return check_hip(ctx, hipStreamWaitEvent(s, ctx->train_ev, 0), "wait for cache training");
+}
+
+void finish_training(fnd_pt_ctx* ctx)
+{
+    if (ctx->train) (void)hipStreamSynchronize(ctx->train);
+    ctx->train_pending = false;
+}
+
 fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, const float4* target, uint32_t n,
-                      uint32_t sample_stride, bool apply, float* dmlp_host, float* dgrid_host, float* loss_host)
+                      uint32_t sample_stride, bool apply, float* dmlp_host, float* dgrid_host, float* loss_host,
+                      hipStream_t s)
 {
     if (n == 0) return FND_OK;
     if (n > cs->tb.cap) return set_error(ctx, FND_E_INVALID_ARG, "cache batch of %u exceeds %u", n, cs->tb.cap);
-    hipStream_t s = ctx->stream;
     launch_zero(cs->tb.grad_mlp, kMlpParams, s);
     // the lazy grid update clears what it reads; only a step without update
     // (gradients for the caller) leaves the grid gradient to be cleared here
@@ -171,6 +184,7 @@ fnd_status upload_queries(fnd_pt_ctx* ctx, CacheState* cs, const float* queries,
 fnd_status need_cache(fnd_pt_ctx* ctx, CacheState** out)
 {
     if (!ctx) return FND_E_INVALID_ARG;
+    finish_training(ctx);
     *out = cache_get(ctx);
     if (!*out) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
     if (!(*out)->bounds_set) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache has no bounds yet");
@@ -296,6 +310,7 @@ fnd_status fnd_pt_cache_set_params(fnd_pt_ctx* ctx, const float* mlp, const floa
 {
     return guard(ctx, [&]() -> fnd_status {
         if (!ctx) return FND_E_INVALID_ARG;
+        finish_training(ctx);
         CacheState* cs = cache_get(ctx);
         if (!cs) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
         fnd_status st = FND_OK;
@@ -336,6 +351,7 @@ fnd_status fnd_pt_cache_get_params(fnd_pt_ctx* ctx, float* mlp, float* grid)
 {
     return guard(ctx, [&]() -> fnd_status {
         if (!ctx) return FND_E_INVALID_ARG;
+        finish_training(ctx);
         CacheState* cs = cache_get(ctx);
         if (!cs) return set_error(ctx, FND_E_INVALID_ARG, "radiance cache is not configured");
         fnd_status st = FND_OK;
@@ -389,7 +405,7 @@ fnd_status fnd_pt_cache_gradients(fnd_pt_ctx* ctx, const float* queries, const f
         st = upload_queries(ctx, cs, queries, targets, n);
         if (st != FND_OK) return st;
         float l = 0;
-        st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, false, dmlp, dgrid, &l);
+        st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, false, dmlp, dgrid, &l, ctx->stream);
         if (loss) *loss = l;
         return st;
     });
@@ -407,7 +423,8 @@ fnd_status fnd_pt_cache_train(fnd_pt_ctx* ctx, const float* queries, const float
         if (st != FND_OK) return st;
         float l = 0;
         for (uint32_t k = 0; k < steps && st == FND_OK; ++k)
-            st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, true, nullptr, nullptr, k + 1 == steps ? &l : nullptr);
+            st = cache_step(ctx, cs, cs->io_query, cs->io_target, n, 1, true, nullptr, nullptr, k + 1 == steps ? &l : nullptr,
+                            ctx->stream);
         if (loss) *loss = l;
         return st;
     });

@@ -39,7 +39,6 @@ struct CacheState {
     bool bounds_set = false;
     bool grid_grad_dirty = false;     // grid gradient holds values no update has consumed
     uint32_t step = 0;
-    float train_loss_host = 0;        // loss of the last step of the last training frame
     uint32_t last_samples = 0;        // training samples gathered in the last training frame
     uint64_t frames = 0;
 };
@@ -47,8 +46,12 @@ struct CacheState {
 CacheState* cache_get(fnd_pt_ctx* ctx);
 void cache_free(fnd_pt_ctx* ctx);
 void cache_set_bounds(CacheState* cs, const float* lo, const float* hi, bool pad);
-// One optimiser step on n samples q[s * sample_stride], target[s * sample_stride].
+// One optimiser step on n samples q[s * sample_stride], target[s * sample_stride],
+// queued on stream s (the host waits on s only to return gradients or the loss).
 fnd_status cache_step(fnd_pt_ctx* ctx, CacheState* cs, const CacheQuery* q, const float4* target, uint32_t n,
-                      uint32_t sample_stride, bool apply, float* dmlp_host, float* dgrid_host, float* loss_host);
+                      uint32_t sample_stride, bool apply, float* dmlp_host, float* dgrid_host, float* loss_host,
+                      hipStream_t s);
+// Stream s waits for the render-queued training (a no-op when none is queued).
+fnd_status join_training(fnd_pt_ctx* ctx, hipStream_t s);
 
 }  // namespace fnd

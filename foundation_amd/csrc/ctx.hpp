@@ -104,6 +104,12 @@ struct fnd_pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;                // radiance-cache inference overlapped with a frame's last bounces
+    // Radiance-cache training of a frame runs here, overlapped with the next
+    // frame until that frame first reads the cache; train_ev marks its end
+    // (train_pending: an event has been recorded since the last host join).
+    hipStream_t train = nullptr;
+    hipEvent_t train_ev = nullptr;
+    bool train_pending = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     fnd::CtxVector<hipEvent_t> timing_events;  // pairs (start, end) per pending trace launch
     size_t timing_used = 0;                    // events in use (even)
@@ -161,6 +167,10 @@ void commands_free(fnd_pt_ctx* ctx);
 
 // Fixed device scratch of fnd_pt_trace_noalloc (noalloc.hip): 256 bytes of
 // work counter, then the traversal's stack spill area.
+// The host waits for the radiance-cache training a render queued (before
+// anything outside a frame reads or replaces the cache state).
+void finish_training(fnd_pt_ctx* ctx);
+
 constexpr uint64_t kNoAllocScratchBytes = 16ull << 20;
 constexpr uint64_t kNoAllocSpillOffset = kCounterWords * sizeof(uint32_t);
 // Device address of the scratch array on the current device.

@@ -716,6 +716,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                             launch_snapshot(wa, kCntQuery, kCntQuerySplit, ctx->stream);
                             e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
                             if (e == FND_OK) e = check_hip(ctx, hipStreamWaitEvent(ctx->side, rs->side_ev[0], 0), "side stream");
+                            if (e == FND_OK) e = join_training(ctx, ctx->side);
                             if (e != FND_OK) return e;
                             launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[cnt(kCntQuerySplit)], rs->b.q_pred,
                                                  nullptr, 0, cg, ctx->side);
@@ -768,6 +769,12 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                         }
                     }
                 }
+                // the previous frame's training must have updated the cache before
+                // this window reads it or gathers the next training samples
+                if (apply_cache || collect) {
+                    e = join_training(ctx, ctx->stream);
+                    if (e != FND_OK) return e;
+                }
                 if (apply_cache) {
                     // the queries not answered early: [split, count), then wait for the early ones
                     launch_cache_forward(cs->view, rs->b.q_query, 1, 0, &rs->b.counters[cnt(kCntQuery)], rs->b.q_pred, nullptr, 0,
@@ -804,7 +811,10 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         st = run(num_jobs, total, cap, max_depth, any_use, any_train);
         if (st != FND_OK) return st;
 
-        // ---- optimise on the samples the frame gathered (the next frame reads the result)
+        // ---- optimise on the samples the frame gathered.  The steps run on the
+        // training stream, overlapped with the next call until it first reads the
+        // cache (join_training); only the next frame needs them, so this call's
+        // time ends with its image.
         if (any_train) {
             uint32_t nsamp = 0;
             st = check_hip(ctx, hipMemcpyAsync(&nsamp, &rs->b.counters[cnt(kCntTrain)], 4, hipMemcpyDeviceToHost, ctx->stream),
@@ -818,11 +828,15 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                 const uint32_t steps = std::min(cs->desc.train_steps, nsamp);
                 const uint32_t per = std::max(1u, std::min(cs->desc.train_batch, nsamp / steps));
                 const uint32_t S = nsamp / per;
+                // the main stream is idle here (the sample count was read after it)
                 for (uint32_t k = 0; k < steps; ++k) {
-                    float* lp = (k + 1 == steps) ? &cs->train_loss_host : nullptr;
-                    st = cache_step(ctx, cs, cs->s_query + k, cs->s_target + k, per, S, true, nullptr, nullptr, lp);
+                    st = cache_step(ctx, cs, cs->s_query + k, cs->s_target + k, per, S, true, nullptr, nullptr, nullptr,
+                                    ctx->train);
                     if (st != FND_OK) return st;
                 }
+                st = check_hip(ctx, hipEventRecord(ctx->train_ev, ctx->train), "cache training");
+                if (st != FND_OK) return st;
+                ctx->train_pending = true;
                 cs->last_samples = nsamp;
             }
             cs->frames += 1;

@@ -31,7 +31,10 @@
  * Threading: a context is single-threaded, like the reference's one render
  * thread (src/Editor/Editor.cpp:20-23).  One context owns one HIP stream for
  * its work, plus a second one that only answers radiance-cache queries during
- * a frame's last bounces (joined before fnd_pt_render returns).
+ * a frame's last bounces (joined before fnd_pt_render returns), and a third
+ * that runs a frame's radiance-cache training after fnd_pt_render returns,
+ * overlapped with the next call until that call first reads the cache;
+ * fnd_pt_wait, every fnd_pt_cache_* call and fnd_pt_destroy wait for it.
  * Calls are synchronous unless FND_ASYNC is set, then fnd_pt_wait is required.
  */
 #ifndef FND_PT_H
