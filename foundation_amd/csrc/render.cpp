@@ -604,8 +604,12 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
         if (st != FND_OK) return st;
 
         TraceLaunch tl{};
-        tl.two_level = true;
-        tl.multi = true;
+        // A call over one scene passes its traversal constants as kernel
+        // arguments (no per-ray scene lookup at each refill, no per-lane scene
+        // registers), and a single-level scene walks without the world-ray
+        // registers of two-level scenes.
+        tl.multi = slots.size() > 1;
+        tl.two_level = tl.multi || slots[0]->two_level;
         tl.block = ctx->tuning.block_size;
         tl.stack_lds = ctx->tuning.stack_lds;
         tl.mode = kModeMixedInternal;  // the mode every launch below uses
@@ -659,6 +663,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             wa.num_jobs = njobs;
             const uint64_t wmax = (apply_cache || collect) ? uint64_t(slots) : uint64_t(slots) * kWindowSlots;
             TraceArgs ta{};
+            ta.scene = hd[0];
             ta.scenes = rs->d_scenes;
             ta.hits = rs->b.ext_hits;
             ta.counter = rs->b.trace_ctr;

@@ -32,6 +32,11 @@ static const void* kernel_cfg(const TraceLaunch& l)
         if (l.mode == kModeMixedInternal) return reinterpret_cast<const void*>(&k_trace_render<kModeMixedInternal, true, true, false, S, B>);
         return nullptr;
     }
+    if (l.mode == kModeMixedInternal) {
+        // a render of one scene: its constants are kernel arguments
+        if (l.two_level) return reinterpret_cast<const void*>(&k_trace_render<kModeMixedInternal, true, false, false, S, B>);
+        return reinterpret_cast<const void*>(&k_trace_render<kModeMixedInternal, false, false, false, S, B>);
+    }
     if (l.count) return nullptr;  // counting variants exist for the default configuration only
     if (l.two_level) {
         if (l.mode == kModeClosest) return reinterpret_cast<const void*>(&k_trace<kModeClosest, true, false, false, S, B>);
