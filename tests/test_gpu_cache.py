@@ -224,3 +224,37 @@ def test_cache_beats_plain_path_tracing_at_equal_samples(ctx):
         assert e_cache < e_plain
     finally:
         ctx.scene_destroy(scene)
+
+
+def test_next_frame_reads_the_weights_after_deferred_training(ctx, params):
+    """A frame's training runs after its image, on its own stream, and the
+    next frame waits for it where it first reads the cache.  So a cached
+    frame drawn right after a training frame must equal, bit for bit, the
+    same frame drawn with the trained weights read back and set again."""
+    mlp, grid, _, _ = params
+    g = pt.GeneratedScene(A.GEN_CORNELL_SOUP)
+    scene = ctx.scene_create(g.desc)
+    w, h = 64, 48
+    fb = ctx.framebuffer_create(w, h)
+
+    def draw(flags, offset):
+        ctx.framebuffer_clear(fb)
+        ctx.render([pt.make_job(scene, fb, g.camera, 1, 6, seed=11, sample_offset=offset, flags=flags)])
+        return ctx.framebuffer_read(fb, w, h)[0]
+
+    try:
+        configure(ctx, lo=(0, 0, 0), hi=(0, 0, 0), train_paths=1 << 14)  # bounds from the scene
+        set_params(ctx, mlp, grid)
+        draw(A.JOB_USE_CACHE | A.JOB_TRAIN_CACHE, 0)
+        after = draw(A.JOB_USE_CACHE, 1)
+        mlp1 = np.zeros_like(mlp)
+        grid1 = np.zeros_like(grid)
+        ctx._ck(ctx.lib.fnd_pt_cache_get_params(ctx.h, mlp1.ctypes.data, grid1.ctypes.data))
+        assert not np.array_equal(mlp1, mlp)
+        set_params(ctx, mlp1, grid1)
+        again = draw(A.JOB_USE_CACHE, 1)
+        assert np.all(np.isfinite(after))
+        assert np.array_equal(after.view(np.uint32), again.view(np.uint32))
+    finally:
+        ctx.framebuffer_destroy(fb)
+        ctx.scene_destroy(scene)
