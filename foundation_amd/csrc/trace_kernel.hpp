@@ -65,6 +65,10 @@ struct TraceArgs {
     const uint32_t* ray_scene2;
     const uint32_t* nrays2_dev;
     unsigned long long* ray_total;  // if set, this launch's ray count is added here (statistics)
+    // kModeMixedInternal: a launch whose two sets hold fewer rays than this
+    // returns at once, and the cooperative launch (trace_coop.hpp) queued
+    // beside it traces them instead, and the other way round (0: off)
+    uint32_t coop_below;
 };
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -88,6 +92,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args)
     const uint32_t gtid = blockIdx.x * BLOCK + tid;
     const uint32_t nrays1 = args.nrays_dev ? *args.nrays_dev : args.nrays;
     const uint32_t nrays = MIXED ? nrays1 + *args.nrays2_dev : nrays1;
+    if constexpr (MIXED) {
+        if (nrays < args.coop_below) return;
+    }
     if (args.ray_total && gtid == 0) atomicAdd(args.ray_total, (unsigned long long)nrays);
     bool lane_any = ANY_CONST;  // per lane in the mixed mode, a constant otherwise
 
@@ -405,6 +412,7 @@ struct TraceLaunch {
     bool two_level;
     bool multi;
     bool count;
+    bool coop;     // kModeMixedInternal: one wave per ray (trace_coop.hpp), for launches that hold few rays
     uint32_t block;
     uint32_t stack_lds;
     uint32_t grid;

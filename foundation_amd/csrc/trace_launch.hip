@@ -1,4 +1,5 @@
 // trace_launch.hip — instantiations of k_trace and the launcher that picks one.
+#include "trace_coop.hpp"
 #include "trace_kernel.hpp"
 
 namespace fnd {
@@ -26,6 +27,12 @@ bool trace_config_supported(uint32_t block, uint32_t stack_lds)
 template <int B, int S>
 static const void* kernel_cfg(const TraceLaunch& l)
 {
+    if (l.coop) {
+        if (l.mode != kModeMixedInternal) return nullptr;
+        if (l.multi) return reinterpret_cast<const void*>(&k_trace_coop<true, true, S, B>);
+        if (l.two_level) return reinterpret_cast<const void*>(&k_trace_coop<true, false, S, B>);
+        return reinterpret_cast<const void*>(&k_trace_coop<false, false, S, B>);
+    }
     if (l.multi) {
         if (l.mode == kModeClosestInternal) return reinterpret_cast<const void*>(&k_trace_render<kModeClosestInternal, true, true, false, S, B>);
         if (l.mode == kModeAnyInternal) return reinterpret_cast<const void*>(&k_trace_render<kModeAnyInternal, true, true, false, S, B>);
