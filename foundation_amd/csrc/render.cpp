@@ -633,10 +633,12 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             wa.b.t_cap = cs->sample_cap;
         }
         const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
-        // Launches with fewer rays than coop_rays trace with one wave per ray
-        // (trace_coop.hpp): kCoopRaysPerWave per wave of the cooperative grid.
-        // Measured on 1080p frames of the 1.5M-triangle scene: 2, 4 and 6 rays
-        // per wave gave 5.06, 5.01 and 5.00 ms per plain frame (5.38 without).
+        // Launches with fewer rays than coop_rays trace with a group of lanes
+        // per ray (trace_coop.hpp): kCoopRaysPerWave per wave of the
+        // cooperative grid.  Measured on 1080p frames of the 1.5M-triangle
+        // scene: 6, 12, 24 and 48 rays per wave gave 4.92, 5.01, 5.06 and 6.52
+        // ms per plain frame (5.42 without); one wave per ray with 2, 4 and 6
+        // gave 5.06, 5.01 and 5.00.
         constexpr uint32_t kCoopRaysPerWave = 6;
         uint32_t coop_grid = 0, coop_rays = 0;
         {
