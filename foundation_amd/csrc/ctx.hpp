@@ -130,6 +130,9 @@ struct fnd_pt_ctx {
     unsigned long long* d_stats = nullptr;     // 8 counters
     uint2* d_spill = nullptr;
     uint64_t spill_bytes = 0;
+    uint32_t* d_order = nullptr;               // a ray-query batch's ray indices in spatial order (ray_order.hip)
+    uint64_t order_bytes = 0;
+    void* d_order_scratch = nullptr;           // its cell counters (ray_order_scratch_bytes())
     void* d_tmp = nullptr;                     // host-array trace staging
     uint64_t tmp_bytes = 0;
     void* frame_memory = nullptr;              // caller's block for the per-frame queues, or null
@@ -149,6 +152,15 @@ fnd_status dev_alloc(fnd_pt_ctx* ctx, void** ptr, uint64_t bytes);
 fnd_status copy_sync(fnd_pt_ctx* ctx, void* dst, const void* src, uint64_t bytes, hipMemcpyKind kind, const char* what);
 void dev_free(fnd_pt_ctx* ctx, void* ptr, uint64_t bytes);
 fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_lds, uint32_t threads);
+// ray_order.hip: the order in which a ray-query launch takes its rays, by the
+// cells of a coarse Morton grid over [lo, hi] that hold their origins.
+constexpr uint32_t kOrderMinRays = 131072;  // smaller batches are traced in the caller's order
+uint64_t ray_order_scratch_bytes();
+// The word in the scratch that says whether the order is used (1) or the
+// caller's (0): set on the device by launch_ray_order.
+const uint32_t* ray_order_flag(const void* scratch);
+hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const float* hi, void* scratch,
+                            uint32_t* order, uint32_t blocks, hipStream_t s);
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
 Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
 Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h);

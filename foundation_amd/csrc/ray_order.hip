@@ -1,0 +1,187 @@
+// ray_order.hip — the spatial order in which a ray-query launch takes its rays.
+//
+// A trace launch keeps about 390k rays in flight, a sliding window over the
+// ray array.  Incoherent rays in the caller's order (say, pixel order of the
+// bounces they come from) spread that window over much of the scene, and the
+// walks then share few nodes in L2 and the Infinity Cache.  Taking the rays
+// in the order of their origins' cells of a coarse Morton grid over the
+// scene's bounds (8 x 8 x 8 cells) keeps the window in a compact region: on
+// the 10.08M-triangle incoherent set the traversal measured 0.589 ms against
+// 0.684 ms in pixel order, and finer keys (12 to 30 bits) did no better.
+// Only the order in which rays are taken changes; every ray's record is its
+// own, so the hits are the same bits.
+//
+// A stable counting sort of the ray indices, in three launches: count the
+// rays per cell (a block's counts in LDS, then one atomic per cell and
+// block), scan the 512 counts, and place each block's rays at its cells'
+// cursors (one atomic per cell and block; within the block each wave places
+// its contiguous part in index order, 64 rays at a time, one pass per
+// distinct cell among them).  Within a cell the rays keep the caller's order,
+// so a batch that is already coherent stays so; and when one cell holds more
+// than half the rays (camera rays from one eye, say), the order is not used
+// at all (the scan sets a flag the traversal reads).  Rays with a non-finite
+// origin fall in a border cell.
+#include <algorithm>
+
+#include "ctx.hpp"
+#include "pt_layout.hpp"
+
+namespace fnd {
+
+namespace {
+
+constexpr uint32_t kCellBits = 3;                        // per axis
+constexpr uint32_t kCells = 1u << (3 * kCellBits);        // 512
+constexpr uint32_t kCellStride = 64;                      // words between cell counters: one 256-byte line each
+constexpr uint32_t kOrderBlock = 256;
+
+struct OrderArgs {
+    const float4* rays;  // two float4 per ray: origin + tmin, direction + tmax
+    uint32_t n;
+    float lo[3];
+    float scale[3];      // cells per unit length
+    uint32_t* count;     // kCells counters, kCellStride words apart
+    uint32_t* cursor;    // the same layout: each cell's next free position; word 1: use the order (0 or 1)
+    uint32_t* order;     // n ray indices
+};
+
+constexpr uint32_t kOrderWaves = kOrderBlock / 64;
+
+__device__ __forceinline__ uint32_t spread3(uint32_t x)
+{
+    // bits b of x to bit 3b (x < 8)
+    return (x & 1u) | ((x & 2u) << 2) | ((x & 4u) << 4);
+}
+
+__device__ __forceinline__ uint32_t cell_of(const OrderArgs& a, uint32_t i)
+{
+    const float4 o = a.rays[2ull * i];
+    uint32_t q[3];
+    const float v[3] = {o.x, o.y, o.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        // fmaxf drops a NaN, so a non-finite origin lands in a border cell
+        const float c = fminf(fmaxf((v[k] - a.lo[k]) * a.scale[k], 0.0f), float((1u << kCellBits) - 1u));
+        q[k] = uint32_t(c);
+    }
+    return spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+}
+
+// The block's contiguous share of the rays.
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& begin, uint32_t& end)
+{
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    begin = min(n, blockIdx.x * per);
+    end = min(n, begin + per);
+}
+
+__global__ __launch_bounds__(kOrderBlock) void k_order_count(OrderArgs a)
+{
+    __shared__ uint32_t h[kCells];
+    for (uint32_t c = threadIdx.x; c < kCells; c += kOrderBlock) h[c] = 0u;
+    __syncthreads();
+    uint32_t b, e;
+    block_range(a.n, b, e);
+    for (uint32_t i = b + threadIdx.x; i < e; i += kOrderBlock) atomicAdd(&h[cell_of(a, i)], 1u);
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < kCells; c += kOrderBlock)
+        if (h[c]) atomicAdd(&a.count[c * kCellStride], h[c]);
+}
+
+// One block of kCells threads: cursor = exclusive prefix sum of the counts,
+// and whether the order is worth using (no cell holds more than half the rays).
+__global__ __launch_bounds__(kCells) void k_order_scan(OrderArgs a)
+{
+    __shared__ uint32_t s[kCells];
+    __shared__ uint32_t big;
+    const uint32_t t = threadIdx.x;
+    const uint32_t c = a.count[t * kCellStride];
+    if (t == 0) big = 0u;
+    s[t] = c;
+    __syncthreads();
+    if (2ull * c > a.n) big = 1u;
+    for (uint32_t d = 1; d < kCells; d <<= 1) {
+        const uint32_t v = t >= d ? s[t - d] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    a.cursor[t * kCellStride] = s[t] - c;
+    a.count[t * kCellStride] = 0u;  // zero for the next batch (the scratch starts zeroed)
+    if (t == 0) a.cursor[1] = big ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(kOrderBlock) void k_order_place(OrderArgs a)
+{
+    if (a.cursor[1] == 0u) return;  // the traversal takes the caller's order
+    __shared__ uint32_t wc[kOrderWaves][kCells];  // per wave: its count per cell, then its next position
+    const uint32_t w = threadIdx.x / 64u, lane = threadIdx.x % 64u;
+    for (uint32_t c = threadIdx.x; c < kCells; c += kOrderBlock)
+        for (uint32_t v = 0; v < kOrderWaves; ++v) wc[v][c] = 0u;
+    __syncthreads();
+    uint32_t b, e;
+    block_range(a.n, b, e);
+    const uint32_t per = (e - b + kOrderWaves - 1) / kOrderWaves;
+    const uint32_t wb = min(e, b + w * per), we = min(e, wb + per);
+    for (uint32_t i = wb + lane; i < we; i += 64u) atomicAdd(&wc[w][cell_of(a, i)], 1u);
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < kCells; c += kOrderBlock) {
+        uint32_t total = 0;
+        for (uint32_t v = 0; v < kOrderWaves; ++v) total += wc[v][c];
+        uint32_t pos = total ? atomicAdd(&a.cursor[c * kCellStride], total) : 0u;
+        for (uint32_t v = 0; v < kOrderWaves; ++v) {
+            const uint32_t t = wc[v][c];
+            wc[v][c] = pos;
+            pos += t;
+        }
+    }
+    __syncthreads();
+    // the wave's rays in index order, 64 at a time: one pass per distinct cell
+    for (uint32_t r = wb; r < we; r += 64u) {
+        const uint32_t i = r + lane;
+        const bool valid = i < we;
+        const uint32_t c = valid ? cell_of(a, i) : 0u;
+        uint64_t todo = __ballot(valid);
+        while (todo != 0) {
+            const int leader = __builtin_ctzll(todo);
+            const uint32_t cl = uint32_t(__builtin_amdgcn_readlane(int(c), leader));
+            const uint64_t m = __ballot(valid && c == cl);
+            const uint32_t pos = wc[w][cl];
+            if (valid && c == cl)
+                a.order[pos + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u))] = i;
+            if (int(lane) == leader) wc[w][cl] = pos + uint32_t(__popcll(m));
+            // the next pass (or round) reads this wave's counters again
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            todo &= ~m;
+        }
+    }
+}
+
+}  // namespace
+
+uint64_t ray_order_scratch_bytes() { return 2ull * kCells * kCellStride * sizeof(uint32_t); }
+
+const uint32_t* ray_order_flag(const void* scratch) { return static_cast<const uint32_t*>(scratch) + kCells * kCellStride + 1; }
+
+hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const float* hi, void* scratch,
+                            uint32_t* order, uint32_t blocks, hipStream_t s)
+{
+    OrderArgs a{};
+    a.rays = reinterpret_cast<const float4*>(rays);
+    a.n = n;
+    for (int k = 0; k < 3; ++k) {
+        const float ext = hi[k] - lo[k];
+        a.lo[k] = lo[k];
+        a.scale[k] = ext > 0.0f ? float(1u << kCellBits) / ext : 0.0f;
+    }
+    a.count = static_cast<uint32_t*>(scratch);
+    a.cursor = a.count + kCells * kCellStride;
+    a.order = order;
+    blocks = std::max(1u, std::min(blocks, (n + kOrderBlock - 1) / kOrderBlock));
+    hipLaunchKernelGGL(k_order_count, dim3(blocks), dim3(kOrderBlock), 0, s, a);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kCells), 0, s, a);
+    hipLaunchKernelGGL(k_order_place, dim3(blocks), dim3(kOrderBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace fnd

@@ -69,6 +69,11 @@ struct TraceArgs {
     // returns at once, and the cooperative launch (trace_coop.hpp) queued
     // beside it traces them instead, and the other way round (0: off)
     uint32_t coop_below;
+    // kModeClosest / kModeAny: if set and *ray_order_use is nonzero, the
+    // launch's i-th ray is ray_order[i] (records still go to hits[ray]); a
+    // spatial order of the batch
+    const uint32_t* ray_order;
+    const uint32_t* ray_order_use;
 };
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
@@ -96,6 +101,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args)
         if (nrays < args.coop_below) return;
     }
     if (args.ray_total && gtid == 0) atomicAdd(args.ray_total, (unsigned long long)nrays);
+    const bool use_order = (MODE == kModeClosest || MODE == kModeAny) && args.ray_order && *args.ray_order_use != 0u;
     bool lane_any = ANY_CONST;  // per lane in the mixed mode, a constant otherwise
 
     SceneDev sc = args.scene;
@@ -160,6 +166,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& args)
                     const bool second = MIXED && idx >= nrays1;
                     ray_idx = second ? idx - nrays1 : idx;
                     if (MIXED) lane_any = second;
+                    if constexpr (MODE == kModeClosest || MODE == kModeAny) {
+                        if (use_order) ray_idx = args.ray_order[ray_idx];
+                    }
                     const float4* rp = reinterpret_cast<const float4*>((second ? args.rays2 : args.rays) + ray_idx);
                     const float4 ra = rp[0];
                     const float4 rb = rp[1];

@@ -180,3 +180,28 @@ def test_device_buffers_path(ctx, cornell):
     finally:
         ctx.buffer_destroy(rb)
         ctx.buffer_destroy(hb)
+
+
+def test_spatially_ordered_batches_match_oracle(ctx, instanced):
+    """Batches of at least 131072 rays are taken in the order of their
+    origins' cells of a coarse grid over the scene (ray_order.hip); every
+    record must still be the ray's own.  Incoherent rays (bounces, with
+    refused rays and origins far outside the scene's bounds mixed in) use the
+    order; camera rays from one eye fall in one cell and keep the caller's
+    order.  Closest-hit records bit for bit, any-hit occlusion."""
+    gen, scene, ref = instanced
+    _, bounce, _ = _ray_sets(gen, ref, 256, 192, [0, 0, 9])
+    reps = -(-140000 // bounce.shape[0])
+    rays = np.concatenate([bounce] * reps)
+    rng = np.random.default_rng(5)
+    idx = rng.choice(rays.shape[0], 2000, replace=False)
+    rays[idx[:500], 0] = np.nan                 # refused
+    rays[idx[500:1000], 0:3] += 1e4             # far outside the bounds: border cells
+    rays[idx[1000:1500], 2] = -np.inf           # refused
+    rays[idx[1500:], 4:7] *= -1.0
+    assert rays.shape[0] >= 131072
+    _assert_same(ctx.trace(scene, rays), ref.trace(rays), "ordered incoherent batch")
+    assert np.array_equal(_occluded(ctx.trace(scene, rays, anyhit=True)), _occluded(ref.trace(rays, anyhit=True)))
+    camera = R.primary_rays(gen.camera, 512, 288)
+    assert camera.shape[0] >= 131072
+    _assert_same(ctx.trace(scene, camera), ref.trace(camera), "camera batch")
