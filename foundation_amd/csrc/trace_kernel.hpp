@@ -71,7 +71,9 @@ struct TraceArgs {
     uint32_t coop_below;
     // kModeClosest / kModeAny: if set and *ray_order_use is nonzero, the
     // launch's i-th ray is ray_order[i] (records still go to hits[ray]); a
-    // spatial order of the batch
+    // spatial order of the batch.  (The renderer's extension rays, in tile
+    // order, measured slower ordered: 5.10 against 4.92 ms per plain 1080p
+    // frame, 663 against 687 Msamples/s for the 64-spp render.)
     const uint32_t* ray_order;
     const uint32_t* ray_order_use;
 };
