@@ -679,9 +679,11 @@ fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4*
         a.ray_order = ctx->d_order;
         a.ray_order_use = ray_order_flag(ctx->d_order_scratch);
     }
-    st = check_hip(ctx, hipMemsetAsync(ctx->d_counters, 0, kCounterWords * sizeof(uint32_t), ctx->stream),
-                   "reset counter");
-    if (st != FND_OK) return st;
+    if (!order) {  // the ordering launches clear the work counter themselves
+        st = check_hip(ctx, hipMemsetAsync(ctx->d_counters, 0, kCounterWords * sizeof(uint32_t), ctx->stream),
+                       "reset counter");
+        if (st != FND_OK) return st;
+    }
     if (count) {
         st = check_hip(ctx, hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream), "reset stats");
         if (st != FND_OK) return st;
@@ -694,7 +696,7 @@ fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4*
         // two blocks per CU: same-box runs of the 10M-triangle incoherent set
         // gave 0.622-0.626, 0.609 and 0.618 ms per call for 1, 2 and 4
         st = check_hip(ctx, launch_ray_order(rays, n, s->info.bounds_lo, s->info.bounds_hi, ctx->d_order_scratch,
-                                             ctx->d_order, 2u * ctx->num_cus, ctx->stream),
+                                             ctx->d_order, ctx->d_counters, 2u * ctx->num_cus, ctx->stream),
                        "ray order");
         if (st != FND_OK) return st;
     }

@@ -159,8 +159,10 @@ uint64_t ray_order_scratch_bytes();
 // The word in the scratch that says whether the order is used (1) or the
 // caller's (0): set on the device by launch_ray_order.
 const uint32_t* ray_order_flag(const void* scratch);
+// It also clears the traversal's work counter (kCounterWords) for the trace
+// launch that follows.
 hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const float* hi, void* scratch,
-                            uint32_t* order, uint32_t blocks, hipStream_t s);
+                            uint32_t* order, uint32_t* trace_counter, uint32_t blocks, hipStream_t s);
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
 Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
 Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h);

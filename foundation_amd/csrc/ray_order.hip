@@ -13,8 +13,8 @@
 //
 // A stable counting sort of the ray indices, in three launches: count the
 // rays per cell (a block's counts in LDS, then one atomic per cell and
-// block), scan the 512 counts, and place each block's rays at its cells'
-// cursors (one atomic per cell and block; within the block each wave places
+// block), scan the 512 counts (and clear the traversal's work counter), and
+// place each block's rays at its cells' cursors (one atomic per cell and block; within the block each wave places
 // its contiguous part in index order, 64 rays at a time, one pass per
 // distinct cell among them).  Within a cell the rays keep the caller's order,
 // so a batch that is already coherent stays so; and when one cell holds more
@@ -24,6 +24,7 @@
 #include <algorithm>
 
 #include "ctx.hpp"
+#include "pt_device.hpp"
 #include "pt_layout.hpp"
 
 namespace fnd {
@@ -43,6 +44,7 @@ struct OrderArgs {
     uint32_t* count;     // kCells counters, kCellStride words apart
     uint32_t* cursor;    // the same layout: each cell's next free position; word 1: use the order (0 or 1)
     uint32_t* order;     // n ray indices
+    uint32_t* trace_counter;  // the traversal's work counter (kCounterWords), cleared by the scan
 };
 
 constexpr uint32_t kOrderWaves = kOrderBlock / 64;
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(kCells) void k_order_scan(OrderArgs a)
     }
     a.cursor[t * kCellStride] = s[t] - c;
     a.count[t * kCellStride] = 0u;  // zero for the next batch (the scratch starts zeroed)
+    zero_trace_counter(a.trace_counter, t);
     if (t == 0) a.cursor[1] = big ? 0u : 1u;
 }
 
@@ -164,7 +167,7 @@ uint64_t ray_order_scratch_bytes() { return 2ull * kCells * kCellStride * sizeof
 const uint32_t* ray_order_flag(const void* scratch) { return static_cast<const uint32_t*>(scratch) + kCells * kCellStride + 1; }
 
 hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const float* hi, void* scratch,
-                            uint32_t* order, uint32_t blocks, hipStream_t s)
+                            uint32_t* order, uint32_t* trace_counter, uint32_t blocks, hipStream_t s)
 {
     OrderArgs a{};
     a.rays = reinterpret_cast<const float4*>(rays);
@@ -177,6 +180,7 @@ hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const 
     a.count = static_cast<uint32_t*>(scratch);
     a.cursor = a.count + kCells * kCellStride;
     a.order = order;
+    a.trace_counter = trace_counter;
     blocks = std::max(1u, std::min(blocks, (n + kOrderBlock - 1) / kOrderBlock));
     hipLaunchKernelGGL(k_order_count, dim3(blocks), dim3(kOrderBlock), 0, s, a);
     hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kCells), 0, s, a);
