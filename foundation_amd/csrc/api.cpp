@@ -91,6 +91,47 @@ fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_
     return FND_OK;
 }
 
+// Buffers and events of ray-order slot k for a batch of n rays.
+fnd_status ensure_order_slot(fnd_pt_ctx* ctx, uint32_t k, uint64_t n)
+{
+    fnd_status st = FND_OK;
+    if (!ctx->order_ready[k]) {
+        st = check_hip(ctx, hipEventCreateWithFlags(&ctx->order_ready[k], hipEventDisableTiming), "hipEventCreate");
+        if (st == FND_OK)
+            st = check_hip(ctx, hipEventCreateWithFlags(&ctx->order_free[k], hipEventDisableTiming), "hipEventCreate");
+        if (st != FND_OK) return st;
+    }
+    if (!ctx->d_order_scratch[k]) {
+        void* p = nullptr;
+        st = dev_alloc(ctx, &p, ray_order_scratch_bytes());
+        if (st != FND_OK) return st;
+        ctx->d_order_scratch[k] = p;
+        // the counters start at zero; each batch's scan zeroes them again
+        st = check_hip(ctx, hipMemsetAsync(p, 0, ray_order_scratch_bytes(), ctx->stream), "ray order");
+        if (st != FND_OK) return st;
+        st = dev_alloc(ctx, &p, kCounterWords * sizeof(uint32_t));
+        if (st != FND_OK) return st;
+        ctx->d_order_counter[k] = static_cast<uint32_t*>(p);
+        // the side stream uses the scratch next
+        st = check_hip(ctx, hipStreamSynchronize(ctx->stream), "ray order");
+        if (st != FND_OK) return st;
+    }
+    const uint64_t need = n * (sizeof(uint32_t) + sizeof(uint16_t));
+    if (need <= ctx->order_bytes[k]) return FND_OK;
+    // a traversal still in flight may read the old order
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
+    dev_free(ctx, ctx->d_order[k], ctx->order_bytes[k]);
+    ctx->d_order[k] = nullptr;
+    ctx->order_bytes[k] = 0;
+    void* p = nullptr;
+    st = dev_alloc(ctx, &p, need);
+    if (st != FND_OK) return st;
+    ctx->d_order[k] = static_cast<uint32_t*>(p);
+    ctx->order_bytes[k] = need;
+    return FND_OK;
+}
+
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h)
 {
     if (handle_kind(h) != kHandleScene) return nullptr;
@@ -324,8 +365,14 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
     dev_free(ctx, ctx->d_present, 255 * sizeof(float));
     dev_free(ctx, ctx->d_stats, 8 * sizeof(unsigned long long));
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);
-    dev_free(ctx, ctx->d_order, ctx->order_bytes);
-    dev_free(ctx, ctx->d_order_scratch, ray_order_scratch_bytes());
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    for (int k = 0; k < 2; ++k) {
+        dev_free(ctx, ctx->d_order[k], ctx->order_bytes[k]);
+        dev_free(ctx, ctx->d_order_scratch[k], ray_order_scratch_bytes());
+        dev_free(ctx, ctx->d_order_counter[k], kCounterWords * sizeof(uint32_t));
+        if (ctx->order_ready[k]) (void)hipEventDestroy(ctx->order_ready[k]);
+        if (ctx->order_free[k]) (void)hipEventDestroy(ctx->order_free[k]);
+    }
     dev_free(ctx, ctx->d_tmp, ctx->tmp_bytes);
     for (auto& ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -625,7 +672,8 @@ fnd_status fnd_pt_buffer_read(fnd_pt_ctx* ctx, fnd_handle h, uint64_t offset, vo
 
 }  // extern "C"
 
-fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags)
+fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags,
+                             bool staged)
 {
     const bool any = (flags & FND_TRACE_ANYHIT) != 0;
     const bool count = (flags & FND_TRACE_COUNT) != 0;
@@ -653,31 +701,14 @@ fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4*
     a.stats = ctx->d_stats;
     // counting launches describe the walks in the caller's order
     const bool order = !count && n >= kOrderMinRays;
+    const uint32_t slot = ctx->order_slot;
     if (order) {
-        if (!ctx->d_order_scratch) {
-            void* p = nullptr;
-            st = dev_alloc(ctx, &p, ray_order_scratch_bytes());
-            if (st != FND_OK) return st;
-            ctx->d_order_scratch = p;
-            // the counters start at zero; each batch's scan zeroes them again
-            st = check_hip(ctx, hipMemsetAsync(p, 0, ray_order_scratch_bytes(), ctx->stream), "ray order");
-            if (st != FND_OK) return st;
-        }
-        const uint64_t need = uint64_t(n) * sizeof(uint32_t);
-        if (need > ctx->order_bytes) {
-            // an FND_ASYNC trace still in flight may read the old order
-            (void)hipStreamSynchronize(ctx->stream);
-            dev_free(ctx, ctx->d_order, ctx->order_bytes);
-            ctx->d_order = nullptr;
-            ctx->order_bytes = 0;
-            void* p = nullptr;
-            st = dev_alloc(ctx, &p, need);
-            if (st != FND_OK) return st;
-            ctx->d_order = static_cast<uint32_t*>(p);
-            ctx->order_bytes = need;
-        }
-        a.ray_order = ctx->d_order;
-        a.ray_order_use = ray_order_flag(ctx->d_order_scratch);
+        st = ensure_order_slot(ctx, slot, n);
+        if (st != FND_OK) return st;
+        a.ray_order = ctx->d_order[slot];
+        a.ray_order_use = ray_order_flag(ctx->d_order_scratch[slot]);
+        a.counter = ctx->d_order_counter[slot];
+        ctx->order_slot ^= 1u;
     }
     if (!order) {  // the ordering launches clear the work counter themselves
         st = check_hip(ctx, hipMemsetAsync(ctx->d_counters, 0, kCounterWords * sizeof(uint32_t), ctx->stream),
@@ -691,18 +722,36 @@ fnd_status fnd::trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4*
     hipEvent_t e0, e1;
     st = timing_pair(ctx, &e0, &e1);
     if (st != FND_OK) return st;
-    (void)hipEventRecord(e0, ctx->stream);
     if (order) {
-        // two blocks per CU: same-box runs of the 10M-triangle incoherent set
-        // gave 0.622-0.626, 0.609 and 0.618 ms per call for 1, 2 and 4
-        st = check_hip(ctx, launch_ray_order(rays, n, s->info.bounds_lo, s->info.bounds_hi, ctx->d_order_scratch,
-                                             ctx->d_order, ctx->d_counters, 2u * ctx->num_cus, ctx->stream),
-                       "ray order");
+        // The order is made on the side stream, once the traversal that last
+        // used this slot is done, so it overlaps the previous batch's traversal
+        // (whose drain leaves most CUs idle).  Two blocks per CU: same-box runs
+        // of the 10M-triangle incoherent set gave 0.622-0.626, 0.609 and 0.618
+        // ms per call for 1, 2 and 4.
+        st = check_hip(ctx, hipStreamWaitEvent(ctx->side, ctx->order_free[slot], 0), "ray order");
+        if (st == FND_OK && staged) {
+            // the rays were uploaded on the main stream just now: the side
+            // stream reads them after the upload
+            st = check_hip(ctx, hipEventRecord(ctx->order_ready[slot], ctx->stream), "ray order");
+            if (st == FND_OK) st = check_hip(ctx, hipStreamWaitEvent(ctx->side, ctx->order_ready[slot], 0), "ray order");
+        }
+        if (st == FND_OK)
+            st = check_hip(ctx, launch_ray_order(rays, n, s->info.bounds_lo, s->info.bounds_hi, ctx->d_order_scratch[slot],
+                                                 ctx->d_order[slot], reinterpret_cast<uint16_t*>(ctx->d_order[slot] + n),
+                                                 ctx->d_order_counter[slot], 2u * ctx->num_cus, ctx->side),
+                           "ray order");
+        if (st == FND_OK) st = check_hip(ctx, hipEventRecord(ctx->order_ready[slot], ctx->side), "ray order");
+        if (st == FND_OK) st = check_hip(ctx, hipStreamWaitEvent(ctx->stream, ctx->order_ready[slot], 0), "ray order");
         if (st != FND_OK) return st;
     }
+    (void)hipEventRecord(e0, ctx->stream);
     st = check_hip(ctx, launch_trace(l, a, ctx->stream), "trace launch");
     if (st != FND_OK) return st;
     (void)hipEventRecord(e1, ctx->stream);
+    if (order) {
+        st = check_hip(ctx, hipEventRecord(ctx->order_free[slot], ctx->stream), "ray order");
+        if (st != FND_OK) return st;
+    }
     ctx->stats.last_grid = l.grid;
     ctx->stats.rays_traced += n;
     if (flags & FND_ASYNC) return FND_OK;
@@ -822,7 +871,7 @@ fnd_status fnd_pt_trace(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, 
         fnd_status st = check_hip(ctx, hipMemcpyAsync(dr, rays, uint64_t(n) * sizeof(Ray), hipMemcpyHostToDevice, ctx->stream),
                                   "upload rays");
         if (st != FND_OK) return st;
-        st = trace_device(ctx, s, dr, dh, n, flags);
+        st = trace_device(ctx, s, dr, dh, n, flags, true);
         if (st != FND_OK) return st;
         st = check_hip(ctx, hipMemcpyAsync(hits, dh, uint64_t(n) * sizeof(Hit), hipMemcpyDeviceToHost, ctx->stream),
                        "download hits");

@@ -130,9 +130,19 @@ struct fnd_pt_ctx {
     unsigned long long* d_stats = nullptr;     // 8 counters
     uint2* d_spill = nullptr;
     uint64_t spill_bytes = 0;
-    uint32_t* d_order = nullptr;               // a ray-query batch's ray indices in spatial order (ray_order.hip)
-    uint64_t order_bytes = 0;
-    void* d_order_scratch = nullptr;           // its cell counters (ray_order_scratch_bytes())
+    // Ray-query batches in spatial order (ray_order.hip), two slots used in
+    // turn: a batch's order is made on the side stream while the previous
+    // batch's traversal runs on the main stream.  Per slot: the ray indices
+    // and, after them, each ray's cell (uint16); the cell counters
+    // (ray_order_scratch_bytes()); the traversal's work counter
+    // (kCounterWords); and events for "order made" and "traversal done".
+    uint32_t* d_order[2] = {nullptr, nullptr};
+    uint64_t order_bytes[2] = {0, 0};
+    void* d_order_scratch[2] = {nullptr, nullptr};
+    uint32_t* d_order_counter[2] = {nullptr, nullptr};
+    hipEvent_t order_ready[2] = {nullptr, nullptr};
+    hipEvent_t order_free[2] = {nullptr, nullptr};
+    uint32_t order_slot = 0;
     void* d_tmp = nullptr;                     // host-array trace staging
     uint64_t tmp_bytes = 0;
     void* frame_memory = nullptr;              // caller's block for the per-frame queues, or null
@@ -156,19 +166,22 @@ fnd_status ensure_spill(fnd_pt_ctx* ctx, uint32_t stack_entries, uint32_t stack_
 // cells of a coarse Morton grid over [lo, hi] that hold their origins.
 constexpr uint32_t kOrderMinRays = 131072;  // smaller batches are traced in the caller's order
 uint64_t ray_order_scratch_bytes();
+fnd_status ensure_order_slot(fnd_pt_ctx* ctx, uint32_t k, uint64_t n);
 // The word in the scratch that says whether the order is used (1) or the
 // caller's (0): set on the device by launch_ray_order.
 const uint32_t* ray_order_flag(const void* scratch);
 // It also clears the traversal's work counter (kCounterWords) for the trace
 // launch that follows.
 hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const float* hi, void* scratch,
-                            uint32_t* order, uint32_t* trace_counter, uint32_t blocks, hipStream_t s);
+                            uint32_t* order, uint16_t* cells, uint32_t* trace_counter, uint32_t blocks, hipStream_t s);
 Scene* find_scene(fnd_pt_ctx* ctx, fnd_handle h);
 Framebuffer* find_framebuffer(fnd_pt_ctx* ctx, fnd_handle h);
 Buffer* find_buffer(fnd_pt_ctx* ctx, fnd_handle h);
 // One trace launch over device-resident rays and hits (FND_TRACE_* flags;
 // with FND_ASYNC it returns once the launch is queued).
-fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags);
+// staged: the rays were uploaded on the context's stream within this call.
+fnd_status trace_device(fnd_pt_ctx* ctx, Scene* s, const Ray* rays, float4* hits, uint32_t n, uint32_t flags,
+                        bool staged = false);
 // Queues the R8G8B8A8 conversion of a framebuffer's mean into device memory
 // (`pitch` in pixels) on the context's stream; no wait.
 fnd_status present_async(fnd_pt_ctx* ctx, Framebuffer* f, float exposure, uint32_t* out, uint32_t pitch);

@@ -14,9 +14,12 @@
 // A stable counting sort of the ray indices, in three launches: count the
 // rays per cell (a block's counts in LDS, then one atomic per cell and
 // block), scan the 512 counts (and clear the traversal's work counter), and
-// place each block's rays at its cells' cursors (one atomic per cell and block; within the block each wave places
-// its contiguous part in index order, 64 rays at a time, one pass per
-// distinct cell among them).  Within a cell the rays keep the caller's order,
+// place each block's rays at its cells' cursors (one atomic per cell and
+// block; within the block each wave places its contiguous part in index
+// order, 64 rays at a time, one pass per distinct cell among them).  The
+// placing launch reads the cells the count launch wrote, not the rays again,
+// so the result is a permutation of the batch whatever the rays hold
+// meanwhile.  Within a cell the rays keep the caller's order,
 // so a batch that is already coherent stays so; and when one cell holds more
 // than half the rays (camera rays from one eye, say), the order is not used
 // at all (the scan sets a flag the traversal reads).  Rays with a non-finite
@@ -44,6 +47,7 @@ struct OrderArgs {
     uint32_t* count;     // kCells counters, kCellStride words apart
     uint32_t* cursor;    // the same layout: each cell's next free position; word 1: use the order (0 or 1)
     uint32_t* order;     // n ray indices
+    uint16_t* cells;     // n: each ray's cell, as the count launch found it
     uint32_t* trace_counter;  // the traversal's work counter (kCounterWords), cleared by the scan
 };
 
@@ -84,34 +88,50 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_count(OrderArgs a)
     __syncthreads();
     uint32_t b, e;
     block_range(a.n, b, e);
-    for (uint32_t i = b + threadIdx.x; i < e; i += kOrderBlock) atomicAdd(&h[cell_of(a, i)], 1u);
+    for (uint32_t i = b + threadIdx.x; i < e; i += kOrderBlock) {
+        const uint32_t c = cell_of(a, i);
+        a.cells[i] = uint16_t(c);
+        atomicAdd(&h[c], 1u);
+    }
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < kCells; c += kOrderBlock)
         if (h[c]) atomicAdd(&a.count[c * kCellStride], h[c]);
 }
 
-// One block of kCells threads: cursor = exclusive prefix sum of the counts,
-// and whether the order is worth using (no cell holds more than half the rays).
-__global__ __launch_bounds__(kCells) void k_order_scan(OrderArgs a)
+// One wave: cursor = exclusive prefix sum of the counts, and whether the
+// order is worth using (no cell holds more than half the rays).  A single
+// wave finds a slot beside a running traversal's persistent grid, where a
+// block of 512 threads waited for its drain.
+__global__ __launch_bounds__(64) void k_order_scan(OrderArgs a)
 {
-    __shared__ uint32_t s[kCells];
-    __shared__ uint32_t big;
+    constexpr uint32_t kPer = kCells / 64u;  // cells per lane, consecutive
     const uint32_t t = threadIdx.x;
-    const uint32_t c = a.count[t * kCellStride];
-    if (t == 0) big = 0u;
-    s[t] = c;
-    __syncthreads();
-    if (2ull * c > a.n) big = 1u;
-    for (uint32_t d = 1; d < kCells; d <<= 1) {
-        const uint32_t v = t >= d ? s[t - d] : 0u;
-        __syncthreads();
-        s[t] += v;
-        __syncthreads();
+    uint32_t c[kPer];
+    uint32_t sum = 0;
+    bool big = false;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        c[k] = a.count[(t * kPer + k) * kCellStride];
+        sum += c[k];
+        big = big || 2ull * c[k] > a.n;
     }
-    a.cursor[t * kCellStride] = s[t] - c;
-    a.count[t * kCellStride] = 0u;  // zero for the next batch (the scratch starts zeroed)
+    // inclusive scan of the lanes' sums over the wave
+    uint32_t inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = uint32_t(__shfl_up(int(inc), d));
+        if (int(t) >= d) inc += v;
+    }
+    uint32_t pos = inc - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        a.cursor[(t * kPer + k) * kCellStride] = pos;
+        pos += c[k];
+        a.count[(t * kPer + k) * kCellStride] = 0u;  // zero for the next batch (the scratch starts zeroed)
+    }
     zero_trace_counter(a.trace_counter, t);
-    if (t == 0) a.cursor[1] = big ? 0u : 1u;
+    const bool any_big = __ballot(big) != 0;
+    if (t == 0) a.cursor[1] = any_big ? 0u : 1u;
 }
 
 __global__ __launch_bounds__(kOrderBlock) void k_order_place(OrderArgs a)
@@ -126,7 +146,7 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_place(OrderArgs a)
     block_range(a.n, b, e);
     const uint32_t per = (e - b + kOrderWaves - 1) / kOrderWaves;
     const uint32_t wb = min(e, b + w * per), we = min(e, wb + per);
-    for (uint32_t i = wb + lane; i < we; i += 64u) atomicAdd(&wc[w][cell_of(a, i)], 1u);
+    for (uint32_t i = wb + lane; i < we; i += 64u) atomicAdd(&wc[w][a.cells[i]], 1u);
     __syncthreads();
     for (uint32_t c = threadIdx.x; c < kCells; c += kOrderBlock) {
         uint32_t total = 0;
@@ -143,7 +163,7 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_place(OrderArgs a)
     for (uint32_t r = wb; r < we; r += 64u) {
         const uint32_t i = r + lane;
         const bool valid = i < we;
-        const uint32_t c = valid ? cell_of(a, i) : 0u;
+        const uint32_t c = valid ? uint32_t(a.cells[i]) : 0u;
         uint64_t todo = __ballot(valid);
         while (todo != 0) {
             const int leader = __builtin_ctzll(todo);
@@ -167,7 +187,7 @@ uint64_t ray_order_scratch_bytes() { return 2ull * kCells * kCellStride * sizeof
 const uint32_t* ray_order_flag(const void* scratch) { return static_cast<const uint32_t*>(scratch) + kCells * kCellStride + 1; }
 
 hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const float* hi, void* scratch,
-                            uint32_t* order, uint32_t* trace_counter, uint32_t blocks, hipStream_t s)
+                            uint32_t* order, uint16_t* cells, uint32_t* trace_counter, uint32_t blocks, hipStream_t s)
 {
     OrderArgs a{};
     a.rays = reinterpret_cast<const float4*>(rays);
@@ -180,10 +200,11 @@ hipError_t launch_ray_order(const Ray* rays, uint32_t n, const float* lo, const 
     a.count = static_cast<uint32_t*>(scratch);
     a.cursor = a.count + kCells * kCellStride;
     a.order = order;
+    a.cells = cells;
     a.trace_counter = trace_counter;
     blocks = std::max(1u, std::min(blocks, (n + kOrderBlock - 1) / kOrderBlock));
     hipLaunchKernelGGL(k_order_count, dim3(blocks), dim3(kOrderBlock), 0, s, a);
-    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(kCells), 0, s, a);
+    hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(64), 0, s, a);
     hipLaunchKernelGGL(k_order_place, dim3(blocks), dim3(kOrderBlock), 0, s, a);
     return hipGetLastError();
 }
