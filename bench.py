@@ -389,11 +389,12 @@ CPU_SHARE = 16  # host threads a one-GPU job may use on the GPU box (its CPU sha
 
 
 def kernel_source_sha():
-    """SHA-256 (16 hex digits) of the traversal kernel's sources: a PMC summary
-    made from other sources no longer describes the kernel that runs."""
+    """SHA-256 (16 hex digits) of the traversal kernel's sources and of the
+    ray order it takes a batch in: a PMC summary made from other sources no
+    longer describes the kernel that runs."""
     import hashlib
     h = hashlib.sha256()
-    for f in ("trace_kernel.hpp", "pt_device.hpp", "pt_layout.hpp", "trace_launch.hip"):
+    for f in ("trace_kernel.hpp", "pt_device.hpp", "pt_layout.hpp", "trace_launch.hip", "ray_order.hip"):
         with open(os.path.join(ROOT, "foundation_amd", "csrc", f), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()[:16]

@@ -30,8 +30,10 @@
  *
  * Threading: a context is single-threaded, like the reference's one render
  * thread (src/Editor/Editor.cpp:20-23).  One context owns one HIP stream for
- * its work, plus a second one that only answers radiance-cache queries during
- * a frame's last bounces (joined before fnd_pt_render returns), and a third
+ * its work, plus a second one that answers radiance-cache queries during a
+ * frame's last bounces (joined before fnd_pt_render returns) and puts a ray
+ * query's rays in spatial order while the previous query's traversal runs
+ * (the traversal waits for it), and a third
  * that runs a frame's radiance-cache training after fnd_pt_render returns,
  * overlapped with the next call until that call first reads the cache;
  * fnd_pt_wait, every fnd_pt_cache_* call and fnd_pt_destroy wait for it.
@@ -279,7 +281,11 @@ fnd_status fnd_pt_buffer_read(fnd_pt_ctx* ctx, fnd_handle buffer, uint64_t offse
  * Host arrays in and out (uploads, traces, downloads). */
 fnd_status fnd_pt_trace(fnd_pt_ctx* ctx, fnd_handle scene, const fnd_ray* rays, uint32_t num_rays,
                         fnd_hit* hits, uint32_t flags);
-/* Device-resident rays and hits (fnd_ray / fnd_hit arrays in buffers). */
+/* Device-resident rays and hits (fnd_ray / fnd_hit arrays in buffers).
+ * Both calls take a batch of 131072 rays or more in the order of its rays'
+ * origin cells on a coarse grid over the scene, for locality; each ray's
+ * record is its own whatever the order.  With FND_ASYNC the caller keeps the
+ * rays unchanged until fnd_pt_wait. */
 fnd_status fnd_pt_trace_buffers(fnd_pt_ctx* ctx, fnd_handle scene, fnd_handle rays, fnd_handle hits,
                                 uint32_t num_rays, uint32_t flags);
 
