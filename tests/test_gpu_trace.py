@@ -205,3 +205,35 @@ def test_spatially_ordered_batches_match_oracle(ctx, instanced):
     camera = R.primary_rays(gen.camera, 512, 288)
     assert camera.shape[0] >= 131072
     _assert_same(ctx.trace(scene, camera), ref.trace(camera), "camera batch")
+
+
+def test_back_to_back_async_ordered_batches(ctx, instanced):
+    """Asynchronous ray queries of 131072 rays or more make their order on a
+    second stream while the previous batch traces, in two slots used in turn:
+    three different batches queued back to back, then one wait, give the same
+    records as each batch traced alone."""
+    from foundation_amd import _abi as A
+    gen, scene, ref = instanced
+    prim = R.primary_rays(gen.camera, 256, 192)
+    ph = ref.trace(prim)
+    sets = []
+    for seed in (1, 2, 3):
+        b, _ = R.bounce_rays(prim, ph, gen.world_triangles(), seed=seed)
+        sets.append(np.ascontiguousarray(np.concatenate([b] * -(-140000 // b.shape[0]))))
+    bufs = []
+    try:
+        for rays in sets:
+            n = rays.shape[0]
+            rb, hb = ctx.buffer_create(n * 32), ctx.buffer_create(n * 16)
+            ctx.buffer_write(rb, rays)
+            bufs.append((rb, hb, n))
+        for rb, hb, n in bufs:
+            ctx.trace_buffers(scene, rb, hb, n, A.ASYNC)
+        ctx.wait()
+        for (rb, hb, n), rays in zip(bufs, sets):
+            got = ctx.buffer_read(hb, np.zeros((n, 4), np.float32))
+            _assert_same(got, ctx.trace(scene, rays), "async ordered batch")
+    finally:
+        for rb, hb, _ in bufs:
+            ctx.buffer_destroy(rb)
+            ctx.buffer_destroy(hb)
