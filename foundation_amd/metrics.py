@@ -16,3 +16,48 @@ def rel_mse(img, ref, eps=REL_MSE_EPS):
     if x.shape != y.shape:
         raise ValueError(f"shape mismatch {x.shape} vs {y.shape}")
     return float(np.mean((x - y) ** 2 / (y * y + eps)))
+
+
+class RelMseEvaluator:
+    """rel_mse(img, ref) of many images against one reference, split over
+    host threads.  The weights 1 / (ref^2 + eps) are formed once; each call
+    evaluates sum((img - ref)^2 * w) in float64 over `threads` contiguous
+    parts of the pixels, each part a dot product, so a 1920x1080 image takes a
+    few milliseconds instead of the ~90 ms of the one-pass form.  The parts'
+    sums are added in a fixed order: the result does not depend on thread
+    timing, and agrees with rel_mse to float64 rounding."""
+
+    def __init__(self, ref, threads=8, eps=REL_MSE_EPS):
+        from concurrent.futures import ThreadPoolExecutor
+        y = np.ascontiguousarray(ref, np.float64).ravel()
+        self.shape = np.shape(ref)
+        self.n = y.size
+        self.w = 1.0 / (y * y + eps)
+        self.y = y
+        threads = max(1, int(threads))
+        step = -(-self.n // threads)
+        self.parts = [(i, min(self.n, i + step)) for i in range(0, self.n, step)]
+        self.pool = ThreadPoolExecutor(max_workers=threads)
+
+    def _part(self, x, lo, hi):
+        d = x[lo:hi].astype(np.float64)
+        d -= self.y[lo:hi]
+        d *= d
+        return float(np.dot(d, self.w[lo:hi]))
+
+    def submit(self, img):
+        """Starts the evaluation of img (not modified until result());
+        returns a handle for result()."""
+        if np.shape(img) != self.shape:
+            raise ValueError(f"shape mismatch {np.shape(img)} vs {self.shape}")
+        x = np.ascontiguousarray(img).ravel()
+        return [self.pool.submit(self._part, x, lo, hi) for lo, hi in self.parts]
+
+    def result(self, handle):
+        return sum(f.result() for f in handle) / self.n
+
+    def __call__(self, img):
+        return self.result(self.submit(img))
+
+    def close(self):
+        self.pool.shutdown(wait=True)

@@ -9,14 +9,14 @@ against checks that do not restate its formulas:
   f is reciprocal, and the albedo of a white (F0 = 1) near-mirror is close to 1;
 * the estimator: the furnace closed form L = Le * sum_{k<=d} a^k exactly,
   and the MIS, BSDF-only and next-event-only estimators agree in expectation;
-* relMSE: closed forms.
+* relMSE: closed forms; the threaded evaluator against the one-pass form.
 """
 import numpy as np
 import pytest
 
 from foundation_amd import _abi as A
 from foundation_amd import pt
-from foundation_amd.metrics import rel_mse
+from foundation_amd.metrics import RelMseEvaluator, rel_mse
 
 LAMBERT, GGX = 0, 1
 
@@ -217,3 +217,24 @@ def test_rel_mse_closed_forms():
     assert rel_mse(np.ones((2, 2)), np.zeros((2, 2))) == pytest.approx(100.0)
     with pytest.raises(ValueError):
         rel_mse(np.ones(3), np.ones(4))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_rel_mse_evaluator_matches_one_pass_form(threads):
+    # the threaded evaluator (the bench's convergence check) against the
+    # one-pass form, on float32 images like the framebuffer reads, a ragged
+    # pixel count (not a multiple of the thread count), and a zero reference
+    rng = np.random.default_rng(1)
+    ref = rng.uniform(0, 3, (37, 53, 3))
+    ref[0, :5] = 0.0
+    ev = RelMseEvaluator(ref, threads=threads)
+    try:
+        assert ev(ref.astype(np.float32)) == pytest.approx(rel_mse(ref.astype(np.float32), ref), rel=1e-12)
+        imgs = [rng.uniform(0, 3, ref.shape).astype(np.float32) for _ in range(3)]
+        handles = [ev.submit(x) for x in imgs]
+        for x, h in zip(imgs, handles):
+            assert ev.result(h) == pytest.approx(rel_mse(x, ref), rel=1e-12)
+        with pytest.raises(ValueError):
+            ev(np.ones((37, 52, 3), np.float32))
+    finally:
+        ev.close()
