@@ -324,7 +324,8 @@ def frames_only(frames, width, height, depth, rr, grid_gradient=0):
     ctx.close()
 
 
-def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, grid_gradient=0, eval_threads=8):
+def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, grid_gradient=0, eval_threads=8,
+                    back_to_back=40):
     """Progressive 1 spp/frame rendering of the displaced-sphere scene, plain and
     with the radiance cache (trained every frame on the frame's own paths),
     until relMSE <= target against an FP64 GPU reference of ref_spp samples.
@@ -332,7 +333,12 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
 
     Each frame's relMSE is evaluated on host threads before the next frame is
     queued (a few ms at 1080p): evaluating it while the next frame renders
-    took host time from the frames' own launches (cache frames 2.95 -> 3.37 ms)."""
+    took host time from the frames' own launches (cache frames 2.95 -> 3.37 ms).
+    A frame queued after the device sat idle runs slower than one queued
+    right behind the previous frame (plain: +0.13 ms after a 2 ms host pause,
+    +0.08 ms after a framebuffer read), so each mode also times
+    `back_to_back` frames rendered one after another, as a render loop
+    queues them (ms_per_frame_back_to_back)."""
     import ctypes as C
     from foundation_amd import _abi as A
     from foundation_amd import pt
@@ -379,7 +385,18 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
             ctx.framebuffer_destroy(fb)
             out[mode] = {"frames": f, "seconds": round(gpu_ms / 1e3, 3), "ms_per_frame": round(gpu_ms / f, 3),
                          "wall_seconds": round(wall, 3), "rel_mse": round(e, 5), "reached": bool(e <= target)}
-            log(f"relMSE {mode}: {e:.4g} after {f} frames, {gpu_ms / 1e3:.2f} s")
+            if back_to_back:
+                fb = ctx.framebuffer_create(width, height)
+                b2b = 0.0
+                for k in range(back_to_back):
+                    ctx.render([pt.make_job(scene, fb, gen.camera, 1, depth, rr_depth=rr, seed=5, sample_offset=k,
+                                            flags=flags)])
+                    b2b += ctx.stats().last_call_ms
+                ctx.framebuffer_destroy(fb)
+                out[mode]["ms_per_frame_back_to_back"] = round(b2b / back_to_back, 3)
+                out[mode]["frames_back_to_back"] = back_to_back
+            log(f"relMSE {mode}: {e:.4g} after {f} frames, {gpu_ms / 1e3:.2f} s; "
+                f"{out[mode].get('ms_per_frame_back_to_back')} ms per frame back to back")
         if out["plain"]["reached"] and out["cache"]["reached"]:
             out["speedup_cache_vs_plain"] = round(out["plain"]["seconds"] / out["cache"]["seconds"], 3)
             out["speedup_cache_vs_plain_wall"] = round(out["plain"]["wall_seconds"] / out["cache"]["wall_seconds"], 3)
