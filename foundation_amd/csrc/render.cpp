@@ -632,7 +632,8 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             wa.b.t_target = cs->s_target;
             wa.b.t_cap = cs->sample_cap;
         }
-        const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);
+        const uint32_t stage_grid = std::max(1u, ctx->num_cus * 8u);  // 256-thread blocks; the other stages
+                                                                       // launch as many threads
         // Launches with fewer rays than coop_rays trace with a group of lanes
         // per ray (trace_coop.hpp): kCoopRaysPerWave per wave of the
         // cooperative grid.  Measured on 1080p frames of the 1.5M-triangle
@@ -715,7 +716,10 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                 const uint32_t W = uint32_t(std::min<uint64_t>(wmax, paths - g0));
                 const uint32_t n0 = std::min(W, slots);
                 const uint32_t wgrid = std::min(stage_grid, (W + 255u) / 256u);
-                const uint32_t sgrid = std::min(stage_grid, (n0 + 255u) / 256u);  // a window uses n0 slots
+                // a window uses n0 slots
+                const uint32_t sgrid = std::min(stage_grid, (n0 + 255u) / 256u);
+                const uint32_t ggrid = std::min(stage_grid * 256u / kGenerateBlock, (n0 + kGenerateBlock - 1u) / kGenerateBlock);
+                const uint32_t hgrid = std::min(stage_grid * 256u / kShadeBlock, (n0 + kShadeBlock - 1u) / kShadeBlock);
                 // A window that fits in the slots has no regeneration: slot i holds
                 // window path i to its end, so its radiance stays in place and the
                 // bounce count bounds the iterations (no polling).
@@ -726,7 +730,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                 uint32_t cur = kCntExtA;
                 launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntFree0), ctx->stream);
                 launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter
-                launch_generate(wa, g0, n0, sgrid, ctx->stream);
+                launch_generate(wa, g0, n0, ggrid, ctx->stream);
                 // the window's first paths: n0 camera rays at most (fewer for tile padding)
                 fnd_status e = trace(cur, "primary trace", n0 < coop_rays + 1024u);
                 if (e != FND_OK) return e;
@@ -742,7 +746,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                     for (uint32_t k = 0; k <= depth; ++k) {
                         const uint32_t nxt = cur ^ 1u, fw = (k & 1u) ^ 1u;  // (the free lists go unused)
                         launch_prep(wa, nxt, fw, ctx->stream);
-                        launch_shade(wa, cur, fw, sgrid, ctx->stream);
+                        launch_shade(wa, cur, fw, hgrid, ctx->stream);
                         if (apply_cache && k == ksplit && k < depth) {
                             launch_snapshot(wa, kCntQuery, kCntQuerySplit, ctx->stream);
                             e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
@@ -776,7 +780,7 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                         const uint32_t nxt = cur ^ 1u;
                         const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
                         launch_prep(wa, nxt, fw, ctx->stream);
-                        launch_shade(wa, cur, fw, sgrid, ctx->stream);
+                        launch_shade(wa, cur, fw, hgrid, ctx->stream);
                         launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
                         e = trace(nxt, "extension + shadow trace");
                         if (e != FND_OK) return e;

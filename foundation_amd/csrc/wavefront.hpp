@@ -82,6 +82,18 @@ enum WfCounter : uint32_t {
 constexpr uint32_t kCntStride = 64;
 __host__ __device__ constexpr uint32_t cnt(uint32_t slot) { return slot * kCntStride; }
 
+// Threads per block of the stages that append to the queues: each block
+// takes one atomic per queue per pass over its share, so larger blocks mean
+// fewer atomics on the hot counter lines, but a block's waves also meet at
+// the append's barriers.  Generation (54 VGPRs, two 1024-thread blocks per
+// CU) took 103 -> 60 us per 1080p frame with 1024 threads.  The shade is
+// not bound by its appends: per 1080p frame it took 565 us in 256-thread
+// blocks, 620 in 512 and 731 in 1024 (one block per CU, its 16 waves in step
+// at the barriers).
+constexpr uint32_t kGenerateBlock = 1024;
+constexpr uint32_t kShadeBlock = 256;
+constexpr uint32_t kStageBlock = 256;  // regeneration, shadow resolve
+
 // Paths are rendered in windows of consecutive global path indices; a window
 // holds at most kWindowSlots times as many paths as there are slots, and a
 // slot whose path ends takes the window's next path (regeneration), so the
