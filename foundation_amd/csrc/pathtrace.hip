@@ -381,17 +381,17 @@ __global__ void k_zero_slots(uint32_t* counters, uint32_t mask)
 // Zeroes the counters one iteration appends to: the next extension queue,
 // the shadow queue, the traversal work counter's shards and the free list
 // this iteration's retired slots go to.
-__global__ void k_prep(uint32_t* counters, uint32_t* trace_ctr, uint32_t next, uint32_t free_zero)
+__global__ void k_prep(uint32_t* counters, uint32_t* trace_ctr, uint32_t next, uint32_t free_zero, uint32_t shadow)
 {
     if (threadIdx.x == 0) {
         counters[cnt(next)] = 0u;
-        counters[cnt(kCntShadow)] = 0u;
+        counters[cnt(shadow)] = 0u;
         counters[cnt(kCntFree0 + free_zero)] = 0u;
     }
     zero_trace_counter(trace_ctr, threadIdx.x);
 }
 
-__global__ __launch_bounds__(kShadeBlock) void k_shade(WfArgs a, uint32_t cur, uint32_t fw)
+__global__ __launch_bounds__(kShadeBlock) void k_shade(WfArgs a, uint32_t cur, uint32_t fw, uint32_t shq)
 {
     const uint32_t n = a.b.counters[cnt(cur)];
     const uint32_t nxt = cur ^ 1u;
@@ -574,7 +574,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WfArgs a, uint32_t cur, u
         const bool freed = a.b.retire && i < n && !cont && !shadow;
         uint32_t slots[4];
         block_append<4, kShadeBlock>({cont, query, shadow, freed},
-                        {&a.b.counters[cnt(nxt)], &a.b.counters[cnt(kCntQuery)], &a.b.counters[cnt(kCntShadow)],
+                        {&a.b.counters[cnt(nxt)], &a.b.counters[cnt(kCntQuery)], &a.b.counters[cnt(shq)],
                          &a.b.counters[cnt(kCntFree0 + fw)]},
                         slots);
         const uint32_t es = slots[0], qs = slots[1], ss = slots[2];
@@ -601,10 +601,22 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WfArgs a, uint32_t cur, u
 
 // One shadow ray per path per bounce, so the adds below never collide.  A
 // path whose last vertex cast this ray is retired after the add, and its slot
-// goes to free list fw.
-__global__ __launch_bounds__(kStageBlock) void k_shadow_resolve(WfArgs a, uint32_t fw)
+// goes to free list fw.  Block 0 also clears the counters the next iteration
+// appends to (zero_ext, zero_sh; kCntCount: none) and then the traversal work
+// counter: the iteration's traces, which read them last, have finished, and
+// this launch reads neither (it reads shadow queue shq).  A window without
+// regeneration saves a launch per bounce this way.
+__global__ __launch_bounds__(kStageBlock) void k_shadow_resolve(WfArgs a, uint32_t fw, uint32_t shq, uint32_t zero_ext,
+                                                                uint32_t zero_sh)
 {
-    const uint32_t n = a.b.counters[cnt(kCntShadow)];
+    const uint32_t n = a.b.counters[cnt(shq)];
+    if (blockIdx.x == 0 && zero_sh < kCntCount) {
+        if (threadIdx.x == 0) {
+            if (zero_ext < kCntCount) a.b.counters[cnt(zero_ext)] = 0u;
+            a.b.counters[cnt(zero_sh)] = 0u;
+        }
+        zero_trace_counter(a.b.trace_ctr, threadIdx.x);
+    }
     if (!a.b.retire) {  // a window without regeneration: paths stay in their slots
         for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
             if (__float_as_uint(a.b.sh_hits[i].y) != 0xFFFFFFFFu) continue;
@@ -825,19 +837,20 @@ void launch_present(const float* accum, uint64_t pixels, float inv, float exposu
                        pitch);
 }
 
-void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, hipStream_t s)
+void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, uint32_t shadow, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, a.b.trace_ctr, next, free_zero);
+    hipLaunchKernelGGL(k_prep, dim3(1), dim3(64), 0, s, a.b.counters, a.b.trace_ctr, next, free_zero, shadow);
 }
 
-void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t grid, hipStream_t s)
+void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t shadow, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, s, a, cur, fw);
+    hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kShadeBlock), 0, s, a, cur, fw, shadow);
 }
 
-void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t grid, hipStream_t s)
+void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t shadow, uint32_t zero_ext, uint32_t zero_shadow,
+                           uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(kStageBlock), 0, s, a, fw);
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(kStageBlock), 0, s, a, fw, shadow, zero_ext, zero_shadow);
 }
 
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)

@@ -688,7 +688,6 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             ta.rays2 = rs->b.sh_rays;
             ta.hits2 = rs->b.sh_hits;
             ta.ray_scene2 = rs->b.sh_scene;
-            ta.nrays2_dev = &rs->b.counters[cnt(kCntShadow)];
             ta.ray_total = rs->b.rays;
             TraceLaunch l = tl;
             l.mode = kModeMixedInternal;
@@ -699,10 +698,11 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
             // kernel returns at once below coop_rays rays, the cooperative one
             // (one wave per ray) at or above it.  The ray count is only known
             // on the device; either way the records are the same.
-            auto trace = [&](uint32_t q, const char* what, bool maybe_few = true) -> fnd_status {
+            auto trace = [&](uint32_t q, uint32_t sh, const char* what, bool maybe_few = true) -> fnd_status {
                 ta.rays = rs->b.ext_rays[q];
                 ta.ray_scene = rs->b.ext_scene[q];
                 ta.nrays_dev = &rs->b.counters[cnt(q)];
+                ta.nrays2_dev = &rs->b.counters[cnt(sh)];
                 const bool dual = maybe_few && coop_rays != 0;
                 ta.coop_below = dual ? coop_rays : 0u;
                 fnd_status e = check_hip(ctx, launch_trace(l, ta, ctx->stream), what);
@@ -729,10 +729,10 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                 wa.b.retire = regen ? 1u : 0u;
                 uint32_t cur = kCntExtA;
                 launch_zero_slots(wa, (1u << kCntExtA) | (1u << kCntQuery) | (1u << kCntFree0), ctx->stream);
-                launch_prep(wa, cur ^ 1u, 1u, ctx->stream);  // empty shadow queue and lists, zero work counter
+                launch_prep(wa, cur ^ 1u, 1u, kCntShadow, ctx->stream);  // empty shadow queue and lists, zero work counter
                 launch_generate(wa, g0, n0, ggrid, ctx->stream);
                 // the window's first paths: n0 camera rays at most (fewer for tile padding)
-                fnd_status e = trace(cur, "primary trace", n0 < coop_rays + 1024u);
+                fnd_status e = trace(cur, kCntShadow, "primary trace", n0 < coop_rays + 1024u);
                 if (e != FND_OK) return e;
                 const uint32_t cg = std::max(1u, std::min<uint32_t>(ctx->num_cus * 4u, (W + 127u) / 128u));
                 bool early = false;  // queries answered on the side stream
@@ -745,8 +745,13 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                     const uint32_t ksplit = std::min(1u, depth);
                     for (uint32_t k = 0; k <= depth; ++k) {
                         const uint32_t nxt = cur ^ 1u, fw = (k & 1u) ^ 1u;  // (the free lists go unused)
-                        launch_prep(wa, nxt, fw, ctx->stream);
-                        launch_shade(wa, cur, fw, hgrid, ctx->stream);
+                        // the bounces' shadow queues alternate between two counters, so the
+                        // shadow resolve of bounce k clears what bounce k + 1 appends to
+                        // (its extension queue is the one bounce k shaded) and no counter
+                        // launch runs between bounces
+                        const uint32_t sh = kCntShadow + (k & 1u);
+                        if (k == 0) launch_prep(wa, nxt, fw, sh, ctx->stream);
+                        launch_shade(wa, cur, fw, sh, hgrid, ctx->stream);
                         if (apply_cache && k == ksplit && k < depth) {
                             launch_snapshot(wa, kCntQuery, kCntQuerySplit, ctx->stream);
                             e = check_hip(ctx, hipEventRecord(rs->side_ev[0], ctx->stream), "side stream");
@@ -760,9 +765,9 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                             early = true;
                         }
                         if (k < depth) {  // the last shade appends no rays
-                            e = trace(nxt, "extension + shadow trace");
+                            e = trace(nxt, sh, "extension + shadow trace");
                             if (e != FND_OK) return e;
-                            launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
+                            launch_shadow_resolve(wa, fw, sh, cur, kCntShadow + ((k + 1u) & 1u), sgrid, ctx->stream);
                         }
                         cur = nxt;
                     }
@@ -779,12 +784,12 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                                              (unsigned long long)max_iters);
                         const uint32_t nxt = cur ^ 1u;
                         const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
-                        launch_prep(wa, nxt, fw, ctx->stream);
-                        launch_shade(wa, cur, fw, hgrid, ctx->stream);
+                        launch_prep(wa, nxt, fw, kCntShadow, ctx->stream);
+                        launch_shade(wa, cur, fw, kCntShadow, hgrid, ctx->stream);
                         launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
-                        e = trace(nxt, "extension + shadow trace");
+                        e = trace(nxt, kCntShadow, "extension + shadow trace");
                         if (e != FND_OK) return e;
-                        launch_shadow_resolve(wa, fw, sgrid, ctx->stream);
+                        launch_shadow_resolve(wa, fw, kCntShadow, kCntCount, kCntCount, sgrid, ctx->stream);
                         const uint32_t ring = uint32_t(k % kPollRing);
                         e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount * kCntStride, rs->b.counters,
                                                           kCntCount * kCntStride * sizeof(uint32_t), hipMemcpyDeviceToHost,
