@@ -331,14 +331,15 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
     until relMSE <= target against an FP64 GPU reference of ref_spp samples.
     Device time of the frames is summed (relMSE evaluation excluded).
 
-    Each frame's relMSE is evaluated on host threads before the next frame is
-    queued (a few ms at 1080p): evaluating it while the next frame renders
-    took host time from the frames' own launches (cache frames 2.95 -> 3.37 ms).
-    A frame queued after the device sat idle runs slower than one queued
-    right behind the previous frame (plain: +0.13 ms after a 2 ms host pause,
-    +0.08 ms after a framebuffer read), so each mode also times
-    `back_to_back` frames rendered one after another, as a render loop
-    queues them (ms_per_frame_back_to_back)."""
+    Each frame's relMSE is evaluated on the device against the FP64
+    reference framebuffer (fnd_pt_framebuffer_rel_mse), so only 8 bytes come
+    back between frames: a frame queued after the device sat idle runs slower
+    than one queued right behind the previous frame (plain: +0.13 ms after a
+    2 ms host pause, +0.08 ms after an image readback).  The last frame of
+    each mode is also read back and evaluated on the host (rel_mse_host), as
+    a check of the device evaluation.  Each mode also times `back_to_back`
+    frames rendered one after another, as a render loop queues them
+    (ms_per_frame_back_to_back)."""
     import ctypes as C
     from foundation_amd import _abi as A
     from foundation_amd import pt
@@ -349,6 +350,7 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
            "width": width, "height": height, "max_depth": depth, "rr_depth": rr, "target": target,
            "cache_grid_gradient": "bf16" if grid_gradient == 0 else "fp32"}
     evaluator = None
+    ref_fb = None
     try:
         ref_fb = ctx.framebuffer_create(width, height, fp64=True)
         t = time.perf_counter()
@@ -357,7 +359,6 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
             ctx.render([pt.make_job(scene, ref_fb, gen.camera, chunk, depth, rr_depth=rr, seed=99,
                                     sample_offset=k * chunk)])
         ref = ctx.framebuffer_read(ref_fb, width, height, fp64=True)[0]
-        ctx.framebuffer_destroy(ref_fb)
         out["reference"] = {"spp": ref_spp, "accumulation": "fp64", "seconds": round(time.perf_counter() - t, 1)}
         log(f"relMSE reference: {ref_spp} spp FP64 in {out['reference']['seconds']} s")
         evaluator = RelMseEvaluator(ref, threads=eval_threads)
@@ -381,10 +382,12 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
                 wall += time.perf_counter() - t0
                 gpu_ms += ctx.stats().last_call_ms
                 f += 1
-                e = evaluator(ctx.framebuffer_read(fb, width, height)[0])
+                e = ctx.framebuffer_rel_mse(fb, ref_fb)
+            e_host = evaluator(ctx.framebuffer_read(fb, width, height)[0])
             ctx.framebuffer_destroy(fb)
             out[mode] = {"frames": f, "seconds": round(gpu_ms / 1e3, 3), "ms_per_frame": round(gpu_ms / f, 3),
-                         "wall_seconds": round(wall, 3), "rel_mse": round(e, 5), "reached": bool(e <= target)}
+                         "wall_seconds": round(wall, 3), "rel_mse": round(e, 5), "reached": bool(e <= target),
+                         "rel_mse_host": round(e_host, 5), "rel_mse_device_host_agree": bool(abs(e - e_host) <= 1e-9 * e_host)}
             if back_to_back:
                 fb = ctx.framebuffer_create(width, height)
                 b2b = 0.0
@@ -403,6 +406,8 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
     finally:
         if evaluator is not None:
             evaluator.close()
+        if ref_fb is not None:
+            ctx.framebuffer_destroy(ref_fb)
         d = A.CacheDesc()
         d.enable = 0
         ctx.lib.fnd_pt_cache_configure(ctx.h, C.byref(d))

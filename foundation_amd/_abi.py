@@ -344,6 +344,7 @@ PROTOTYPES = {
     "fnd_pt_framebuffer_clear": (_S, [_P, _H]),
     "fnd_pt_framebuffer_read": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_read_f64": (_S, [_P, _H, _P, C.POINTER(C.c_uint32)]),
+    "fnd_pt_framebuffer_rel_mse": (_S, [_P, _H, _H, C.c_double, C.POINTER(C.c_double)]),
     "fnd_pt_framebuffer_read_rgba8": (_S, [_P, _H, C.c_float, _P, C.POINTER(C.c_uint32)]),
     "fnd_pt_framebuffer_present": (_S, [_P, _H, C.c_float, _P, C.c_uint64, C.POINTER(C.c_uint32)]),
     "fnd_pt_render": (_S, [_P, C.POINTER(RenderJob), C.c_uint32]),

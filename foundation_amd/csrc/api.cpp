@@ -23,6 +23,7 @@
 #include <thread>
 
 #include "ctx.hpp"
+#include "wavefront.hpp"
 
 using namespace fnd;
 
@@ -363,6 +364,7 @@ void fnd_pt_destroy(fnd_pt_ctx* ctx)
     cache_free(ctx);
     dev_free(ctx, ctx->d_counters, kCounterWords * sizeof(uint32_t));
     dev_free(ctx, ctx->d_present, 255 * sizeof(float));
+    dev_free(ctx, ctx->d_relmse, (kRelMseParts + 1) * sizeof(double));
     dev_free(ctx, ctx->d_stats, 8 * sizeof(unsigned long long));
     dev_free(ctx, ctx->d_spill, ctx->spill_bytes);
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);

@@ -126,6 +126,7 @@ struct fnd_pt_ctx {
     fnd::CtxMap<uint64_t, fnd::Framebuffer*> framebuffers;
     // scratch owned by the context
     float* d_present = nullptr;                // 255 byte thresholds of the RGBA8 transfer
+    double* d_relmse = nullptr;                // kRelMseParts partial sums + the result of fnd_pt_framebuffer_rel_mse
     uint32_t* d_counters = nullptr;            // traversal work counter (kCounterWords)
     unsigned long long* d_stats = nullptr;     // 8 counters
     uint2* d_spill = nullptr;

@@ -829,6 +829,51 @@ __global__ __launch_bounds__(256) void k_present(const float* accum, uint64_t pi
     }
 }
 
+// relMSE of a framebuffer against an FP64 reference: each block sums its
+// strided share in double, the shares are added in block order by one block
+// (a tree over 256 threads), so the result does not depend on timing.
+__global__ __launch_bounds__(256) void k_rel_mse_part(const float* x, float xinv, const double* y, double yinv,
+                                                      uint64_t n, double eps, double* part)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (uint64_t i = uint64_t(blockIdx.x) * 256u + threadIdx.x; i < n; i += uint64_t(gridDim.x) * 256u) {
+        const double xv = double(x[i] * xinv);  // the FP32 mean, as the FP32 readback forms it
+        const double yv = y[i] * yinv;
+        const double d = xv - yv;
+        s += d * d / (yv * yv + eps);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_rel_mse_sum(const double* part, uint32_t parts, double inv_n, double* out)
+{
+    __shared__ double red[256];
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < parts; i += 256u) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0] * inv_n;
+}
+
+void launch_rel_mse(const float* x, float xinv, const double* y, double yinv, uint64_t n, double eps, double* part,
+                    double* out, hipStream_t s)
+{
+    const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(kRelMseParts, (n + 255) / 256)));
+    hipLaunchKernelGGL(k_rel_mse_part, dim3(grid), dim3(256), 0, s, x, xinv, y, yinv, n, eps, part);
+    hipLaunchKernelGGL(k_rel_mse_sum, dim3(1), dim3(256), 0, s, part, grid, n ? 1.0 / double(n) : 0.0, out);
+}
+
 void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
                     uint32_t* out, uint32_t width, uint32_t pitch, hipStream_t s)
 {

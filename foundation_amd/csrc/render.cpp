@@ -487,6 +487,35 @@ fnd_status fnd_pt_framebuffer_read_f64(fnd_pt_ctx* ctx, fnd_handle h, double* rg
     });
 }
 
+fnd_status fnd_pt_framebuffer_rel_mse(fnd_pt_ctx* ctx, fnd_handle h, fnd_handle r, double eps, double* out)
+{
+    return guard(ctx, [&]() -> fnd_status {
+        if (!ctx || !out) return FND_E_INVALID_ARG;
+        Framebuffer* f = find_framebuffer(ctx, h);
+        Framebuffer* g = find_framebuffer(ctx, r);
+        if (!f || !g) return set_error(ctx, FND_E_NOT_FOUND, "unknown framebuffer handle");
+        if (!g->accum64) return set_error(ctx, FND_E_INVALID_ARG, "reference framebuffer was created without FND_FB_FP64");
+        if (f->width != g->width || f->height != g->height)
+            return set_error(ctx, FND_E_INVALID_ARG, "framebuffers differ in size (%ux%u against %ux%u)", f->width,
+                             f->height, g->width, g->height);
+        if (!(eps > 0.0)) return set_error(ctx, FND_E_INVALID_ARG, "eps must be positive");
+        if (!ctx->d_relmse) {
+            void* p = nullptr;
+            fnd_status st = dev_alloc(ctx, &p, (kRelMseParts + 1) * sizeof(double));
+            if (st != FND_OK) return st;
+            ctx->d_relmse = static_cast<double*>(p);
+        }
+        const uint64_t n = uint64_t(f->width) * f->height * 3;
+        const float xinv = f->samples ? 1.0f / float(f->samples) : 1.0f;
+        const double yinv = g->samples ? 1.0 / double(g->samples) : 1.0;
+        launch_rel_mse(f->accum, xinv, g->accum64, yinv, n, eps, ctx->d_relmse, ctx->d_relmse + kRelMseParts,
+                       ctx->stream);
+        fnd_status st = check_hip(ctx, hipGetLastError(), "relMSE");
+        if (st == FND_OK) st = copy_sync(ctx, out, ctx->d_relmse + kRelMseParts, sizeof(double), hipMemcpyDeviceToHost, "relMSE");
+        return st;
+    });
+}
+
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs)
 {
     return guard(ctx, [&]() -> fnd_status {

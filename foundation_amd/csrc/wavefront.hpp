@@ -159,6 +159,11 @@ struct WfArgs {
 void launch_generate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_regen(const WfArgs& a, uint64_t g0, uint32_t fidx, uint32_t next, uint32_t grid, hipStream_t s);
 // out[y * pitch + x] for pixel y * width + x (pitch in pixels, >= width)
+// relMSE of x (FP32 sums, times xinv) against y (FP64 sums, times yinv) over
+// n values into out[0], with kRelMseParts partial sums in part.
+constexpr uint32_t kRelMseParts = 1024;
+void launch_rel_mse(const float* x, float xinv, const double* y, double yinv, uint64_t n, double eps, double* part,
+                    double* out, hipStream_t s);
 void launch_present(const float* accum, uint64_t pixels, float inv, float exposure, const float* thresholds,
                     uint32_t* out, uint32_t width, uint32_t pitch, hipStream_t s);
 void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, uint32_t shadow, hipStream_t s);

@@ -478,6 +478,13 @@ class Context:
         self._ck(fn(self.h, fb, out.ctypes.data, C.byref(n)))
         return out, n.value
 
+    def framebuffer_rel_mse(self, fb, ref, eps=0.01):
+        """relMSE of fb's mean radiance against the FP64 reference framebuffer
+        ref, evaluated on the device (fnd_pt_framebuffer_rel_mse)."""
+        out = C.c_double()
+        self._ck(self.lib.fnd_pt_framebuffer_rel_mse(self.h, fb, ref, eps, C.byref(out)))
+        return out.value
+
     def framebuffer_read_rgba8(self, fb, width, height, exposure=1.0):
         """R8G8B8A8 bytes of the mean radiance, converted on the device."""
         out = np.zeros((height, width, 4), np.uint8)

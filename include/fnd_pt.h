@@ -331,6 +331,17 @@ fnd_status fnd_pt_framebuffer_read_rgba8(fnd_pt_ctx* ctx, fnd_handle fb, float e
  * not written.  Returns after the device has finished. */
 fnd_status fnd_pt_framebuffer_present(fnd_pt_ctx* ctx, fnd_handle fb, float exposure, void* device_rgba,
                                       uint64_t row_pitch, uint32_t* out_samples);
+/* Relative mean squared error of framebuffer fb's mean radiance against
+ * reference ref's, evaluated on the device: the mean over pixels and
+ * channels of (x - y)^2 / (y^2 + eps) in double precision, with x as
+ * fnd_pt_framebuffer_read forms it (FP32 sum times 1/samples in FP32) and y
+ * as fnd_pt_framebuffer_read_f64 forms it (ref needs FND_FB_FP64).  For
+ * progressive renders that check convergence after every frame: only the
+ * 8-byte result comes back, so the device does not wait for an image
+ * readback between frames.  No counterpart in the reference (it has no
+ * progressive renderer).  FND_E_INVALID_ARG if the sizes differ, ref has no
+ * FP64 sums, or eps is not positive.  Returns after the device has finished. */
+fnd_status fnd_pt_framebuffer_rel_mse(fnd_pt_ctx* ctx, fnd_handle fb, fnd_handle ref, double eps, double* out);
 /* Renders all jobs in one wavefront launch sequence: their paths form one path
  * space, processed in windows (see fnd_tuning.queue_capacity). */
 fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t num_jobs);

@@ -210,3 +210,35 @@ def test_render_rejects_bad_jobs(ctx, cornell):
             ctx.render([pt.make_job(s, fb, bad, 1, 4)])
     finally:
         ctx.framebuffer_destroy(fb)
+
+
+def test_device_rel_mse_matches_the_host_form(ctx, cornell):
+    """fnd_pt_framebuffer_rel_mse against the host evaluation of the same two
+    readbacks (float64, one pass), on progressive frames of a Cornell box
+    against an FP64 reference of other samples; an empty framebuffer (sums 0)
+    gives the mean of y^2 / (y^2 + eps); mismatched sizes, a reference without
+    FP64 sums and eps <= 0 are refused."""
+    g, s = cornell
+    w, h = 40, 30
+    ref = ctx.framebuffer_create(w, h, fp64=True)
+    fb = ctx.framebuffer_create(w, h)
+    other = ctx.framebuffer_create(w, h + 1, fp64=True)
+    try:
+        ctx.render([pt.make_job(s, ref, g.camera, 64, 4, seed=9)])
+        y = ctx.framebuffer_read(ref, w, h, fp64=True)[0]
+        e0 = ctx.framebuffer_rel_mse(fb, ref)
+        np.testing.assert_allclose(e0, np.mean(y * y / (y * y + 0.01)), rtol=1e-12)
+        for f in range(3):
+            ctx.render([pt.make_job(s, fb, g.camera, 1, 4, seed=2, sample_offset=f)])
+            x = ctx.framebuffer_read(fb, w, h)[0]
+            for eps in (0.01, 0.5):
+                host = float(np.mean((x.astype(np.float64) - y) ** 2 / (y * y + eps)))
+                np.testing.assert_allclose(ctx.framebuffer_rel_mse(fb, ref, eps), host, rtol=1e-12)
+        for bad in ((fb, other), (fb, fb), (ref, fb)):
+            with pytest.raises(RuntimeError):
+                ctx.framebuffer_rel_mse(*bad)
+        with pytest.raises(RuntimeError):
+            ctx.framebuffer_rel_mse(fb, ref, 0.0)
+    finally:
+        for f in (ref, fb, other):
+            ctx.framebuffer_destroy(f)
