@@ -400,9 +400,10 @@ def time_to_rel_mse(ctx, width, height, depth, rr, target, ref_spp, max_frames, 
                 out[mode]["frames_back_to_back"] = back_to_back
             log(f"relMSE {mode}: {e:.4g} after {f} frames, {gpu_ms / 1e3:.2f} s; "
                 f"{out[mode].get('ms_per_frame_back_to_back')} ms per frame back to back")
-        if out["plain"]["reached"] and out["cache"]["reached"]:
+        if out["plain"]["reached"] and out["cache"]["reached"] and out["cache"]["seconds"] > 0:
             out["speedup_cache_vs_plain"] = round(out["plain"]["seconds"] / out["cache"]["seconds"], 3)
-            out["speedup_cache_vs_plain_wall"] = round(out["plain"]["wall_seconds"] / out["cache"]["wall_seconds"], 3)
+            if out["cache"]["wall_seconds"] > 0:
+                out["speedup_cache_vs_plain_wall"] = round(out["plain"]["wall_seconds"] / out["cache"]["wall_seconds"], 3)
     finally:
         if evaluator is not None:
             evaluator.close()
