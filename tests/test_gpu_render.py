@@ -212,14 +212,15 @@ def test_render_rejects_bad_jobs(ctx, cornell):
         ctx.framebuffer_destroy(fb)
 
 
-def test_device_rel_mse_matches_the_host_form(ctx, cornell):
+@pytest.mark.parametrize("w,h", [(40, 30), (640, 480)])
+def test_device_rel_mse_matches_the_host_form(ctx, cornell, w, h):
     """fnd_pt_framebuffer_rel_mse against the host evaluation of the same two
     readbacks (float64, one pass), on progressive frames of a Cornell box
     against an FP64 reference of other samples; an empty framebuffer (sums 0)
     gives the mean of y^2 / (y^2 + eps); mismatched sizes, a reference without
-    FP64 sums and eps <= 0 are refused."""
+    FP64 sums and eps <= 0 are refused.  640 x 480 x 3 values fill the
+    reduction's 1024 partial sums with several values per thread."""
     g, s = cornell
-    w, h = 40, 30
     ref = ctx.framebuffer_create(w, h, fp64=True)
     fb = ctx.framebuffer_create(w, h)
     other = ctx.framebuffer_create(w, h + 1, fp64=True)
