@@ -602,17 +602,19 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(WfArgs a, uint32_t cur, u
 // One shadow ray per path per bounce, so the adds below never collide.  A
 // path whose last vertex cast this ray is retired after the add, and its slot
 // goes to free list fw.  Block 0 also clears the counters the next iteration
-// appends to (zero_ext, zero_sh; kCntCount: none) and then the traversal work
-// counter: the iteration's traces, which read them last, have finished, and
-// this launch reads neither (it reads shadow queue shq).  A window without
-// regeneration saves a launch per bounce this way.
+// appends to (zero_ext, zero_sh, zero_free; kCntCount: none) and then the
+// traversal work counter: the iteration's shade, regeneration and traces,
+// which read them last, have finished, and this launch reads none of them (it
+// reads shadow queue shq and appends to free list fw).  No counter launch runs
+// between iterations this way.
 __global__ __launch_bounds__(kStageBlock) void k_shadow_resolve(WfArgs a, uint32_t fw, uint32_t shq, uint32_t zero_ext,
-                                                                uint32_t zero_sh)
+                                                                uint32_t zero_sh, uint32_t zero_free)
 {
     const uint32_t n = a.b.counters[cnt(shq)];
     if (blockIdx.x == 0 && zero_sh < kCntCount) {
         if (threadIdx.x == 0) {
             if (zero_ext < kCntCount) a.b.counters[cnt(zero_ext)] = 0u;
+            if (zero_free < kCntCount) a.b.counters[cnt(zero_free)] = 0u;
             a.b.counters[cnt(zero_sh)] = 0u;
         }
         zero_trace_counter(a.b.trace_ctr, threadIdx.x);
@@ -893,9 +895,10 @@ void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t shadow, u
 }
 
 void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t shadow, uint32_t zero_ext, uint32_t zero_shadow,
-                           uint32_t grid, hipStream_t s)
+                           uint32_t zero_free, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(kStageBlock), 0, s, a, fw, shadow, zero_ext, zero_shadow);
+    hipLaunchKernelGGL(k_shadow_resolve, dim3(grid), dim3(kStageBlock), 0, s, a, fw, shadow, zero_ext, zero_shadow,
+                       zero_free);
 }
 
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s)

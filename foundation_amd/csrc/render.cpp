@@ -796,7 +796,8 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                         if (k < depth) {  // the last shade appends no rays
                             e = trace(nxt, sh, "extension + shadow trace");
                             if (e != FND_OK) return e;
-                            launch_shadow_resolve(wa, fw, sh, cur, kCntShadow + ((k + 1u) & 1u), sgrid, ctx->stream);
+                            launch_shadow_resolve(wa, fw, sh, cur, kCntShadow + ((k + 1u) & 1u), kCntCount, sgrid,
+                                                  ctx->stream);
                         }
                         cur = nxt;
                     }
@@ -813,12 +814,17 @@ fnd_status fnd_pt_render(fnd_pt_ctx* ctx, const fnd_render_job* jobs, uint32_t n
                                              (unsigned long long)max_iters);
                         const uint32_t nxt = cur ^ 1u;
                         const uint32_t fr = uint32_t(k & 1u), fw = fr ^ 1u;  // free lists read / written
-                        launch_prep(wa, nxt, fw, kCntShadow, ctx->stream);
-                        launch_shade(wa, cur, fw, kCntShadow, hgrid, ctx->stream);
+                        // as above, the shadow resolve of iteration k clears what iteration
+                        // k + 1 appends to: the extension queue iteration k shaded, the other
+                        // shadow counter and the free list iteration k's regeneration read
+                        const uint32_t sh = kCntShadow + uint32_t(k & 1u);
+                        if (k == 0) launch_prep(wa, nxt, fw, sh, ctx->stream);
+                        launch_shade(wa, cur, fw, sh, hgrid, ctx->stream);
                         launch_regen(wa, g0, fr, nxt, sgrid, ctx->stream);
-                        e = trace(nxt, kCntShadow, "extension + shadow trace");
+                        e = trace(nxt, sh, "extension + shadow trace");
                         if (e != FND_OK) return e;
-                        launch_shadow_resolve(wa, fw, kCntShadow, kCntCount, kCntCount, sgrid, ctx->stream);
+                        launch_shadow_resolve(wa, fw, sh, cur, kCntShadow + uint32_t((k + 1u) & 1u), kCntFree0 + fr, sgrid,
+                                              ctx->stream);
                         const uint32_t ring = uint32_t(k % kPollRing);
                         e = check_hip(ctx, hipMemcpyAsync(rs->poll_host + ring * kCntCount * kCntStride, rs->b.counters,
                                                           kCntCount * kCntStride * sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -65,7 +65,7 @@ struct ShadeScene {
 enum WfCounter : uint32_t {
     kCntExtA = 0,
     kCntExtB = 1,
-    kCntShadow = 2,  // shadow queue; 2 and 3 used alternately by the bounces of a window without regeneration
+    kCntShadow = 2,  // shadow queue; 2 and 3 used alternately by the iterations of a window
     // 4: unused (the traversal's work counter is WfBuffers::trace_ctr)
     kCntQuery = 5,
     kCntTrain = 6,
@@ -168,10 +168,11 @@ void launch_present(const float* accum, uint64_t pixels, float inv, float exposu
                     uint32_t* out, uint32_t width, uint32_t pitch, hipStream_t s);
 void launch_prep(const WfArgs& a, uint32_t next, uint32_t free_zero, uint32_t shadow, hipStream_t s);
 void launch_shade(const WfArgs& a, uint32_t cur, uint32_t fw, uint32_t shadow, uint32_t grid, hipStream_t s);
-// zero_ext, zero_shadow: counter slots the launch clears for the next
-// iteration (kCntCount: none), and with them the traversal's work counter.
+// zero_ext, zero_shadow, zero_free: counter slots the launch clears for the
+// next iteration (kCntCount: none), and with them the traversal's work
+// counter (when zero_shadow is a slot).
 void launch_shadow_resolve(const WfArgs& a, uint32_t fw, uint32_t shadow, uint32_t zero_ext, uint32_t zero_shadow,
-                           uint32_t grid, hipStream_t s);
+                           uint32_t zero_free, uint32_t grid, hipStream_t s);
 void launch_accumulate(const WfArgs& a, uint64_t g0, uint32_t n, uint32_t grid, hipStream_t s);
 void launch_cache_apply(const WfArgs& a, uint32_t grid, hipStream_t s);
 void launch_zero_slots(const WfArgs& a, uint32_t mask, hipStream_t s);
